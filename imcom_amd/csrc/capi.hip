@@ -1,0 +1,577 @@
+// C ABI of the solve path (declared in include/imcom_gpu.h).  Each entry
+// point stands in for one procedure of the reference's imcom_matrix /
+// imcom_bisect modules; the context owns the device copies of the tile's
+// inputs and intermediates.
+#include "../../include/imcom_gpu.h"
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace imcom;
+
+namespace {
+
+constexpr int kNB = 32;    // tridiagonal-reduction panel width
+constexpr int kNBB = 128;  // back-transform block
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      cap = 0;
+      if (e != hipSuccess) return e;
+    }
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace
+
+struct imcom_gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int64_t n = 0, m = 0;
+  double C_a = 0.0;
+  std::vector<int32_t> h_exp;
+  // tables
+  int psfconst = 1, nexppsf = 1, n1 = 0, n2 = 0;
+  bool have_tables = false;
+  // stage flags
+  bool have_tile = false, have_A = false, have_B = false, have_QL = false, have_P = false,
+       have_K = false, have_T = false;
+  // inputs
+  DevBuf x, y, ex, Xa, Ya, Nd, Atab, Btab, I;
+  // stages
+  DevBuf A, B, Q, L, P, K, US, T, W, H, S, U, evals, bracket;
+  // eigensolver workspace
+  DevBuf sW, sVW, sWV, sy, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb, sTb,
+      sY, sY2, slam, sZ, stile;
+  DcWork* dc = nullptr;
+  DevBuf split_work;
+};
+
+static int set_err(imcom_gpu_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+static int hip_err(imcom_gpu_ctx* c, hipError_t e, const char* where) {
+  if (e == hipSuccess) return IMCOM_OK;
+  const int code = (e == hipErrorOutOfMemory) ? IMCOM_ENOMEM : IMCOM_EHIP;
+  return set_err(c, code, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define CK(expr, where)                                   \
+  do {                                                    \
+    hipError_t _e = (expr);                               \
+    if (_e != hipSuccess) return hip_err(ctx, _e, where); \
+  } while (0)
+
+#define NEED(cond, code, msg) \
+  do {                        \
+    if (!(cond)) return set_err(ctx, code, msg); \
+  } while (0)
+
+static hipError_t upload(imcom_gpu_ctx* ctx, DevBuf& b, const void* h, size_t bytes) {
+  hipError_t e = b.reserve(bytes);
+  if (e != hipSuccess) return e;
+  if (bytes == 0) return hipSuccess;
+  return hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, ctx->stream);
+}
+
+static hipError_t download(imcom_gpu_ctx* ctx, void* h, const DevBuf& b, size_t bytes) {
+  if (!h || bytes == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(h, b.p, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->stream);
+}
+
+// Q(:, j) = Z(:, n-1-j), L(j) = lam(n-1-j): ascending -> descending
+__global__ void reverse_cols_kernel(int64_t n, const double* __restrict__ Z, double* __restrict__ Q,
+                                    const double* __restrict__ lam, double* __restrict__ L) {
+  const int64_t j = blockIdx.y;
+  const int64_t src = n - 1 - j;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    Q[i + j * n] = Z[i + src * n];
+  if (blockIdx.x == 0 && threadIdx.x == 0) L[j] = lam[src];
+}
+
+namespace imcom {
+hipError_t launch_reverse_cols(hipStream_t st, int64_t n, const double* Zin, double* Qout,
+                               const double* lam_in, double* L_out) {
+  dim3 grid((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)n);
+  hipLaunchKernelGGL(reverse_cols_kernel, grid, dim3(256), 0, st, n, Zin, Qout, lam_in, L_out);
+  return hipGetLastError();
+}
+}  // namespace imcom
+
+extern "C" {
+
+int imcom_gpu_init(imcom_gpu_ctx** out, int device) {
+  if (!out) return IMCOM_EARG;
+  *out = nullptr;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev <= 0) return IMCOM_EHIP;
+  if (device < 0 || device >= ndev) return IMCOM_EARG;
+  e = hipSetDevice(device);
+  if (e != hipSuccess) return IMCOM_EHIP;
+  imcom_gpu_ctx* ctx = new imcom_gpu_ctx();
+  ctx->device = device;
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return IMCOM_EHIP;
+  }
+  ctx->dc = dc_create();
+  *out = ctx;
+  return IMCOM_OK;
+}
+
+void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  DevBuf* bufs[] = {&ctx->x,     &ctx->y,    &ctx->ex,     &ctx->Xa,    &ctx->Ya,   &ctx->Nd,
+                    &ctx->Atab,  &ctx->Btab, &ctx->I,      &ctx->A,     &ctx->B,    &ctx->Q,
+                    &ctx->L,     &ctx->P,    &ctx->K,      &ctx->US,    &ctx->T,    &ctx->W,
+                    &ctx->H,     &ctx->S,    &ctx->U,      &ctx->evals, &ctx->bracket,
+                    &ctx->sW,    &ctx->sVW,  &ctx->sWV,    &ctx->sy,    &ctx->swpre, &ctx->svtmp,
+                    &ctx->sd,    &ctx->se,   &ctx->stau,   &ctx->spart, &ctx->spart2, &ctx->ss,
+                    &ctx->sscal, &ctx->sVb,  &ctx->sSb,    &ctx->sTb,   &ctx->sY,   &ctx->sY2,
+                    &ctx->slam,  &ctx->sZ,   &ctx->stile,  &ctx->split_work};
+  for (DevBuf* b : bufs) b->release();
+  dc_destroy(ctx->dc);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* imcom_gpu_last_error(const imcom_gpu_ctx* ctx) {
+  return ctx ? ctx->err.c_str() : "no context";
+}
+
+int imcom_gpu_synchronize(imcom_gpu_ctx* ctx) {
+  if (!ctx) return IMCOM_EARG;
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_set_tile(imcom_gpu_ctx* ctx, int64_t n, int64_t m, const double* x_i,
+                       const double* y_i, const int32_t* exp_i, const double* X_a,
+                       const double* Y_a, const double* Ndiag, double C_a) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(n >= 1 && m >= 1, IMCOM_EARG, "set_tile: n and m must be >= 1");
+  NEED(n <= 32768, IMCOM_EARG, "set_tile: n > 32768 is not supported");
+  NEED(x_i && y_i && exp_i && X_a && Y_a && Ndiag, IMCOM_EARG, "set_tile: NULL input");
+  NEED(C_a > 0.0, IMCOM_EARG, "set_tile: C_a must be positive");
+  for (int64_t i = 0; i < n; ++i) {
+    NEED(exp_i[i] >= 1, IMCOM_EARG, "set_tile: exposure ids are 1-based");
+    NEED(i == 0 || exp_i[i] >= exp_i[i - 1], IMCOM_EARG,
+         "set_tile: pixels must be exposure-major (exp_i non-decreasing)");
+  }
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  ctx->n = n;
+  ctx->m = m;
+  ctx->C_a = C_a;
+  ctx->h_exp.assign(exp_i, exp_i + n);
+  CK(upload(ctx, ctx->x, x_i, n * 8), "upload x_i");
+  CK(upload(ctx, ctx->y, y_i, n * 8), "upload y_i");
+  CK(upload(ctx, ctx->ex, exp_i, n * 4), "upload exp_i");
+  CK(upload(ctx, ctx->Xa, X_a, m * 8), "upload X_a");
+  CK(upload(ctx, ctx->Ya, Y_a, m * 8), "upload Y_a");
+  CK(upload(ctx, ctx->Nd, Ndiag, n * 8), "upload Ndiag");
+  CK(hipStreamSynchronize(ctx->stream), "set_tile sync");
+  ctx->have_tile = true;
+  ctx->have_A = ctx->have_B = ctx->have_QL = ctx->have_P = ctx->have_K = ctx->have_T = false;
+  return IMCOM_OK;
+}
+
+int imcom_gpu_set_tables(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1, int n2,
+                         const double* Alookup, const double* Blookup) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(psfconst == 0 || psfconst == 1, IMCOM_EARG, "set_tables: PSFCONST must be 1 or 0");
+  NEED(nexppsf >= 1 && n1 >= 1 && n2 >= 1, IMCOM_EARG, "set_tables: bad sizes");
+  NEED(psfconst == 0 || nexppsf == 1, IMCOM_EARG, "set_tables: psfconst=1 needs nexppsf=1");
+  NEED(Alookup || Blookup, IMCOM_EARG, "set_tables: NULL tables");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  const size_t ts = (size_t)n1 * n2 * 8;
+  const int nA = nexppsf * (nexppsf + 1) / 2;
+  if (Alookup) CK(upload(ctx, ctx->Atab, Alookup, ts * nA), "upload Alookup");
+  if (Blookup) CK(upload(ctx, ctx->Btab, Blookup, ts * nexppsf), "upload Blookup");
+  CK(hipStreamSynchronize(ctx->stream), "set_tables sync");
+  ctx->psfconst = psfconst;
+  ctx->nexppsf = nexppsf;
+  ctx->n1 = n1;
+  ctx->n2 = n2;
+  ctx->have_tables = true;
+  return IMCOM_OK;
+}
+
+static int check_interp(imcom_gpu_ctx* ctx, int npoly, int npad, double sx, double sy) {
+  NEED(ctx->have_tile, IMCOM_ESTATE, "tile inputs not set (imcom_gpu_set_tile)");
+  NEED(ctx->have_tables, IMCOM_ESTATE, "lookup tables not set (imcom_gpu_set_tables)");
+  NEED(npoly >= 1 && npoly <= 8, IMCOM_EARG, "npoly must be in 1..7 (8 is taken as 7)");
+  NEED(npad >= 1, IMCOM_EARG, "npad must be >= 1");
+  NEED(sx > 0.0 && sy > 0.0, IMCOM_EARG, "PSF pixel scales must be positive");
+  const int ne = ctx->psfconst == 1 ? 1 : ctx->nexppsf;
+  NEED(ctx->h_exp.back() <= ne || ctx->psfconst == 1, IMCOM_EARG,
+       "exposure id exceeds the number of PSF tables");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale,
+                       double psfyscale, double* A_out) {
+  if (!ctx) return IMCOM_EARG;
+  int rc = check_interp(ctx, npoly, npad, psfxscale, psfyscale);
+  if (rc) return rc;
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  NEED(ctx->Atab.p, IMCOM_ESTATE, "A lookup table not set");
+  const int64_t n = ctx->n;
+  CK(ctx->A.reserve((size_t)n * n * 8), "alloc A");
+  int np = npoly % 2 == 0 ? npoly - 1 : npoly;
+  CK(launch_lookup_A(ctx->stream, ctx->device, n, ctx->x.as<double>(), ctx->y.as<double>(),
+                     ctx->ex.as<int32_t>(), ctx->psfconst, ctx->Atab.as<double>(),
+                     (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
+                     psfyscale, ctx->A.as<double>()),
+     "lookup_A");
+  ctx->have_A = true;
+  CK(download(ctx, A_out, ctx->A, (size_t)n * n * 8), "download A");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_set_A(imcom_gpu_ctx* ctx, const double* A) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_tile && A, IMCOM_EARG, "set_A: tile not set or NULL A");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(upload(ctx, ctx->A, A, (size_t)ctx->n * ctx->n * 8), "upload A");
+  CK(hipStreamSynchronize(ctx->stream), "set_A sync");
+  ctx->have_A = true;
+  return IMCOM_OK;
+}
+
+int imcom_gpu_lookup_B(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale,
+                       double psfyscale, double* B_out) {
+  if (!ctx) return IMCOM_EARG;
+  int rc = check_interp(ctx, npoly, npad, psfxscale, psfyscale);
+  if (rc) return rc;
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  NEED(ctx->Btab.p, IMCOM_ESTATE, "B lookup table not set");
+  const int64_t n = ctx->n, m = ctx->m;
+  CK(ctx->B.reserve((size_t)n * m * 8), "alloc B");
+  int np = npoly % 2 == 0 ? npoly - 1 : npoly;
+  CK(launch_lookup_B(ctx->stream, ctx->device, n, m, ctx->x.as<double>(), ctx->y.as<double>(),
+                     ctx->ex.as<int32_t>(), ctx->Xa.as<double>(), ctx->Ya.as<double>(),
+                     ctx->psfconst, ctx->Btab.as<double>(), (int64_t)ctx->n1 * ctx->n2, ctx->n1,
+                     ctx->n2, np / 2, npad, psfxscale, psfyscale, ctx->B.as<double>()),
+     "lookup_B");
+  ctx->have_B = true;
+  CK(download(ctx, B_out, ctx->B, (size_t)n * m * 8), "download B");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_set_B(imcom_gpu_ctx* ctx, const double* B) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_tile && B, IMCOM_EARG, "set_B: tile not set or NULL B");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(upload(ctx, ctx->B, B, (size_t)ctx->n * ctx->m * 8), "upload B");
+  CK(hipStreamSynchronize(ctx->stream), "set_B sync");
+  ctx->have_B = true;
+  return IMCOM_OK;
+}
+
+static hipError_t reserve_eigen(imcom_gpu_ctx* ctx, int64_t n, SyTile& t) {
+  const size_t d = 8;
+  IMCOM_HIP_TRY(ctx->sW.reserve(n * kNB * d));
+  IMCOM_HIP_TRY(ctx->sVW.reserve(n * 2 * kNB * d));
+  IMCOM_HIP_TRY(ctx->sWV.reserve(n * 2 * kNB * d));
+  IMCOM_HIP_TRY(ctx->sy.reserve(n * d));
+  IMCOM_HIP_TRY(ctx->swpre.reserve(n * d));
+  IMCOM_HIP_TRY(ctx->svtmp.reserve(n * d));
+  IMCOM_HIP_TRY(ctx->sd.reserve(n * d));
+  IMCOM_HIP_TRY(ctx->se.reserve(n * d));
+  IMCOM_HIP_TRY(ctx->stau.reserve(n * d));
+  IMCOM_HIP_TRY(ctx->spart.reserve((n / 256 + 2) * d));
+  IMCOM_HIP_TRY(ctx->spart2.reserve((n / 256 + 2) * d));
+  IMCOM_HIP_TRY(ctx->ss.reserve(2 * kNB * d));
+  IMCOM_HIP_TRY(ctx->sscal.reserve(d));
+  IMCOM_HIP_TRY(ctx->sVb.reserve((size_t)std::max<int64_t>(n - 1, 1) * kNBB * d));
+  IMCOM_HIP_TRY(ctx->sSb.reserve(kNBB * kNBB * d));
+  IMCOM_HIP_TRY(ctx->sTb.reserve(kNBB * kNBB * d));
+  IMCOM_HIP_TRY(ctx->sY.reserve(kNBB * n * d));
+  IMCOM_HIP_TRY(ctx->sY2.reserve(kNBB * n * d));
+  IMCOM_HIP_TRY(ctx->slam.reserve(n * d));
+  IMCOM_HIP_TRY(ctx->sZ.reserve((size_t)n * n * d));
+  IMCOM_HIP_TRY(ctx->stile.reserve(sizeof(SyTile)));
+  IMCOM_HIP_TRY(ctx->Q.reserve((size_t)n * n * d));
+  IMCOM_HIP_TRY(ctx->L.reserve(n * d));
+  t.n = n;
+  t.A = ctx->A.as<double>();
+  t.W = ctx->sW.as<double>();
+  t.VW = ctx->sVW.as<double>();
+  t.WV = ctx->sWV.as<double>();
+  t.y = ctx->sy.as<double>();
+  t.wpre = ctx->swpre.as<double>();
+  t.vtmp = ctx->svtmp.as<double>();
+  t.d = ctx->sd.as<double>();
+  t.e = ctx->se.as<double>();
+  t.tau = ctx->stau.as<double>();
+  t.part = ctx->spart.as<double>();
+  t.part2 = ctx->spart2.as<double>();
+  t.s = ctx->ss.as<double>();
+  t.scal = ctx->sscal.as<double>();
+  t.Vb = ctx->sVb.as<double>();
+  t.Sb = ctx->sSb.as<double>();
+  t.Tb = ctx->sTb.as<double>();
+  t.Y = ctx->sY.as<double>();
+  t.Y2 = ctx->sY2.as<double>();
+  t.Z = ctx->Q.as<double>();
+  return hipSuccess;
+}
+
+int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* condition) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_A, IMCOM_ESTATE, "eigen_A: A not built (lookup_A or set_A first)");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int64_t n = ctx->n;
+  SyTile t;
+  CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
+  CK(hipMemcpyAsync(ctx->stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
+     "upload tile descriptor");
+  std::vector<SyTile> host(1, t);
+  const SyTile* dt = ctx->stile.as<SyTile>();
+  // 1. A = H T H^T (reflectors left in A's lower part)
+  CK(sytrd_run(ctx->stream, dt, 1, n, host, kNB), "tridiagonal reduction");
+  // 2. T = Z diag(lam) Z^T, ascending
+  CK(dc_solve(ctx->stream, ctx->dc, n, ctx->sd.as<double>(), ctx->se.as<double>(),
+              ctx->slam.as<double>(), ctx->sZ.as<double>()),
+     "divide-and-conquer");
+  // 3. descending order into Q, L (imcom_matrix.f90:656-663)
+  CK(launch_reverse_cols(ctx->stream, n, ctx->sZ.as<double>(), ctx->Q.as<double>(),
+                         ctx->slam.as<double>(), ctx->L.as<double>()),
+     "reverse");
+  // 4. Q = H Q
+  CK(ormtr_run(ctx->stream, dt, 1, n, host, kNBB), "back-transform");
+  ctx->have_A = false;  // A was overwritten by the reduction
+  ctx->have_QL = true;
+  ctx->have_P = ctx->have_K = ctx->have_T = false;
+  CK(download(ctx, Q_out, ctx->Q, (size_t)n * n * 8), "download Q");
+  CK(download(ctx, L_out, ctx->L, (size_t)n * 8), "download L");
+  if (condition) {
+    double ends[2];
+    CK(hipMemcpyAsync(&ends[0], ctx->L.as<double>(), 8, hipMemcpyDeviceToHost, ctx->stream),
+       "download L0");
+    CK(hipMemcpyAsync(&ends[1], ctx->L.as<double>() + (n - 1), 8, hipMemcpyDeviceToHost,
+                      ctx->stream),
+       "download Ln");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+    *condition = ends[0] / ends[1];
+  }
+  return IMCOM_OK;
+}
+
+int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_A, IMCOM_ESTATE, "tridiagonalize: A not built (lookup_A or set_A first)");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int64_t n = ctx->n;
+  SyTile t;
+  CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
+  CK(hipMemcpyAsync(ctx->stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
+     "upload tile descriptor");
+  std::vector<SyTile> host(1, t);
+  CK(sytrd_run(ctx->stream, ctx->stile.as<SyTile>(), 1, n, host, kNB), "tridiagonal reduction");
+  ctx->have_A = false;
+  CK(download(ctx, d_out, ctx->sd, (size_t)n * 8), "download d");
+  if (n > 1) CK(download(ctx, e_out, ctx->se, (size_t)(n - 1) * 8), "download e");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_tridiag_eig(imcom_gpu_ctx* ctx, int64_t n, const double* d, const double* e,
+                          double* lam_out, double* Z_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(n >= 1 && n <= 32768 && d && (e || n == 1), IMCOM_EARG, "tridiag_eig: bad arguments");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(upload(ctx, ctx->sd, d, (size_t)n * 8), "upload d");
+  CK(ctx->se.reserve((size_t)n * 8), "alloc e");
+  if (n > 1) CK(hipMemcpyAsync(ctx->se.p, e, (size_t)(n - 1) * 8, hipMemcpyHostToDevice, ctx->stream), "upload e");
+  CK(ctx->slam.reserve((size_t)n * 8), "alloc lam");
+  CK(ctx->sZ.reserve((size_t)n * n * 8), "alloc Z");
+  CK(dc_solve(ctx->stream, ctx->dc, n, ctx->sd.as<double>(), ctx->se.as<double>(),
+              ctx->slam.as<double>(), ctx->sZ.as<double>()),
+     "divide-and-conquer");
+  CK(download(ctx, lam_out, ctx->slam, (size_t)n * 8), "download lam");
+  CK(download(ctx, Z_out, ctx->sZ, (size_t)n * n * 8), "download Z");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_set_QL(imcom_gpu_ctx* ctx, const double* Q, const double* L) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_tile && Q && L, IMCOM_EARG, "set_QL: tile not set or NULL input");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int64_t n = ctx->n;
+  CK(upload(ctx, ctx->Q, Q, (size_t)n * n * 8), "upload Q");
+  CK(upload(ctx, ctx->L, L, (size_t)n * 8), "upload L");
+  CK(hipStreamSynchronize(ctx->stream), "set_QL sync");
+  ctx->have_QL = true;
+  ctx->have_P = ctx->have_K = ctx->have_T = false;
+  return IMCOM_OK;
+}
+
+static int gemm_prec(imcom_gpu_ctx* ctx, int precision, bool ta, int M, int N, int K,
+                     double alpha, const double* A, int64_t lda, const double* B, int64_t ldb,
+                     double* C, int64_t ldc, const char* what) {
+  if (precision == IMCOM_PREC_FP64) {
+    CK(dgemm(ctx->stream, ta, false, M, N, K, alpha, A, lda, B, ldb, 0.0, C, ldc), what);
+    return IMCOM_OK;
+  }
+  NEED(precision == IMCOM_PREC_FP32X3 || precision == IMCOM_PREC_BF16X6, IMCOM_EARG,
+       "precision must be IMCOM_PREC_FP64, _FP32X3 or _BF16X6");
+  const size_t wb = gemm_split_work_bytes(precision, M, N, K);
+  CK(ctx->split_work.reserve(wb), "alloc split workspace");
+  CK(gemm_split(ctx->stream, precision, ta, M, N, K, alpha, A, lda, B, ldb, C, ldc,
+                ctx->split_work.p, wb),
+     what);
+  return IMCOM_OK;
+}
+
+int imcom_gpu_build_P(imcom_gpu_ctx* ctx, int precision, double* P_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_QL, IMCOM_ESTATE, "build_P: Q, L not available (eigen_A or set_QL first)");
+  NEED(ctx->have_B, IMCOM_ESTATE, "build_P: B not built (lookup_B or set_B first)");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int64_t n = ctx->n, m = ctx->m;
+  CK(ctx->P.reserve((size_t)n * m * 8), "alloc P");
+  int rc = gemm_prec(ctx, precision, true, (int)n, (int)m, (int)n, 1.0, ctx->Q.as<double>(), n,
+                     ctx->B.as<double>(), n, ctx->P.as<double>(), n, "P = Q^T B");
+  if (rc) return rc;
+  ctx->have_P = true;
+  ctx->have_K = ctx->have_T = false;
+  CK(download(ctx, P_out, ctx->P, (size_t)n * m * 8), "download P");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_set_P(imcom_gpu_ctx* ctx, const double* P) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_tile && P, IMCOM_EARG, "set_P: tile not set or NULL P");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(upload(ctx, ctx->P, P, (size_t)ctx->n * ctx->m * 8), "upload P");
+  CK(hipStreamSynchronize(ctx->stream), "set_P sync");
+  ctx->have_P = true;
+  ctx->have_K = ctx->have_T = false;
+  return IMCOM_OK;
+}
+
+int imcom_gpu_bisect_kappa(imcom_gpu_ctx* ctx, int UnotS, double US_max, double k_min,
+                           double k_max, double tol, int maxnits, const float* bracket,
+                           double* K_out, double* US_out, int32_t* evals_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_P && ctx->have_QL, IMCOM_ESTATE, "bisect_kappa: needs L and P");
+  NEED(k_min > 0.0 && k_max > k_min, IMCOM_EARG, "bisect_kappa: need 0 < k_min < k_max");
+  NEED(maxnits >= 0, IMCOM_EARG, "bisect_kappa: maxnits must be >= 0");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int64_t n = ctx->n, m = ctx->m;
+  CK(ctx->K.reserve(m * 8), "alloc K");
+  CK(ctx->US.reserve(m * 8), "alloc US");
+  CK(ctx->evals.reserve(m * 4), "alloc evals");
+  const float* dbr = nullptr;
+  if (bracket) {
+    CK(upload(ctx, ctx->bracket, bracket, (size_t)m * 2 * sizeof(float)), "upload bracket");
+    dbr = ctx->bracket.as<float>();
+  }
+  CK(launch_bisect(ctx->stream, n, m, ctx->L.as<double>(), ctx->P.as<double>(), n,
+                   ctx->Nd.as<double>(), ctx->C_a, US_max, tol, k_min, k_max, maxnits,
+                   UnotS ? 1 : 0, dbr, ctx->K.as<double>(), ctx->US.as<double>(),
+                   ctx->evals.as<int32_t>()),
+     "bisect");
+  ctx->have_K = true;
+  ctx->have_T = false;
+  CK(download(ctx, K_out, ctx->K, m * 8), "download K");
+  CK(download(ctx, US_out, ctx->US, m * 8), "download U/S");
+  CK(download(ctx, evals_out, ctx->evals, m * 4), "download evals");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_build_T(imcom_gpu_ctx* ctx, int precision, double* T_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_K && ctx->have_P && ctx->have_QL, IMCOM_ESTATE,
+       "build_T: needs Q, L, P and K (bisect_kappa first)");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  const int64_t n = ctx->n, m = ctx->m;
+  CK(ctx->W.reserve((size_t)n * m * 8), "alloc W");
+  CK(ctx->T.reserve((size_t)n * m * 8), "alloc T");
+  CK(launch_scale_W(ctx->stream, n, m, ctx->P.as<double>(), n, ctx->L.as<double>(),
+                    ctx->K.as<double>(), ctx->W.as<double>(), n),
+     "W = P / (L + K)");
+  int rc = gemm_prec(ctx, precision, false, (int)n, (int)m, (int)n, -0.5, ctx->Q.as<double>(), n,
+                     ctx->W.as<double>(), n, ctx->T.as<double>(), n, "T = -Q W / 2");
+  if (rc) return rc;
+  ctx->have_T = true;
+  CK(download(ctx, T_out, ctx->T, (size_t)n * m * 8), "download T");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_build_S(imcom_gpu_ctx* ctx, double* S_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_T, IMCOM_ESTATE, "build_S: T not built");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(ctx->S.reserve(ctx->m * 8), "alloc S");
+  CK(launch_maps(ctx->stream, ctx->n, ctx->m, ctx->T.as<double>(), ctx->n, nullptr,
+                 ctx->Nd.as<double>(), nullptr, ctx->S.as<double>()),
+     "build_S");
+  CK(download(ctx, S_out, ctx->S, ctx->m * 8), "download S");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_build_U(imcom_gpu_ctx* ctx, double* U_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_K && ctx->have_P && ctx->have_QL, IMCOM_ESTATE, "build_U: needs L, P and K");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(ctx->U.reserve(ctx->m * 8), "alloc U");
+  CK(launch_U_from_K(ctx->stream, ctx->n, ctx->m, ctx->L.as<double>(), ctx->P.as<double>(), ctx->n,
+                     ctx->C_a, ctx->K.as<double>(), ctx->U.as<double>()),
+     "build_U");
+  CK(download(ctx, U_out, ctx->U, ctx->m * 8), "download U");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_build_image(imcom_gpu_ctx* ctx, const double* I_i, double* H_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_T, IMCOM_ESTATE, "build_image: T not built");
+  NEED(I_i != nullptr, IMCOM_EARG, "build_image: NULL I_i");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(upload(ctx, ctx->I, I_i, ctx->n * 8), "upload I_i");
+  CK(ctx->H.reserve(ctx->m * 8), "alloc H");
+  CK(launch_maps(ctx->stream, ctx->n, ctx->m, ctx->T.as<double>(), ctx->n, ctx->I.as<double>(),
+                 nullptr, ctx->H.as<double>(), nullptr),
+     "build_image");
+  CK(download(ctx, H_out, ctx->H, ctx->m * 8), "download H");
+  return IMCOM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,219 @@
+// FP64 GEMM on CDNA4 matrix cores (v_mfma_f64_16x16x4_f64).
+//
+//   C (M x N) = alpha * op(A) * op(B) + beta * C      (column-major)
+//
+// Used for P = Q^T B (imcom_matrix.f90:699, DGEMM('T','N')), for
+// T = -1/2 Q W with W = P / (L + K_a) (imcom_matrix.f90:731-745: the m DGEMVs
+// done as one GEMM), and inside the eigensolver (trailing updates of the
+// tridiagonal reduction, divide-and-conquer eigenvector products, Householder
+// back-transform).  A grouped form runs many independent GEMMs of different
+// shapes in one launch (one descriptor per group, blocks mapped to groups by
+// a prefix array).
+//
+// Tiling: a 256-thread workgroup (4 wavefronts as 2x2) owns a BM x BN tile of
+// C; each wavefront owns (BM/2) x (BN/2) as 16x16 MFMA sub-tiles.  K advances
+// in BK=16 slabs staged through LDS with register prefetch of the next slab
+// (two LDS buffers, one barrier per slab).  The MFMA takes the B slab as its
+// first operand, which puts 16 consecutive rows of one C column in the 16
+// lanes of a lane group: stores leave as 128-byte column segments.
+// (f64 MFMA lane layout, measured on gfx950: A operand lane l = A[l%16][l/16],
+// B operand lane l = B[l/16][l%16], result register r of lane l =
+// D[4r + l/16][l%16].)
+#include "common.h"
+#include "kernels.h"
+
+namespace imcom {
+
+constexpr int BK = 16;
+
+template <bool TA, bool TB, int BM, int BN, bool GROUPED>
+__global__ __launch_bounds__(256) void dgemm_kernel(GemmDesc one, const GemmDesc* __restrict__ descs,
+                                                    const int* __restrict__ prefix, int ngroups) {
+  constexpr int PA = BM + 1;  // padded LDS row lengths
+  constexpr int PB = BN + 1;
+  __shared__ double As[2][BK * PA];
+  __shared__ double Bs[2][BK * PB];
+  constexpr int LA = BM * BK / 256;  // doubles per thread per slab
+  constexpr int LB = BN * BK / 256;
+  constexpr int WM = BM / 2, WN = BN / 2;  // per-wave tile
+  constexpr int TM = WM / 16, TN = WN / 16;
+
+  int b = blockIdx.x;
+  GemmDesc g;
+  if (GROUPED) {
+    int lo = 0, hi = ngroups - 1;  // last group with prefix[grp] <= b
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (prefix[mid] <= b)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    g = descs[lo];
+    b -= prefix[lo];
+  } else {
+    g = one;
+  }
+  const int M = g.M, N = g.N, K = g.K;
+  const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+  const int nt = tiles_m * tiles_n;
+  if (b >= nt) return;
+  // blocks are handed round-robin to the 8 XCDs: give each XCD a contiguous
+  // run of tiles so neighbouring tiles share A rows / B columns in its L2
+  if (!GROUPED && (nt & 7) == 0) b = (b & 7) * (nt >> 3) + (b >> 3);
+  // column-major walk in groups of 8 tile rows
+  const int group = 8 * tiles_n;
+  const int gi0 = b / group;
+  const int first_m = gi0 * 8;
+  const int gm = min(tiles_m - first_m, 8);
+  const int tm = first_m + (b % group) % gm;
+  const int tn = (b % group) / gm;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const double* __restrict__ A = g.A;
+  const double* __restrict__ B = g.B;
+  const int64_t lda = g.lda, ldb = g.ldb;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int wm0 = (wid & 1) * WM, wn0 = (wid >> 1) * WN;
+
+  v4d acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = (v4d){0.0, 0.0, 0.0, 0.0};
+
+  double ra[LA], rb[LB];
+  auto load_slab = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int idx = tid + q * 256;
+      const int i = TA ? idx / BK : idx % BM;
+      const int k = TA ? idx % BK : idx / BM;
+      const int gi = m0 + i, gk = k0 + k;
+      double v = 0.0;
+      if (gi < M && gk < K) v = TA ? A[gk + (int64_t)gi * lda] : A[gi + (int64_t)gk * lda];
+      ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+      const int idx = tid + q * 256;
+      const int j = TB ? idx % BN : idx / BK;
+      const int k = TB ? idx / BN : idx % BK;
+      const int gj = n0 + j, gk = k0 + k;
+      double v = 0.0;
+      if (gj < N && gk < K) v = TB ? B[gj + (int64_t)gk * ldb] : B[gk + (int64_t)gj * ldb];
+      rb[q] = v;
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < LA; ++q) {
+      const int idx = tid + q * 256;
+      const int i = TA ? idx / BK : idx % BM;
+      const int k = TA ? idx % BK : idx / BM;
+      As[buf][k * PA + i] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < LB; ++q) {
+      const int idx = tid + q * 256;
+      const int j = TB ? idx % BN : idx / BK;
+      const int k = TB ? idx / BN : idx % BK;
+      Bs[buf][k * PB + j] = rb[q];
+    }
+  };
+
+  const int nslab = (K + BK - 1) / BK;
+  if (nslab > 0) {
+    load_slab(0);
+    store_slab(0);
+  }
+  __syncthreads();
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int s = 0; s < nslab; ++s) {
+    const int buf = s & 1;
+    if (s + 1 < nslab) load_slab((s + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double av[TM], bv[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) av[i] = As[buf][(kk + lk) * PA + wm0 + i * 16 + lr];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bv[j] = Bs[buf][(kk + lk) * PB + wn0 + j * 16 + lr];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[j], av[i], acc[i][j], 0, 0, 0);
+    }
+    if (s + 1 < nslab) store_slab(buf ^ 1);
+    __syncthreads();
+  }
+
+  // D'[j][i] = sum_k B[k][j] A[i][k] = C[i][j].  The f64 16x16x4 MFMA leaves
+  // D'[4*r + l/16][l%16] in lane l, register r: C row (l%16), column
+  // 4*r + l/16 of the sub-tile.
+  const double alpha = g.alpha, beta = g.beta;
+  double* __restrict__ C = g.C;
+  const int64_t ldc = g.ldc;
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gi = m0 + wm0 + i * 16 + lr;
+        const int gj = n0 + wn0 + j * 16 + 4 * r + lk;
+        if (gi < M && gj < N) {
+          double* c = C + gi + (int64_t)gj * ldc;
+          const double v = alpha * acc[i][j][r];
+          *c = (beta == 0.0) ? v : v + beta * (*c);
+        }
+      }
+}
+
+template <int BM, int BN, bool GROUPED>
+static hipError_t launch_sz(hipStream_t st, bool ta, bool tb, dim3 grid, const GemmDesc& one,
+                            const GemmDesc* descs, const int* prefix, int ngroups) {
+#define IMCOM_DG(TA_, TB_)                                                                   \
+  hipLaunchKernelGGL((dgemm_kernel<TA_, TB_, BM, BN, GROUPED>), grid, dim3(256), 0, st, one, \
+                     descs, prefix, ngroups)
+  if (!ta && !tb)
+    IMCOM_DG(false, false);
+  else if (ta && !tb)
+    IMCOM_DG(true, false);
+  else if (!ta && tb)
+    IMCOM_DG(false, true);
+  else
+    IMCOM_DG(true, true);
+#undef IMCOM_DG
+  return hipGetLastError();
+}
+
+hipError_t dgemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, double alpha,
+                 const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                 double* C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  GemmDesc g{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc};
+  // 128x128 tiles when that still gives >= 2 tiles per CU, else 64x64
+  const int64_t t128 = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+  if (t128 >= 512) {
+    dim3 grid((unsigned)t128);
+    return launch_sz<128, 128, false>(st, ta, tb, grid, g, nullptr, nullptr, 0);
+  }
+  const int64_t t64 = (int64_t)((M + 63) / 64) * ((N + 63) / 64);
+  dim3 grid((unsigned)t64);
+  return launch_sz<64, 64, false>(st, ta, tb, grid, g, nullptr, nullptr, 0);
+}
+
+int gemm_tiles64(int M, int N) { return ((M + 63) / 64) * ((N + 63) / 64); }
+
+hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
+                         const int* d_prefix, int ngroups, int total_blocks) {
+  if (ngroups <= 0 || total_blocks <= 0) return hipSuccess;
+  GemmDesc dummy{};
+  dim3 grid((unsigned)total_blocks);
+  return launch_sz<64, 64, true>(st, ta, tb, grid, dummy, d_descs, d_prefix, ngroups);
+}
+
+}  // namespace imcom

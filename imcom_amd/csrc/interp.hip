@@ -1,0 +1,183 @@
+// Kernel (a): build A (n x n) and B (n x m) by 2-D polynomial interpolation
+// of the PSF-correlation lookup tables.
+//
+// Reference: imcom_lookup_A (imcom_matrix.f90:206-284), imcom_lookup_B
+// (:288-357), imcom_interp_lookup (:497-542) and the Neville routines
+// imcom_polin2/imcom_polint (imcom_proc.f90:351-429).
+//
+// The reference evaluates the interpolating polynomial through the clipped
+// stencil [floor(arg)-npm, ceil(arg)+npm] with Neville's tableau.  Here the
+// same polynomial (same nodes, same argument) is evaluated in Lagrange form:
+// per axis, the <=8 weights come from prefix/suffix products of (arg - node)
+// and a table of exact reciprocal node-difference products, then the value
+// is the separable 2-D weighted sum of the <=8x8 window.  Identical in exact
+// arithmetic; rounding differs at the 1e-16 relative level.
+//
+// Layout: A and B column-major with leading dimension n; the tables are
+// column-major (n1, n2) slices, `tstride` doubles apart.
+#include "common.h"
+#include "kernels.h"
+
+namespace imcom {
+
+// c_invden[cnt-1][k] = 1 / prod_{l != k, l < cnt} (k - l)
+__constant__ double c_invden[8][8];
+
+static bool g_invden_ready[64] = {false};
+
+static hipError_t ensure_invden(int dev) {
+  if (dev >= 0 && dev < 64 && g_invden_ready[dev]) return hipSuccess;
+  double h[8][8];
+  for (int cnt = 1; cnt <= 8; ++cnt)
+    for (int k = 0; k < 8; ++k) {
+      double p = 1.0;
+      if (k < cnt) {
+        for (int l = 0; l < cnt; ++l)
+          if (l != k) p *= (double)(k - l);
+        h[cnt - 1][k] = 1.0 / p;
+      } else {
+        h[cnt - 1][k] = 0.0;
+      }
+    }
+  IMCOM_HIP_TRY(hipMemcpyToSymbol(HIP_SYMBOL(c_invden), h, sizeof(h)));
+  if (dev >= 0 && dev < 64) g_invden_ready[dev] = true;
+  return hipSuccess;
+}
+
+// Lagrange weights at 1-based table coordinate `arg` for nodes lo..lo+cnt-1.
+__device__ __forceinline__ void lagrange8(double arg, int lo, int cnt, double w[8]) {
+  double d[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) d[l] = (l < cnt) ? (arg - (double)(lo + l)) : 1.0;
+  double pre[8], suf[8];
+  pre[0] = 1.0;
+#pragma unroll
+  for (int l = 1; l < 8; ++l) pre[l] = pre[l - 1] * d[l - 1];
+  suf[7] = 1.0;
+#pragma unroll
+  for (int l = 6; l >= 0; --l) suf[l] = suf[l + 1] * d[l + 1];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) w[k] = pre[k] * suf[k] * c_invden[cnt - 1][k];
+}
+
+// imcom_interp_lookup: table value at offset (xarg, yarg) table pixels from
+// the centre pixel (n1/2+1, n2/2+1); 0 outside the table.
+__device__ __forceinline__ double interp_lookup(const double* __restrict__ tab, int n1, int n2,
+                                                int npm, double xarg, double yarg) {
+  const double iarg = (double)(n1 / 2 + 1) + xarg;
+  const double jarg = (double)(n2 / 2 + 1) + yarg;
+  const double fi = floor(iarg), fj = floor(jarg);
+  const double ci = ceil(iarg), cj = ceil(jarg);
+  if (!(fi >= 1.0 && fj >= 1.0 && ci <= (double)n1 && cj <= (double)n2)) return 0.0;
+  const int imin = max(1, (int)fi - npm), jmin = max(1, (int)fj - npm);
+  const int imax = min(n1, (int)ci + npm), jmax = min(n2, (int)cj + npm);
+  const int cx = imax - imin + 1, cy = jmax - jmin + 1;
+  double wx[8], wy[8];
+  lagrange8(iarg, imin, cx, wx);
+  lagrange8(jarg, jmin, cy, wy);
+  const double* base = tab + (imin - 1) + (int64_t)(jmin - 1) * n1;
+  double acc = 0.0;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    const double* col = base + (int64_t)min(l, cy - 1) * n1;
+    double r = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) r += wx[k] * col[min(k, cx - 1)];
+    acc += wy[l] * r;
+  }
+  return acc;
+}
+
+// Upper-triangular 64x64 tiles of A; each tile is also written mirrored.
+__global__ __launch_bounds__(256) void lookup_A_kernel(
+    int64_t n, const double* __restrict__ x, const double* __restrict__ y,
+    const int32_t* __restrict__ ex, int psfconst, const double* __restrict__ tabs,
+    int64_t tstride, int n1, int n2, int npm, double npad, double sx, double sy,
+    double* __restrict__ A) {
+  __shared__ double tileS[64][65];
+  // blockIdx.x -> (bi, bj) with bi <= bj, row-major over the upper triangle
+  const int64_t t = blockIdx.x;
+  int64_t bj = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+  while ((bj + 1) * (bj + 2) / 2 <= t) ++bj;
+  while (bj * (bj + 1) / 2 > t) --bj;
+  const int64_t bi = t - bj * (bj + 1) / 2;
+  const int64_t i0 = bi * 64, j0 = bj * 64;
+  const int li = threadIdx.x & 63, lq = threadIdx.x >> 6;
+  const int64_t i = i0 + li;
+  double xi = 0.0, yi = 0.0;
+  int ei = 1;
+  if (i < n) {
+    xi = x[i];
+    yi = y[i];
+    ei = ex[i];
+  }
+#pragma unroll 2
+  for (int s = 0; s < 16; ++s) {
+    const int lj = lq + 4 * s;
+    const int64_t j = j0 + lj;
+    double v = 0.0;
+    if (i < n && j < n && j >= i) {
+      const int ej = ex[j];
+      const int ial = psfconst == 1 ? 1 : ei + ej * (ej - 1) / 2;
+      const double Delx = npad * (x[j] - xi) / sx;
+      const double Dely = npad * (y[j] - yi) / sy;
+      v = interp_lookup(tabs + (int64_t)(ial - 1) * tstride, n1, n2, npm, Delx, Dely);
+      A[i + j * n] = v;
+    }
+    tileS[lj][li] = v;
+  }
+  __syncthreads();
+  // mirror: A(j, i) = A(i, j) for j > i, consecutive threads -> consecutive j
+#pragma unroll 4
+  for (int s = 0; s < 16; ++s) {
+    const int lj = threadIdx.x & 63;
+    const int li2 = (threadIdx.x >> 6) + 4 * s;
+    const int64_t jj = j0 + lj, ii = i0 + li2;
+    if (jj < n && ii < n && jj > ii) A[jj + ii * n] = tileS[lj][li2];
+  }
+}
+
+__global__ __launch_bounds__(256) void lookup_B_kernel(
+    int64_t n, int64_t m, const double* __restrict__ x, const double* __restrict__ y,
+    const int32_t* __restrict__ ex, const double* __restrict__ Xa, const double* __restrict__ Ya,
+    int psfconst, const double* __restrict__ tabs, int64_t tstride, int n1, int n2, int npm,
+    double npad, double sx, double sy, double* __restrict__ B) {
+  const int64_t i = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const int64_t a0 = (int64_t)blockIdx.y * 64;
+  const int lq = threadIdx.x >> 6;
+  if (i >= n) return;
+  const double xi = x[i], yi = y[i];
+  const double* tab = tabs + (psfconst == 1 ? 0 : (int64_t)(ex[i] - 1) * tstride);
+#pragma unroll 2
+  for (int s = 0; s < 16; ++s) {
+    const int64_t a = a0 + lq + 4 * s;
+    if (a >= m) break;
+    const double Delx = npad * (xi - Xa[a]) / sx;
+    const double Dely = npad * (yi - Ya[a]) / sy;
+    B[i + a * n] = interp_lookup(tab, n1, n2, npm, Delx, Dely);
+  }
+}
+
+hipError_t launch_lookup_A(hipStream_t st, int dev, int64_t n, const double* x, const double* y,
+                           const int32_t* ex, int psfconst, const double* tabs, int64_t tstride,
+                           int n1, int n2, int npm, int npad, double sx, double sy, double* A) {
+  IMCOM_HIP_TRY(ensure_invden(dev));
+  const int64_t T = (n + 63) / 64;
+  const int64_t nblk = T * (T + 1) / 2;
+  hipLaunchKernelGGL(lookup_A_kernel, dim3((unsigned)nblk), dim3(256), 0, st, n, x, y, ex, psfconst,
+                     tabs, tstride, n1, n2, npm, (double)npad, sx, sy, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_lookup_B(hipStream_t st, int dev, int64_t n, int64_t m, const double* x,
+                           const double* y, const int32_t* ex, const double* Xa, const double* Ya,
+                           int psfconst, const double* tabs, int64_t tstride, int n1, int n2,
+                           int npm, int npad, double sx, double sy, double* B) {
+  IMCOM_HIP_TRY(ensure_invden(dev));
+  dim3 grid((unsigned)((n + 63) / 64), (unsigned)((m + 63) / 64));
+  hipLaunchKernelGGL(lookup_B_kernel, grid, dim3(256), 0, st, n, m, x, y, ex, Xa, Ya, psfconst, tabs,
+                     tstride, n1, n2, npm, (double)npad, sx, sy, B);
+  return hipGetLastError();
+}
+
+}  // namespace imcom

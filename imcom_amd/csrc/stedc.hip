@@ -1,0 +1,759 @@
+// Kernel (b), part 2: divide-and-conquer eigensolver for the symmetric
+// tridiagonal matrix produced by the Householder reduction (the role of the
+// tridiagonal stage inside DSYEVR, imcom_matrix.f90:642).
+//
+// Cuppen's method with Gu-Eisenstat eigenvectors, fully on the GPU:
+//  * the matrix is scaled to unit max-norm, every off-diagonal e_i becomes a
+//    split point of a balanced binary tree whose leaves are 1x1, and the
+//    leaves hold d_i - |e_{i-1}| - |e_i| with eigenvector 1;
+//  * merges are processed level by level (all merges of a level in one launch
+//    per step).  A merge of blocks (D1,Z1), (D2,Z2) coupled by beta solves
+//    D + rho z z^T, rho = 2|beta|, z = [last row of Z1, sign(beta) first row
+//    of Z2] / sqrt(2), after deflation:
+//      - small rho|z_i| <= tol, tol = 8 eps max(|d|max, |z|max): the column
+//        keeps its eigenvector;
+//      - close pairs |t c s| <= tol: a Givens rotation zeroes one z entry;
+//    (the deflation rules of LAPACK's dlaed2, kept step for step);
+//  * the k non-deflated roots of 1 + rho sum z_i^2/(d_i - x) = 0 are found
+//    one per thread, stored as (origin pole, offset tau) so d_i - lambda_j is
+//    formed without cancellation; the root iteration models the two nearest
+//    poles (value and slope matched) and falls back to bisection;
+//  * z is recomputed from the roots (Loewner / Gu-Eisenstat) so the
+//    eigenvectors u_j = (zhat_i / (d_i - lambda_j))_i are orthogonal to
+//    working precision;
+//  * new eigenvectors = Z(:, non-deflated) x U on the FP64 matrix cores
+//    (grouped GEMM over the merges of the level);
+//  * eigenvalues of each merge are left in ascending order.
+// One host synchronisation per level reads the deflation counts.
+#include "common.h"
+#include "kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace imcom {
+
+struct DcMerge {
+  int off, n1, n2;
+  double rho;    // |beta|
+  double sigma;  // sign(beta)
+  int64_t u_off;   // offset of this merge's k x k U block
+  int64_t zg_off;  // offset of this merge's K x K gather block (and K x k output)
+  int root_off;    // offset of this merge's roots in the level's root arrays
+};
+
+struct DcCount {
+  int k;     // non-deflated
+  int nrot;  // deflation rotations
+};
+
+struct DcWork {
+  int64_t n = 0;
+  double *D = nullptr, *ds = nullptr, *zs = nullptr, *dl = nullptr, *wz = nullptr,
+         *dfd = nullptr, *rc = nullptr, *rs = nullptr, *tau = nullptr, *lam = nullptr,
+         *zhat = nullptr, *U = nullptr, *Zg = nullptr, *Zo = nullptr, *scale = nullptr;
+  int *cs = nullptr, *ndcol = nullptr, *dfcol = nullptr, *rp = nullptr, *rn = nullptr,
+      *org = nullptr, *d_prefix = nullptr;
+  DcMerge* d_merges = nullptr;
+  DcCount* d_counts = nullptr;
+  GemmDesc* d_descs = nullptr;
+  int cap_merges = 0;
+  DcCount* h_counts = nullptr;  // pinned
+};
+
+static void dc_free(DcWork* w) {
+  void* ptrs[] = {w->D,   w->ds,    w->zs,    w->dl,       w->wz,       w->dfd,    w->rc,
+                  w->rs,  w->tau,   w->lam,   w->zhat,     w->U,        w->Zg,     w->Zo,
+                  w->scale, w->cs,  w->ndcol, w->dfcol,    w->rp,       w->rn,     w->org,
+                  w->d_prefix, w->d_merges, w->d_counts, w->d_descs};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (w->h_counts) (void)hipHostFree(w->h_counts);
+  *w = DcWork();
+}
+
+DcWork* dc_create() { return new DcWork(); }
+void dc_destroy(DcWork* w) {
+  if (!w) return;
+  dc_free(w);
+  delete w;
+}
+
+static hipError_t dc_reserve(DcWork* w, int64_t n) {
+  if (w->n >= n && w->cap_merges >= n) return hipSuccess;
+  dc_free(w);
+  const size_t nd = (size_t)n * sizeof(double), ni = (size_t)n * sizeof(int);
+  const size_t nn = (size_t)n * (size_t)n * sizeof(double);
+  IMCOM_HIP_TRY(hipMalloc(&w->D, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->ds, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->zs, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->dl, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->wz, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->dfd, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->rc, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->rs, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->tau, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->lam, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->zhat, nd));
+  IMCOM_HIP_TRY(hipMalloc(&w->scale, sizeof(double) * 2));
+  IMCOM_HIP_TRY(hipMalloc(&w->U, nn));
+  IMCOM_HIP_TRY(hipMalloc(&w->Zg, nn));
+  IMCOM_HIP_TRY(hipMalloc(&w->Zo, nn));
+  IMCOM_HIP_TRY(hipMalloc(&w->cs, ni));
+  IMCOM_HIP_TRY(hipMalloc(&w->ndcol, ni));
+  IMCOM_HIP_TRY(hipMalloc(&w->dfcol, ni));
+  IMCOM_HIP_TRY(hipMalloc(&w->rp, ni));
+  IMCOM_HIP_TRY(hipMalloc(&w->rn, ni));
+  IMCOM_HIP_TRY(hipMalloc(&w->org, ni));
+  IMCOM_HIP_TRY(hipMalloc(&w->d_prefix, ni + sizeof(int)));
+  IMCOM_HIP_TRY(hipMalloc(&w->d_merges, (size_t)n * sizeof(DcMerge)));
+  IMCOM_HIP_TRY(hipMalloc(&w->d_counts, (size_t)n * sizeof(DcCount)));
+  IMCOM_HIP_TRY(hipMalloc(&w->d_descs, (size_t)n * sizeof(GemmDesc)));
+  IMCOM_HIP_TRY(hipHostMalloc(&w->h_counts, (size_t)n * sizeof(DcCount)));
+  w->n = n;
+  w->cap_merges = (int)n;
+  return hipSuccess;
+}
+
+// ---- set-up ----------------------------------------------------------------
+// scale = max(|d|, |e|) (one workgroup)
+__global__ __launch_bounds__(1024) void dc_scale_kernel(int64_t n, const double* __restrict__ d,
+                                                        const double* __restrict__ e,
+                                                        double* __restrict__ scale) {
+  __shared__ double red[16];
+  double mx = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    mx = fmax(mx, fabs(d[i]));
+    if (i < n - 1) mx = fmax(mx, fabs(e[i]));
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    scale[0] = m > 0.0 ? m : 1.0;
+  }
+}
+
+// leaves: D_i = (d_i - |e_{i-1}| - |e_i|) / scale, Z = I
+__global__ void dc_leaves_kernel(int64_t n, const double* __restrict__ d,
+                                 const double* __restrict__ e, const double* __restrict__ scale,
+                                 double* __restrict__ D) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = scale[0];
+  double v = d[i] / s;
+  if (i > 0) v -= fabs(e[i - 1] / s);
+  if (i < n - 1) v -= fabs(e[i] / s);
+  D[i] = v;
+}
+
+__global__ void dc_identity_kernel(int64_t n, double* __restrict__ Z) {
+  const int64_t j = blockIdx.y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    Z[i + j * n] = (i == j) ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ int find_merge(const DcMerge* __restrict__ mg, int nm, int64_t g) {
+  // last merge with off <= g; -1 when g is not covered by a merge of this level
+  int lo = 0, hi = nm - 1;
+  if (nm <= 0 || g < mg[0].off) return -1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (mg[mid].off <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const DcMerge& m = mg[lo];
+  return (g < (int64_t)m.off + m.n1 + m.n2) ? lo : -1;
+}
+
+// ---- step 1: merge-sort d of the two children, gather z ---------------------
+__global__ void dc_sort_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm,
+                               const double* __restrict__ D, const double* __restrict__ Z,
+                               double* __restrict__ ds, double* __restrict__ zs,
+                               int* __restrict__ cs) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const int mi = find_merge(mg, nm, g);
+  if (mi < 0) return;
+  const DcMerge m = mg[mi];
+  const int q = (int)(g - m.off);
+  const double* D1 = D + m.off;
+  const double* D2 = D + m.off + m.n1;
+  const double v = D[g];
+  int rank;
+  double z;
+  const double r2 = 0.70710678118654752440;  // 1/sqrt(2)
+  if (q < m.n1) {
+    // rank in D2: count of D2 < v
+    int lo = 0, hi = m.n2;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (D2[mid] < v)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    rank = q + lo;
+    z = Z[(m.off + m.n1 - 1) + g * n] * r2;
+  } else {
+    // rank in D1: count of D1 <= v
+    int lo = 0, hi = m.n1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (D1[mid] <= v)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    rank = (q - m.n1) + lo;
+    z = m.sigma * Z[(m.off + m.n1) + g * n] * r2;
+  }
+  ds[m.off + rank] = v;
+  zs[m.off + rank] = z;
+  cs[m.off + rank] = q;
+}
+
+// ---- step 2: deflation scan (dlaed2 rules), one thread per merge -----------
+__device__ __forceinline__ double dlapy2_d(double x, double y) {
+  const double xa = fabs(x), ya = fabs(y);
+  const double w = fmax(xa, ya), z = fmin(xa, ya);
+  if (z == 0.0) return w;
+  const double q = z / w;
+  return w * sqrt(1.0 + q * q);
+}
+
+__global__ void dc_deflate_kernel(const DcMerge* __restrict__ mg, int nm,
+                                  const double* __restrict__ ds, const double* __restrict__ zs,
+                                  const int* __restrict__ cs, double* __restrict__ dl,
+                                  double* __restrict__ wz, int* __restrict__ ndcol,
+                                  double* __restrict__ dfd, int* __restrict__ dfcol,
+                                  int* __restrict__ rp, int* __restrict__ rn,
+                                  double* __restrict__ rc, double* __restrict__ rs,
+                                  DcCount* __restrict__ counts) {
+  const int mi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (mi >= nm) return;
+  const DcMerge m = mg[mi];
+  const int K = m.n1 + m.n2;
+  const int o = m.off;
+  const double rho = 2.0 * m.rho;
+  const double eps = 1.1102230246251565e-16;
+  double dmax = 0.0, zmax = 0.0;
+#pragma unroll 8
+  for (int j = 0; j < K; ++j) {
+    dmax = fmax(dmax, fabs(ds[o + j]));
+    zmax = fmax(zmax, fabs(zs[o + j]));
+  }
+  const double tol = 8.0 * eps * fmax(dmax, zmax);
+  int k = 0, ndf = 0, nrot = 0;
+  double lastv = -HUGE_VAL;  // largest deflated value so far
+  auto push_df = [&](double v, int col) {
+    // ascending insertion (values arrive almost sorted)
+    int p = ndf;
+    if (v < lastv) {
+      while (p > 0 && dfd[o + p - 1] > v) {
+        dfd[o + p] = dfd[o + p - 1];
+        dfcol[o + p] = dfcol[o + p - 1];
+        --p;
+      }
+    } else {
+      lastv = v;
+    }
+    dfd[o + p] = v;
+    dfcol[o + p] = col;
+    ++ndf;
+  };
+  if (rho * zmax <= tol) {
+    for (int j = 0; j < K; ++j) push_df(ds[o + j], cs[o + j]);
+  } else {
+    int j = 0;
+    // first non-negligible entry
+    for (; j < K; ++j) {
+      if (rho * fabs(zs[o + j]) <= tol)
+        push_df(ds[o + j], cs[o + j]);
+      else
+        break;
+    }
+    double pd = ds[o + j], pz = zs[o + j];
+    int pcol = cs[o + j];
+    for (++j; j < K; ++j) {
+      const double nd = ds[o + j], nz = zs[o + j];
+      const int ncol = cs[o + j];
+      if (rho * fabs(nz) <= tol) {
+        push_df(nd, ncol);
+        continue;
+      }
+      double s = pz, c = nz;
+      const double tau = dlapy2_d(c, s);
+      const double t = nd - pd;
+      c = c / tau;
+      s = -s / tau;
+      if (fabs(t * c * s) <= tol) {
+        // rotate columns (pcol, ncol): x' = c x + s y, y' = c y - s x
+        rp[o + nrot] = pcol;
+        rn[o + nrot] = ncol;
+        rc[o + nrot] = c;
+        rs[o + nrot] = s;
+        ++nrot;
+        const double tp = pd * c * c + nd * s * s;
+        const double tn = pd * s * s + nd * c * c;
+        push_df(tp, pcol);
+        pd = tn;
+        pz = tau;
+        pcol = ncol;
+      } else {
+        dl[o + k] = pd;
+        wz[o + k] = pz;
+        ndcol[o + k] = pcol;
+        ++k;
+        pd = nd;
+        pz = nz;
+        pcol = ncol;
+      }
+    }
+    dl[o + k] = pd;
+    wz[o + k] = pz;
+    ndcol[o + k] = pcol;
+    ++k;
+  }
+  counts[mi].k = k;
+  counts[mi].nrot = nrot;
+}
+
+// ---- step 3: apply the deflation rotations to Z's columns ------------------
+__global__ void dc_rotate_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm,
+                                 const DcCount* __restrict__ counts, const int* __restrict__ rp,
+                                 const int* __restrict__ rn, const double* __restrict__ rc,
+                                 const double* __restrict__ rs, double* __restrict__ Z) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // row
+  if (g >= n) return;
+  const int mi = find_merge(mg, nm, g);
+  if (mi < 0) return;
+  const DcMerge m = mg[mi];
+  const int nr = counts[mi].nrot;
+  const int o = m.off;
+  for (int r = 0; r < nr; ++r) {
+    double* x = Z + g + (int64_t)(o + rp[o + r]) * n;
+    double* y = Z + g + (int64_t)(o + rn[o + r]) * n;
+    const double c = rc[o + r], s = rs[o + r];
+    const double xv = *x, yv = *y;
+    *x = c * xv + s * yv;
+    *y = c * yv - s * xv;
+  }
+}
+
+// ---- step 4: secular equation roots ----------------------------------------
+// f(x) = 1 + rho sum_i w_i^2 / (d_i - x); root j lies in (d_j, d_{j+1}) (last
+// root in (d_{k-1}, d_{k-1} + rho |w|^2]).  Stored as lambda_j = d_org + tau.
+__device__ void secular_eval(int k, const double* __restrict__ dl, const double* __restrict__ w,
+                             double rho, int org, double tau, int j, double& f, double& psi,
+                             double& dpsi, double& phi, double& dphi) {
+  const double dorg = dl[org];
+  psi = 0.0;
+  dpsi = 0.0;
+  phi = 0.0;
+  dphi = 0.0;
+  for (int i = 0; i <= j; ++i) {
+    const double del = (dl[i] - dorg) - tau;
+    const double t = w[i] / del;
+    psi += w[i] * t;
+    dpsi += t * t;
+  }
+  for (int i = j + 1; i < k; ++i) {
+    const double del = (dl[i] - dorg) - tau;
+    const double t = w[i] / del;
+    phi += w[i] * t;
+    dphi += t * t;
+  }
+  psi *= rho;
+  dpsi *= rho;
+  phi *= rho;
+  dphi *= rho;
+  f = 1.0 + psi + phi;
+}
+
+__global__ void dc_secular_kernel(const DcMerge* __restrict__ mg, const DcCount* __restrict__ counts,
+                                  const int* __restrict__ rprefix, int nm, int total,
+                                  const double* __restrict__ dlall, const double* __restrict__ wall,
+                                  int* __restrict__ orgall, double* __restrict__ tauall,
+                                  double* __restrict__ lamall) {
+  const int gr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gr >= total) return;
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rprefix[mid] <= gr)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const DcMerge m = mg[lo];
+  const int k = counts[lo].k;
+  const int j = gr - rprefix[lo];
+  if (j >= k) return;
+  const double* dl = dlall + m.off;
+  const double* w = wall + m.off;
+  const double rho = 2.0 * m.rho;
+  const double eps = 1.1102230246251565e-16;
+  int org;
+  double tlo, thi, tau;
+  if (k == 1) {
+    orgall[m.off] = 0;
+    tauall[m.off] = rho * w[0] * w[0];
+    lamall[m.off] = dl[0] + rho * w[0] * w[0];
+    return;
+  }
+  if (j < k - 1) {
+    const double gap = dl[j + 1] - dl[j];
+    const double mid = 0.5 * gap;
+    double f, psi, dpsi, phi, dphi;
+    secular_eval(k, dl, w, rho, j, mid, j, f, psi, dpsi, phi, dphi);
+    if (f >= 0.0) {
+      org = j;  // root in (d_j, mid]
+      tlo = 0.0;
+      thi = mid;
+    } else {
+      org = j + 1;  // root in (mid, d_{j+1}) -> tau in (-(gap - mid), 0)
+      tlo = -(gap - mid);
+      thi = 0.0;
+    }
+    tau = 0.5 * (tlo + thi);
+  } else {
+    double wn = 0.0;
+    for (int i = 0; i < k; ++i) wn += w[i] * w[i];
+    org = k - 1;
+    tlo = 0.0;
+    thi = rho * wn;
+    tau = 0.5 * thi;
+  }
+  const double dorg = dl[org];
+  for (int it = 0; it < 200; ++it) {
+    double f, psi, dpsi, phi, dphi;
+    secular_eval(k, dl, w, rho, org, tau, j, f, psi, dpsi, phi, dphi);
+    if (f == 0.0) break;
+    const double erretm = 8.0 * (phi - psi) + 1.0 + fabs(tau) * (dpsi + dphi);
+    if (fabs(f) <= eps * erretm) break;
+    if (f < 0.0)
+      tlo = tau;
+    else
+      thi = tau;
+    // two nearest poles around the root, relative to the current iterate
+    double tnew;
+    if (j < k - 1) {
+      const double Dj = (dl[j] - dorg) - tau;          // < 0
+      const double Dj1 = (dl[j + 1] - dorg) - tau;     // > 0
+      const double s = Dj * Dj * dpsi, S = Dj1 * Dj1 * dphi;
+      const double c = f - Dj * dpsi - Dj1 * dphi;
+      const double a = c * (Dj + Dj1) + s + S;
+      const double b = c * Dj * Dj1 + s * Dj1 + S * Dj;  // = f Dj Dj1
+      // root of c eta^2 - a eta + b = 0 nearest to 0: eta = 2b / (a + sign(a) sqrt(disc))
+      const double disc = sqrt(fabs(a * a - 4.0 * b * c));
+      const double den = a + copysign(disc, a);
+      tnew = (den != 0.0) ? tau + 2.0 * b / den : 0.5 * (tlo + thi);
+    } else {
+      const double Dk = (dl[k - 1] - dorg) - tau;  // < 0
+      const double s = Dk * Dk * dpsi;
+      const double c = f - Dk * dpsi;
+      // c + s / (Dk - eta) = 0  ->  eta = Dk + s / c
+      tnew = (c != 0.0) ? tau + (Dk + s / c) : 0.5 * (tlo + thi);
+    }
+    if (!(tnew > tlo && tnew < thi)) tnew = 0.5 * (tlo + thi);
+    if (tnew == tau || thi - tlo <= 4.0 * eps * fmax(fabs(tlo), fabs(thi))) {
+      tau = tnew;
+      break;
+    }
+    tau = tnew;
+  }
+  orgall[m.off + j] = org;
+  tauall[m.off + j] = tau;
+  lamall[m.off + j] = dorg + tau;
+}
+
+// ---- step 5: Gu-Eisenstat zhat ----------------------------------------------
+__global__ void dc_zhat_kernel(const DcMerge* __restrict__ mg, const DcCount* __restrict__ counts,
+                               const int* __restrict__ rprefix, int nm, int total,
+                               const double* __restrict__ dlall, const double* __restrict__ wall,
+                               const int* __restrict__ orgall, const double* __restrict__ tauall,
+                               double* __restrict__ zhat) {
+  const int gr = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gr >= total) return;
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rprefix[mid] <= gr)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const DcMerge m = mg[lo];
+  const int k = counts[lo].k;
+  const int i = gr - rprefix[lo];
+  if (i >= k) return;
+  const double* dl = dlall + m.off;
+  const int* org = orgall + m.off;
+  const double* tau = tauall + m.off;
+  const double di = dl[i];
+  // W(i) = delta_ii prod_{j != i} delta_ij / (d_i - d_j),  delta_ij = d_i - lambda_j
+  double Wi = (di - dl[org[i]]) - tau[i];
+  for (int j = 0; j < k; ++j) {
+    if (j == i) continue;
+    const double dij = (di - dl[org[j]]) - tau[j];
+    Wi *= dij / (di - dl[j]);
+  }
+  const double zi = wall[m.off + i];
+  zhat[m.off + i] = copysign(sqrt(fmax(-Wi, 0.0)), zi);
+}
+
+// ---- step 6: eigenvectors of D + rho z z^T, normalised ----------------------
+// one workgroup per (merge, column j); U_m is k x k, ld k, at u_off.
+__global__ __launch_bounds__(256) void dc_vec_kernel(const DcMerge* __restrict__ mg,
+                                                     const DcCount* __restrict__ counts,
+                                                     const int* __restrict__ rprefix, int nm,
+                                                     const double* __restrict__ dlall,
+                                                     const int* __restrict__ orgall,
+                                                     const double* __restrict__ tauall,
+                                                     const double* __restrict__ zhat,
+                                                     double* __restrict__ Uall) {
+  __shared__ double red[4];
+  const int gr = blockIdx.x;
+  int lo = 0, hi = nm - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (rprefix[mid] <= gr)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const DcMerge m = mg[lo];
+  const int k = counts[lo].k;
+  const int j = gr - rprefix[lo];
+  if (j >= k) return;
+  const double* dl = dlall + m.off;
+  const double dorg = dl[orgall[m.off + j]];
+  const double tj = tauall[m.off + j];
+  double* U = Uall + m.u_off + (int64_t)j * k;
+  double ss = 0.0;
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+    const double del = (dl[i] - dorg) - tj;
+    const double u = zhat[m.off + i] / del;
+    U[i] = u;
+    ss += u * u;
+  }
+  ss = block_sum(ss, red);
+  const double inv = 1.0 / sqrt(ss);
+  for (int i = threadIdx.x; i < k; i += blockDim.x) U[i] *= inv;
+}
+
+// ---- step 7: gather Z columns: non-deflated first, then deflated ------------
+// grid.y = global column g (merge m, local column q); Zg_m is K x K (ld K).
+__global__ void dc_gather_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm,
+                                 const DcCount* __restrict__ counts, const int* __restrict__ ndcol,
+                                 const int* __restrict__ dfcol, const double* __restrict__ Z,
+                                 double* __restrict__ Zg) {
+  const int64_t g = blockIdx.y;
+  const int mi = find_merge(mg, nm, g);
+  if (mi < 0) return;
+  const DcMerge m = mg[mi];
+  const int K = m.n1 + m.n2;
+  const int k = counts[mi].k;
+  const int q = (int)(g - m.off);
+  const int src = (q < k) ? ndcol[m.off + q] : dfcol[m.off + (q - k)];
+  const double* zc = Z + m.off + (int64_t)(m.off + src) * n;
+  double* out = Zg + m.zg_off + (int64_t)q * K;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < K; r += gridDim.x * blockDim.x)
+    out[r] = zc[r];
+}
+
+// ---- step 8: place eigenpairs in ascending order ----------------------------
+// Non-deflated column j (new vectors in Zo_m, K x k) and deflated column jd
+// (in Zg_m at column k + jd) are merged by value.
+__global__ void dc_place_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm,
+                                const DcCount* __restrict__ counts, const double* __restrict__ lam,
+                                const double* __restrict__ dfd, const double* __restrict__ Zg,
+                                const double* __restrict__ Zo, double* __restrict__ D,
+                                double* __restrict__ Z) {
+  const int64_t g = blockIdx.y;
+  const int mi = find_merge(mg, nm, g);
+  if (mi < 0) return;
+  const DcMerge m = mg[mi];
+  const int K = m.n1 + m.n2;
+  const int k = counts[mi].k;
+  const int ndf = K - k;
+  const int q = (int)(g - m.off);
+  const double* L = lam + m.off;
+  const double* F = dfd + m.off;
+  int pos;
+  double val;
+  const double* src;
+  if (q < k) {
+    val = L[q];
+    int lo = 0, hi = ndf;  // count deflated < val
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (F[mid] < val)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    pos = q + lo;
+    src = Zo + m.zg_off + (int64_t)q * K;
+  } else {
+    const int jd = q - k;
+    val = F[jd];
+    int lo = 0, hi = k;  // count non-deflated <= val
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (L[mid] <= val)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    pos = jd + lo;
+    src = Zg + m.zg_off + (int64_t)q * K;
+  }
+  double* dst = Z + m.off + (int64_t)(m.off + pos) * n;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < K; r += gridDim.x * blockDim.x)
+    dst[r] = src[r];
+  if (blockIdx.x == 0 && threadIdx.x == 0) D[m.off + pos] = val;
+}
+
+__global__ void dc_unscale_kernel(int64_t n, const double* __restrict__ D,
+                                  const double* __restrict__ scale, double* __restrict__ lam) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) lam[i] = D[i] * scale[0];
+}
+
+// ---- driver -----------------------------------------------------------------
+struct Node {
+  int off, size, height;
+};
+
+static int build_tree(int off, int size, std::vector<std::vector<Node>>& levels) {
+  if (size <= 1) return 0;
+  const int n1 = (size + 1) / 2;
+  const int h1 = build_tree(off, n1, levels);
+  const int h2 = build_tree(off + n1, size - n1, levels);
+  const int h = std::max(h1, h2) + 1;
+  if ((int)levels.size() < h) levels.resize(h);
+  levels[h - 1].push_back(Node{off, size, h});
+  return h;
+}
+
+hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const double* e,
+                    double* lam, double* Z) {
+  if (n <= 0) return hipSuccess;
+  IMCOM_HIP_TRY(dc_reserve(w, n));
+  hipLaunchKernelGGL(dc_scale_kernel, dim3(1), dim3(1024), 0, st, n, d, e, w->scale);
+  hipLaunchKernelGGL(dc_leaves_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, d, e, w->scale,
+                     w->D);
+  hipLaunchKernelGGL(dc_identity_kernel, dim3(std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)n),
+                     dim3(256), 0, st, n, Z);
+  std::vector<std::vector<Node>> levels;
+  build_tree(0, (int)n, levels);
+  // host copy of e for the coupling signs/magnitudes
+  std::vector<double> he(std::max<int64_t>(n - 1, 1));
+  double hscale = 1.0;
+  if (n > 1) {
+    IMCOM_HIP_TRY(hipMemcpyAsync(he.data(), e, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+    IMCOM_HIP_TRY(hipMemcpyAsync(&hscale, w->scale, sizeof(double), hipMemcpyDeviceToHost, st));
+    IMCOM_HIP_TRY(hipStreamSynchronize(st));
+  }
+  std::vector<DcMerge> hm;
+  std::vector<int> rprefix, bprefix;
+  std::vector<GemmDesc> descs;
+  for (size_t L = 0; L < levels.size(); ++L) {
+    std::vector<Node>& lv = levels[L];
+    std::sort(lv.begin(), lv.end(), [](const Node& a, const Node& b) { return a.off < b.off; });
+    const int nm = (int)lv.size();
+    hm.resize(nm);
+    int64_t zoff = 0;
+    for (int i = 0; i < nm; ++i) {
+      const int n1 = (lv[i].size + 1) / 2;
+      const double beta = he[lv[i].off + n1 - 1] / hscale;
+      hm[i].off = lv[i].off;
+      hm[i].n1 = n1;
+      hm[i].n2 = lv[i].size - n1;
+      hm[i].rho = fabs(beta);
+      hm[i].sigma = beta < 0.0 ? -1.0 : 1.0;
+      hm[i].zg_off = zoff;
+      hm[i].u_off = 0;
+      hm[i].root_off = 0;
+      zoff += (int64_t)lv[i].size * lv[i].size;
+    }
+    IMCOM_HIP_TRY(hipMemcpyAsync(w->d_merges, hm.data(), nm * sizeof(DcMerge), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(dc_sort_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, w->d_merges, nm,
+                       w->D, Z, w->ds, w->zs, w->cs);
+    hipLaunchKernelGGL(dc_deflate_kernel, dim3(ceil_div(nm, 64)), dim3(64), 0, st, w->d_merges, nm,
+                       w->ds, w->zs, w->cs, w->dl, w->wz, w->ndcol, w->dfd, w->dfcol, w->rp, w->rn,
+                       w->rc, w->rs, w->d_counts);
+    IMCOM_HIP_TRY(hipMemcpyAsync(w->h_counts, w->d_counts, nm * sizeof(DcCount),
+                                 hipMemcpyDeviceToHost, st));
+    IMCOM_HIP_TRY(hipStreamSynchronize(st));
+    // per-merge root prefix and U offsets
+    rprefix.assign(nm + 1, 0);
+    int64_t uoff = 0;
+    bool anyrot = false;
+    for (int i = 0; i < nm; ++i) {
+      const int k = w->h_counts[i].k;
+      rprefix[i + 1] = rprefix[i] + k;
+      hm[i].u_off = uoff;
+      hm[i].root_off = rprefix[i];
+      uoff += (int64_t)k * k;
+      anyrot = anyrot || w->h_counts[i].nrot > 0;
+    }
+    const int total = rprefix[nm];
+    IMCOM_HIP_TRY(hipMemcpyAsync(w->d_merges, hm.data(), nm * sizeof(DcMerge), hipMemcpyHostToDevice, st));
+    IMCOM_HIP_TRY(hipMemcpyAsync(w->d_prefix, rprefix.data(), (nm + 1) * sizeof(int),
+                                 hipMemcpyHostToDevice, st));
+    if (anyrot)
+      hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, w->d_merges,
+                         nm, w->d_counts, w->rp, w->rn, w->rc, w->rs, Z);
+    if (total > 0) {
+      hipLaunchKernelGGL(dc_secular_kernel, dim3(ceil_div(total, 64)), dim3(64), 0, st, w->d_merges,
+                         w->d_counts, w->d_prefix, nm, total, w->dl, w->wz, w->org, w->tau, w->lam);
+      hipLaunchKernelGGL(dc_zhat_kernel, dim3(ceil_div(total, 64)), dim3(64), 0, st, w->d_merges,
+                         w->d_counts, w->d_prefix, nm, total, w->dl, w->wz, w->org, w->tau, w->zhat);
+      hipLaunchKernelGGL(dc_vec_kernel, dim3(total), dim3(256), 0, st, w->d_merges, w->d_counts,
+                         w->d_prefix, nm, w->dl, w->org, w->tau, w->zhat, w->U);
+    }
+    int maxK = 0;
+    for (int i = 0; i < nm; ++i) maxK = std::max(maxK, lv[i].size);
+    hipLaunchKernelGGL(dc_gather_kernel, dim3(std::max(1, std::min(ceil_div(maxK, 256), 16)), (unsigned)n),
+                       dim3(256), 0, st, n, w->d_merges, nm, w->d_counts, w->ndcol, w->dfcol, Z, w->Zg);
+    // grouped GEMM: Zo_m (K x k) = Zg_m(:, 0:k) (K x k) * U_m (k x k)
+    descs.clear();
+    bprefix.assign(1, 0);
+    for (int i = 0; i < nm; ++i) {
+      const int K = lv[i].size, k = w->h_counts[i].k;
+      if (k == 0) continue;
+      descs.push_back(GemmDesc{K, k, k, 1.0, 0.0, w->Zg + hm[i].zg_off, K, w->U + hm[i].u_off, k,
+                               w->Zo + hm[i].zg_off, K});
+      bprefix.push_back(bprefix.back() + gemm_tiles64(K, k));
+    }
+    const int ng = (int)descs.size();
+    if (ng > 0) {
+      IMCOM_HIP_TRY(hipMemcpyAsync(w->d_descs, descs.data(), ng * sizeof(GemmDesc),
+                                   hipMemcpyHostToDevice, st));
+      // prefix goes after the root prefix (both small; reuse the same buffer region)
+      IMCOM_HIP_TRY(hipMemcpyAsync(w->rp, bprefix.data(), (ng + 1) * sizeof(int),
+                                   hipMemcpyHostToDevice, st));
+      IMCOM_HIP_TRY(dgemm_grouped(st, false, false, w->d_descs, w->rp, ng, bprefix.back()));
+    }
+    hipLaunchKernelGGL(dc_place_kernel, dim3(std::max(1, std::min(ceil_div(maxK, 256), 16)), (unsigned)n),
+                       dim3(256), 0, st, n, w->d_merges, nm, w->d_counts, w->lam, w->dfd, w->Zg,
+                       w->Zo, w->D, Z);
+    // host vectors (hm, descs, prefixes) must outlive the async copies
+    IMCOM_HIP_TRY(hipStreamSynchronize(st));
+  }
+  hipLaunchKernelGGL(dc_unscale_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, w->D, w->scale,
+                     lam);
+  return hipGetLastError();
+}
+
+}  // namespace imcom

@@ -1,0 +1,274 @@
+"""Python host mirror of the reference's solve-path procedures over the C ABI
+of libimcom_hip.so (include/imcom_gpu.h).
+
+``ImcomGPU`` keeps the reference's procedure names and argument meaning
+(imcom_get_A / imcom_lookup_A, imcom_get_QL / imcom_eigen_A, imcom_get_B /
+imcom_lookup_B, imcom_get_P / imcom_build_P, imcom_get_T /
+imcom_bisect_kappa + imcom_build_T/S/U, imcom_build_image); errors raise
+``ImcomError`` carrying the library's message, as the reference prints
+"IMCOM ERROR: ..." and stops.  There is no CPU fallback: if the library is
+missing or no GPU is visible, construction fails.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libimcom_hip.so")
+
+IMCOM_PREC_FP64 = 0
+IMCOM_PREC_FP32X3 = 1
+IMCOM_PREC_BF16X6 = 2
+
+# every symbol declared in include/imcom_gpu.h
+EXPORTS = [
+    "imcom_gpu_init", "imcom_gpu_finalize", "imcom_gpu_last_error", "imcom_gpu_synchronize",
+    "imcom_gpu_set_tile", "imcom_gpu_set_tables", "imcom_gpu_lookup_A", "imcom_gpu_set_A",
+    "imcom_gpu_eigen_A", "imcom_gpu_tridiagonalize", "imcom_gpu_tridiag_eig",
+    "imcom_gpu_set_QL", "imcom_gpu_lookup_B", "imcom_gpu_set_B", "imcom_gpu_build_P",
+    "imcom_gpu_set_P", "imcom_gpu_bisect_kappa", "imcom_gpu_build_T", "imcom_gpu_build_S",
+    "imcom_gpu_build_U", "imcom_gpu_build_image",
+]
+
+
+class ImcomError(RuntimeError):
+    pass
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_int = ctypes.c_int
+_dbl = ctypes.c_double
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libimcom_hip.so and declare the C signatures.  Loading works
+    without a GPU; calls need one."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise ImcomError(f"{path} is missing: build it with imcom_amd.build.build()")
+    L = ctypes.CDLL(path)
+    sig = {
+        "imcom_gpu_init": (_int, [ctypes.POINTER(_vp), _int]),
+        "imcom_gpu_finalize": (None, [_vp]),
+        "imcom_gpu_last_error": (ctypes.c_char_p, [_vp]),
+        "imcom_gpu_synchronize": (_int, [_vp]),
+        "imcom_gpu_set_tile": (_int, [_vp, _i64, _i64, _dp, _dp, _ip, _dp, _dp, _dp, _dbl]),
+        "imcom_gpu_set_tables": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
+        "imcom_gpu_lookup_A": (_int, [_vp, _int, _int, _dbl, _dbl, _dp]),
+        "imcom_gpu_set_A": (_int, [_vp, _dp]),
+        "imcom_gpu_eigen_A": (_int, [_vp, _dp, _dp, _dp]),
+        "imcom_gpu_tridiagonalize": (_int, [_vp, _dp, _dp]),
+        "imcom_gpu_tridiag_eig": (_int, [_vp, _i64, _dp, _dp, _dp, _dp]),
+        "imcom_gpu_set_QL": (_int, [_vp, _dp, _dp]),
+        "imcom_gpu_lookup_B": (_int, [_vp, _int, _int, _dbl, _dbl, _dp]),
+        "imcom_gpu_set_B": (_int, [_vp, _dp]),
+        "imcom_gpu_build_P": (_int, [_vp, _int, _dp]),
+        "imcom_gpu_set_P": (_int, [_vp, _dp]),
+        "imcom_gpu_bisect_kappa": (_int, [_vp, _int, _dbl, _dbl, _dbl, _dbl, _int, _fp, _dp, _dp, _ip]),
+        "imcom_gpu_build_T": (_int, [_vp, _int, _dp]),
+        "imcom_gpu_build_S": (_int, [_vp, _dp]),
+        "imcom_gpu_build_U": (_int, [_vp, _dp]),
+        "imcom_gpu_build_image": (_int, [_vp, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _ptr(a, kind=_dp):
+    return None if a is None else a.ctypes.data_as(kind)
+
+
+def _vec(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def _mat(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+class ImcomGPU:
+    """One tile's solve path on one GPU (a context of the C ABI)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.imcom_gpu_init(ctypes.byref(h), device)
+        if rc != 0:
+            raise ImcomError(f"imcom_gpu_init failed (code {rc}): no usable GPU {device}")
+        self.h = h
+        self.n = self.m = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.imcom_gpu_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ck(self, rc, what):
+        if rc != 0:
+            msg = self.lib.imcom_gpu_last_error(self.h).decode(errors="replace")
+            raise ImcomError(f"IMCOM ERROR: {what}: {msg} (code {rc})")
+
+    # ---- inputs ---------------------------------------------------------
+    def set_tile(self, tile):
+        """Hand over the imcom_data globals of one tile."""
+        self.n, self.m = tile.n, tile.m
+        self._keep = [_vec(tile.x_i), _vec(tile.y_i), _vec(tile.exp_i, np.int32),
+                      _vec(tile.X_a), _vec(tile.Y_a), _vec(tile.Ndiag)]
+        x, y, e, X, Y, N = self._keep
+        self._ck(self.lib.imcom_gpu_set_tile(self.h, tile.n, tile.m, _ptr(x), _ptr(y), _ptr(e, _ip),
+                                             _ptr(X), _ptr(Y), _ptr(N), tile.C_a), "set_tile")
+        A, B = _mat(tile.Alookup), _mat(tile.Blookup)
+        self._ck(self.lib.imcom_gpu_set_tables(self.h, tile.psfconst, tile.nexppsf, A.shape[0],
+                                               A.shape[1], _ptr(A), _ptr(B)), "set_tables")
+        self.tile = tile
+
+    # ---- imcom_matrix ---------------------------------------------------
+    def lookup_A(self, npoly=7, npad=3, out=False):
+        t = self.tile
+        A = np.empty((self.n, self.n), order="F") if out else None
+        self._ck(self.lib.imcom_gpu_lookup_A(self.h, npoly, npad, t.psfxscale, t.psfyscale,
+                                             _ptr(A)), "lookup_A")
+        return A
+
+    def set_A(self, A):
+        A = _mat(A)
+        self._ck(self.lib.imcom_gpu_set_A(self.h, _ptr(A)), "set_A")
+
+    def eigen_A(self, out=True):
+        n = self.n
+        Q = np.empty((n, n), order="F") if out else None
+        L = np.empty(n) if out else None
+        cond = ctypes.c_double(0.0)
+        self._ck(self.lib.imcom_gpu_eigen_A(self.h, _ptr(Q), _ptr(L), ctypes.byref(cond)),
+                 "eigen_A")
+        return Q, L, cond.value
+
+    def tridiagonalize(self):
+        n = self.n
+        d = np.empty(n)
+        e = np.empty(max(n - 1, 1))
+        self._ck(self.lib.imcom_gpu_tridiagonalize(self.h, _ptr(d), _ptr(e)), "tridiagonalize")
+        return d, e[:n - 1]
+
+    def tridiag_eig(self, d, e):
+        d = _vec(d)
+        e = _vec(e if len(e) else np.zeros(1))
+        n = d.shape[0]
+        lam = np.empty(n)
+        Z = np.empty((n, n), order="F")
+        self._ck(self.lib.imcom_gpu_tridiag_eig(self.h, n, _ptr(d), _ptr(e), _ptr(lam), _ptr(Z)),
+                 "tridiag_eig")
+        return lam, Z
+
+    def set_QL(self, Q, L):
+        Q, L = _mat(Q), _vec(L)
+        self._ck(self.lib.imcom_gpu_set_QL(self.h, _ptr(Q), _ptr(L)), "set_QL")
+
+    def lookup_B(self, npoly=7, npad=3, out=False):
+        t = self.tile
+        B = np.empty((self.n, self.m), order="F") if out else None
+        self._ck(self.lib.imcom_gpu_lookup_B(self.h, npoly, npad, t.psfxscale, t.psfyscale,
+                                             _ptr(B)), "lookup_B")
+        return B
+
+    def set_B(self, B):
+        B = _mat(B)
+        self._ck(self.lib.imcom_gpu_set_B(self.h, _ptr(B)), "set_B")
+
+    def build_P(self, precision=IMCOM_PREC_FP64, out=False):
+        P = np.empty((self.n, self.m), order="F") if out else None
+        self._ck(self.lib.imcom_gpu_build_P(self.h, precision, _ptr(P)), "build_P")
+        return P
+
+    def set_P(self, P):
+        P = _mat(P)
+        self._ck(self.lib.imcom_gpu_set_P(self.h, _ptr(P)), "set_P")
+
+    # ---- imcom_bisect ---------------------------------------------------
+    def bisect_kappa(self, UnotS=None, US=None, k_min=None, k_max=None, tol=None, maxnits=None,
+                     bracket=None):
+        t = self.tile
+        UnotS = t.USB if UnotS is None else UnotS
+        US = t.US_max if US is None else US
+        k_min = t.kappa_min if k_min is None else k_min
+        k_max = t.kappa_max if k_max is None else k_max
+        tol = t.US_tol if tol is None else tol
+        maxnits = t.maxNbis if maxnits is None else maxnits
+        m = self.m
+        K, USa, ev = np.empty(m), np.empty(m), np.empty(m, dtype=np.int32)
+        br = None if bracket is None else np.ascontiguousarray(bracket, dtype=np.float32)
+        self._ck(self.lib.imcom_gpu_bisect_kappa(self.h, 1 if UnotS else 0, US, k_min, k_max, tol,
+                                                 maxnits, _ptr(br, _fp), _ptr(K), _ptr(USa),
+                                                 _ptr(ev, _ip)), "bisect_kappa")
+        return K, USa, ev
+
+    def build_T(self, precision=IMCOM_PREC_FP64, out=True):
+        T = np.empty((self.n, self.m), order="F") if out else None
+        self._ck(self.lib.imcom_gpu_build_T(self.h, precision, _ptr(T)), "build_T")
+        return T
+
+    def build_S(self):
+        S = np.empty(self.m)
+        self._ck(self.lib.imcom_gpu_build_S(self.h, _ptr(S)), "build_S")
+        return S
+
+    def build_U(self):
+        U = np.empty(self.m)
+        self._ck(self.lib.imcom_gpu_build_U(self.h, _ptr(U)), "build_U")
+        return U
+
+    def build_image(self, I_i=None):
+        I_i = _vec(self.tile.I_i if I_i is None else I_i)
+        H = np.empty(self.m)
+        self._ck(self.lib.imcom_gpu_build_image(self.h, _ptr(I_i), _ptr(H)), "build_image")
+        return H
+
+    def synchronize(self):
+        self._ck(self.lib.imcom_gpu_synchronize(self.h), "synchronize")
+
+    # ---- the driver's sequence (imcom.f90:151-169) ----------------------
+    def solve(self, tile=None, precision_P=IMCOM_PREC_FP64, precision_T=IMCOM_PREC_FP64,
+              bracket=None, want_T=True):
+        """get_A, get_QL, get_B, get_P, get_T (bisect, T, S or U), build_image."""
+        if tile is not None:
+            self.set_tile(tile)
+        t = self.tile
+        self.lookup_A(t.npoly, t.npad)
+        _, L, cond = self.eigen_A(out=False)
+        self.lookup_B(t.npoly, t.npad)
+        self.build_P(precision_P)
+        K, US, ev = self.bisect_kappa(bracket=bracket)
+        T = self.build_T(precision_T, out=want_T)
+        if t.USB:
+            U, S = US, self.build_S()
+        else:
+            S, U = US, self.build_U()
+        H = self.build_image()
+        return dict(K=K, U=U, S=S, T=T, H=H, evals=ev, cond=cond)

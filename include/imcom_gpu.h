@@ -1,0 +1,123 @@
+/*
+ * imcom_gpu.h -- C ABI of the MI355X solve path of IMCOM.
+ *
+ * One context per GPU tile stream.  The context owns the device copies of the
+ * tile's inputs and of every intermediate of the solve path (A, B, Q, L, P,
+ * K, T); each entry point below replaces one procedure of the reference's
+ * imcom_matrix / imcom_bisect modules and keeps its argument meaning.  All
+ * matrices are Fortran column-major with leading dimension n; exposure ids
+ * are 1-based; all host buffers are plain pointers.  A NULL output pointer
+ * leaves that stage's result on the device only (the reference writes it to
+ * a FITS cache only when the file name is not 'None').
+ *
+ * Every function returns IMCOM_OK (0) or an error code; the message is in
+ * imcom_gpu_last_error().  Calls on one context must come from one host
+ * thread at a time; the context's work runs in order on its own HIP stream
+ * and every call that returns host data synchronises.
+ */
+#ifndef IMCOM_GPU_H
+#define IMCOM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct imcom_gpu_ctx imcom_gpu_ctx;
+
+enum {
+  IMCOM_OK = 0,
+  IMCOM_EARG = 1,   /* bad argument (sizes, ids, NULL where required) */
+  IMCOM_EHIP = 2,   /* HIP runtime error */
+  IMCOM_ESTATE = 3, /* a stage was called before the stages it needs */
+  IMCOM_ENOMEM = 4  /* device allocation failed */
+};
+
+/* arithmetic of the Q^T B and Q W products */
+enum {
+  IMCOM_PREC_FP64 = 0,   /* FP64 matrix cores */
+  IMCOM_PREC_FP32X3 = 1, /* split FP32 x3 on FP32 MFMA, FP64 recombination */
+  IMCOM_PREC_BF16X6 = 2  /* split BF16 x6 on BF16 MFMA, FP64 recombination */
+};
+
+/* Context lifetime.  No reference counterpart (the reference keeps module
+ * globals, imcom_data.f90:24-132). */
+int imcom_gpu_init(imcom_gpu_ctx** ctx, int device);
+void imcom_gpu_finalize(imcom_gpu_ctx* ctx);
+const char* imcom_gpu_last_error(const imcom_gpu_ctx* ctx);
+int imcom_gpu_synchronize(imcom_gpu_ctx* ctx);
+
+/* Tile inputs: the imcom_data globals the get_* procedures read
+ * (x_i, y_i, exp_i, X_a, Y_a, Ndiag, C_a; imcom_data.f90:95-132).  exp_i must
+ * be exposure-major (non-decreasing), as imcom_alloc_gims builds it
+ * (imcom_io.f90:84-123). */
+int imcom_gpu_set_tile(imcom_gpu_ctx* ctx, int64_t n, int64_t m, const double* x_i,
+                       const double* y_i, const int32_t* exp_i, const double* X_a,
+                       const double* Y_a, const double* Ndiag, double C_a);
+
+/* Lookup tables built on the host by imcom_build_Alookup / imcom_build_Blookup
+ * (imcom_matrix.f90:361-493): Alookup is (n1, n2, nexppsf*(nexppsf+1)/2),
+ * Blookup is (n1, n2, nexppsf), psfconst as imcom_test_psfconst returns it. */
+int imcom_gpu_set_tables(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1, int n2,
+                         const double* Alookup, const double* Blookup);
+
+/* imcom_lookup_A(npoly, npad, saveA)  (imcom_matrix.f90:206-284).
+ * A_out (n*n) may be NULL.  npoly <= 7 (stencil of at most 8 points). */
+int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale,
+                       double psfyscale, double* A_out);
+/* imcom_get_A cache hit: A read from AFILE (imcom_matrix.f90:49-68). */
+int imcom_gpu_set_A(imcom_gpu_ctx* ctx, const double* A);
+
+/* imcom_eigen_A(saveQL)  (imcom_matrix.f90:586-678): Q (n*n) and L (n) in
+ * descending eigenvalue order, condition = max(L)/min(L).  Consumes the
+ * device copy of A.  Q_out, L_out, condition may be NULL. */
+int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* condition);
+/* The two stages inside imcom_eigen_A's DSYEVR call (imcom_matrix.f90:642),
+ * exposed for staged checks: the Householder reduction of A to tridiagonal
+ * form (d: n, e: n-1; consumes the device copy of A), and the tridiagonal
+ * eigensolver on host (d, e) returning ascending eigenvalues lam (n) and the
+ * eigenvectors Z (n*n) of the tridiagonal matrix. */
+int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out);
+int imcom_gpu_tridiag_eig(imcom_gpu_ctx* ctx, int64_t n, const double* d, const double* e,
+                          double* lam_out, double* Z_out);
+/* imcom_get_QL cache hit: Q, L read from QFILE/LFILE (imcom_matrix.f90:164-201). */
+int imcom_gpu_set_QL(imcom_gpu_ctx* ctx, const double* Q, const double* L);
+
+/* imcom_lookup_B(npoly, npad, saveB)  (imcom_matrix.f90:288-357); B_out (n*m) may be NULL. */
+int imcom_gpu_lookup_B(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale,
+                       double psfyscale, double* B_out);
+/* imcom_get_B cache hit (imcom_matrix.f90:85-108). */
+int imcom_gpu_set_B(imcom_gpu_ctx* ctx, const double* B);
+
+/* imcom_build_P(saveP)  (imcom_matrix.f90:682-712): P = Q^T B; P_out (n*m)
+ * may be NULL.  P2 = P*P is formed on the fly where it is used. */
+int imcom_gpu_build_P(imcom_gpu_ctx* ctx, int precision, double* P_out);
+/* imcom_get_P cache hit (imcom_matrix.f90:122-147). */
+int imcom_gpu_set_P(imcom_gpu_ctx* ctx, const double* P);
+
+/* imcom_bisect_kappa(UnotS, US, k_min, k_max, tol, maxnits)
+ * (imcom_bisect.f90:78-170): K_out (m) and US_out (m, U or S at the root).
+ * evals_out (m, may be NULL) counts U/S evaluations per pixel.  bracket
+ * (2*m floats, log10 kappa low/high per pixel, may be NULL) starts each
+ * pixel's bisection from a predicted bracket; a bracket that does not hold
+ * the root falls back to the full range. */
+int imcom_gpu_bisect_kappa(imcom_gpu_ctx* ctx, int UnotS, double US_max, double k_min,
+                           double k_max, double tol, int maxnits, const float* bracket,
+                           double* K_out, double* US_out, int32_t* evals_out);
+
+/* imcom_build_T  (imcom_matrix.f90:716-750): T = -1/2 Q (P / (L + K_a));
+ * T_out (n*m) may be NULL. */
+int imcom_gpu_build_T(imcom_gpu_ctx* ctx, int precision, double* T_out);
+/* imcom_build_S  (imcom_matrix.f90:839-877): S_a = sum_i T_ia^2 Ndiag(i). */
+int imcom_gpu_build_S(imcom_gpu_ctx* ctx, double* S_out);
+/* imcom_build_U  (imcom_matrix.f90:795-835): U_a = U(K_a). */
+int imcom_gpu_build_U(imcom_gpu_ctx* ctx, double* U_out);
+/* imcom_build_image  (imcom_matrix.f90:754-790): H = T^T I. */
+int imcom_gpu_build_image(imcom_gpu_ctx* ctx, const double* I_i, double* H_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IMCOM_GPU_H */

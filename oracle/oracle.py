@@ -1,0 +1,226 @@
+"""CPU oracle of the IMCOM solve path -- TEST INFRASTRUCTURE ONLY.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+leg, as the checker.  The GPU library never calls it.
+
+Pieces and what follows the reference (file:line):
+  lookup_A / lookup_B   C restatement (imcom_oracle.c) of imcom_lookup_A/B and
+                        the Neville interpolation imcom_interp_lookup ->
+                        imcom_polin2 -> imcom_polint
+                        (imcom_matrix.f90:206-357,497-542; imcom_proc.f90:351-429)
+  eigen_A               LAPACK DSYEVR('V','A','U', abstol=DLAMCH('S')) through
+                        scipy's OpenBLAS 0.3.29, reversed to descending order
+                        (imcom_matrix.f90:586-678) -- the reference's own routine
+  build_P               DGEMM('T','N') P = Q^T B, P2 = P*P (imcom_matrix.f90:682-712)
+  bisect_kappa          C restatement of imcom_bisect_kappa/by_U/by_S
+                        (imcom_bisect.f90:78-267) over imcom_U/S_from_kappa
+                        (imcom_matrix.f90:904-970)
+  build_T/U/S/image     C restatements (imcom_matrix.f90:716-877)
+
+How each piece is pinned (tests/test_oracle_*.py):
+  interp        exact reproduction of degree<=7 polynomials sampled on a table
+                (8-point and edge-clipped stencils), exact node values, and
+                the Gaussian closed form A(D) = s^2/(4 pi sig^2) exp(-D^2/4sig^2)
+  lookup tables Gaussian closed forms for A, B = -2 x cross-correlation, and
+                Parseval for C
+  eigen_A       ||AQ - QL|| and ||Q^TQ - I|| at n*eps; numpy eigvalsh
+  build_P       brute-force sum
+  U/S(kappa)    the identity U = (C + T^T A T + B^T T)/C with T from the
+                direct solve T = -1/2 (A + kappa I)^-1 B; S = sum T^2 N
+  bisection     bracket invariants U(k) < U_max <= U(k*10^dx) or k = k_min,
+                and an independent Python restatement on small inputs
+  build_T       T = -1/2 (A + kappa I)^-1 B by LAPACK solve
+Summation order: DDOT/DGEMV/DGEMM sums are restated as ordered sums; the
+reference's OpenBLAS kernels block their sums differently (last-bit level).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+def build(force=False):
+    """Compile the C restatement into oracle/liboracle.so."""
+    so = os.path.join(_HERE, "liboracle.so")
+    src = os.path.join(_HERE, "imcom_oracle.c")
+    if force or not os.path.exists(so) or os.path.getmtime(so) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "liboracle.so"])
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        L = ctypes.CDLL(so)
+        L.or_interp_lookup.restype = ctypes.c_double
+        L.or_interp_lookup.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int]
+        L.or_U_from_kappa.restype = ctypes.c_double
+        L.or_S_from_kappa.restype = ctypes.c_double
+        L.or_bisect_by_U.restype = ctypes.c_int
+        L.or_bisect_by_S.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f(a, dtype=np.float64):
+    return np.asfortranarray(a, dtype=dtype)
+
+
+def interp_lookup(table, xarg, yarg, npoly=7):
+    t = _f(table)
+    return lib().or_interp_lookup(t.shape[0], t.shape[1], _p(t), float(xarg), float(yarg), npoly)
+
+
+def lookup_A(tile):
+    n = tile.n
+    A = np.zeros((n, n), order="F")
+    tab = _f(tile.Alookup)
+    ex = np.ascontiguousarray(tile.exp_i, dtype=np.int32)
+    x, y = np.ascontiguousarray(tile.x_i), np.ascontiguousarray(tile.y_i)
+    lib().or_lookup_A(ctypes.c_int64(n), _p(x), _p(y), ex.ctypes.data_as(_ip),
+                      ctypes.c_int(tile.psfconst), ctypes.c_int(tab.shape[0]),
+                      ctypes.c_int(tab.shape[1]), _p(tab), ctypes.c_int(tile.npad),
+                      ctypes.c_int(tile.npoly), ctypes.c_double(tile.psfxscale),
+                      ctypes.c_double(tile.psfyscale), _p(A))
+    return A
+
+
+def lookup_B(tile):
+    n, m = tile.n, tile.m
+    B = np.zeros((n, m), order="F")
+    tab = _f(tile.Blookup)
+    ex = np.ascontiguousarray(tile.exp_i, dtype=np.int32)
+    arrs = [np.ascontiguousarray(v) for v in (tile.x_i, tile.y_i, tile.X_a, tile.Y_a)]
+    lib().or_lookup_B(ctypes.c_int64(n), ctypes.c_int64(m), _p(arrs[0]), _p(arrs[1]),
+                      ex.ctypes.data_as(_ip), _p(arrs[2]), _p(arrs[3]),
+                      ctypes.c_int(tile.psfconst), ctypes.c_int(tab.shape[0]),
+                      ctypes.c_int(tab.shape[1]), _p(tab), ctypes.c_int(tile.npad),
+                      ctypes.c_int(tile.npoly), ctypes.c_double(tile.psfxscale),
+                      ctypes.c_double(tile.psfyscale), _p(B))
+    return B
+
+
+def eigen_A(A):
+    """DSYEVR('V','A','U') with abstol = DLAMCH('Safe minimum'), reversed to
+    descending eigenvalues (imcom_matrix.f90:586-678).  Returns (Q, L, cond)."""
+    from scipy.linalg import lapack
+    w, z, mfound, _isuppz, info = lapack.dsyevr(_f(A), compute_v=1, range="A", lower=0,
+                                                abstol=np.finfo(np.float64).tiny)
+    if info != 0:
+        raise RuntimeError(f"DSYEVR info={info}")
+    L = w[::-1].copy()
+    Q = np.asfortranarray(z[:, ::-1])
+    return Q, L, float(L.max() / L.min())
+
+
+def build_P(Q, B):
+    P = np.asfortranarray(Q.T @ B)
+    return P, P * P
+
+
+def bisect_kappa(L, P2, C_a, US, k_min, k_max, tol, maxnits=53, UnotS=True, Ndiag=None):
+    n, m = P2.shape
+    P2 = _f(P2)
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    Nd = np.ascontiguousarray(Ndiag if Ndiag is not None else np.zeros(n), dtype=np.float64)
+    K = np.zeros(m)
+    US_a = np.zeros(m)
+    ev = np.zeros(m, dtype=np.int32)
+    lib().or_bisect_kappa(ctypes.c_int(1 if UnotS else 0), ctypes.c_int64(n),
+                          ctypes.c_int64(m), _p(L), _p(P2), _p(Nd), ctypes.c_double(C_a),
+                          ctypes.c_double(US), ctypes.c_double(k_min), ctypes.c_double(k_max),
+                          ctypes.c_double(tol), ctypes.c_int(maxnits), _p(K), _p(US_a),
+                          ev.ctypes.data_as(_ip))
+    return K, US_a, ev
+
+
+def U_from_kappa(L, P2col, C_a, k):
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    p = np.ascontiguousarray(P2col, dtype=np.float64)
+    return lib().or_U_from_kappa(ctypes.c_int64(L.shape[0]), _p(L), _p(p),
+                                 ctypes.c_double(C_a), ctypes.c_double(k))
+
+
+def S_from_kappa(L, P2col, Ndiag, k):
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    p = np.ascontiguousarray(P2col, dtype=np.float64)
+    Nd = np.ascontiguousarray(Ndiag, dtype=np.float64)
+    return lib().or_S_from_kappa(ctypes.c_int64(L.shape[0]), _p(L), _p(p), _p(Nd),
+                                 ctypes.c_double(k))
+
+
+def build_T(Q, P, L, K):
+    n, m = P.shape
+    T = np.zeros((n, m), order="F")
+    Q, P = _f(Q), _f(P)
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    lib().or_build_T(ctypes.c_int64(n), ctypes.c_int64(m), _p(Q), _p(P), _p(L), _p(K), _p(T))
+    return T
+
+
+def build_image(T, I_i):
+    n, m = T.shape
+    H = np.zeros(m)
+    T = _f(T)
+    I_i = np.ascontiguousarray(I_i, dtype=np.float64)
+    lib().or_build_image(ctypes.c_int64(n), ctypes.c_int64(m), _p(T), _p(I_i), _p(H))
+    return H
+
+
+def build_U(L, P2, C_a, K):
+    n, m = P2.shape
+    U = np.zeros(m)
+    P2 = _f(P2)
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    K = np.ascontiguousarray(K, dtype=np.float64)
+    lib().or_build_U(ctypes.c_int64(n), ctypes.c_int64(m), _p(L), _p(P2), ctypes.c_double(C_a),
+                     _p(K), _p(U))
+    return U
+
+
+def build_S(T, Ndiag):
+    n, m = T.shape
+    S = np.zeros(m)
+    T = _f(T)
+    Nd = np.ascontiguousarray(Ndiag, dtype=np.float64)
+    lib().or_build_S(ctypes.c_int64(n), ctypes.c_int64(m), _p(T), _p(Nd), _p(S))
+    return S
+
+
+def solve(tile, Q=None, L=None):
+    """The whole reference solve path on one tile (imcom.f90:151-169).
+    Returns a dict of every stage's output."""
+    A = lookup_A(tile)
+    if Q is None:
+        Q, L, cond = eigen_A(A)
+    else:
+        cond = float(L.max() / L.min())
+    B = lookup_B(tile)
+    P, P2 = build_P(Q, B)
+    K, US, ev = bisect_kappa(L, P2, tile.C_a, tile.US_max, tile.kappa_min, tile.kappa_max,
+                             tile.US_tol, tile.maxNbis, tile.USB, tile.Ndiag)
+    T = build_T(Q, P, L, K)
+    if tile.USB:
+        U, S = US, build_S(T, tile.Ndiag)
+    else:
+        S, U = US, build_U(L, P2, tile.C_a, K)
+    H = build_image(T, tile.I_i)
+    return dict(A=A, B=B, Q=Q, L=L, cond=cond, P=P, P2=P2, K=K, U=U, S=S, T=T, H=H, evals=ev)

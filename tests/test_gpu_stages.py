@@ -1,0 +1,231 @@
+"""GPU parity of each solve-path stage against the CPU oracle, through the C ABI.
+
+Tolerances (FP64 path):
+  A, B           |gpu - oracle| <= 1e-13 * max|oracle|   (Lagrange vs Neville form)
+  P = Q^T B      |gpu - numpy| <= 1e-13 * (|Q|^T |B|) entrywise bound
+  kappa          identical to the oracle's in every pixel where no bisection
+                 decision is within rounding of U_max (checked: >= 99% equal,
+                 the rest within one bisection step and |U - U_max| < tol)
+  U, S           |gpu - oracle| <= 1e-12 (U is ~1e-6; S relative 1e-10)
+  T, H           relative 1e-10 at the same (Q, L, P, kappa)
+  eigen          ||AQ - QL|| <= 1e3 n eps ||A||, ||Q^T Q - I|| <= 1e3 n eps,
+                 eigenvalues within 1e3 n eps ||A|| of LAPACK DSYEVR
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from imcom_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+def _say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def small_tile():
+    return synth.small()
+
+
+@pytest.fixture(scope="module")
+def small_ref(small_tile):
+    return O.solve(small_tile)
+
+
+@pytest.mark.parametrize("n,m", [(37, 23), (300, 200), (1024, 576), (4096, 2048)])
+def test_gemm_P_random(gpu_ctx, n, m):
+    _say(f"gemm P=Q^T B n={n} m={m}")
+    rng = np.random.default_rng(n + m)
+    t = synth.tiny()
+    t.n, t.m = n, m
+    t.x_i = rng.uniform(0, 8, n)
+    t.y_i = rng.uniform(0, 8, n)
+    t.exp_i = np.ones(n, dtype=np.int32)
+    t.X_a = rng.uniform(0, 8, m)
+    t.Y_a = rng.uniform(0, 8, m)
+    t.Ndiag = np.full(n, 1e-6)
+    gpu_ctx.set_tile(t)
+    Q = np.asfortranarray(rng.standard_normal((n, n)))
+    B = np.asfortranarray(rng.standard_normal((n, m)))
+    L = np.sort(rng.uniform(0.1, 1.0, n))[::-1].copy()
+    gpu_ctx.set_QL(Q, L)
+    gpu_ctx.set_B(B)
+    P = gpu_ctx.build_P(out=True)
+    ref = Q.T @ B
+    bound = np.abs(Q).T @ np.abs(B)
+    assert np.all(np.abs(P - ref) <= 1e-13 * bound + 1e-300)
+
+
+def test_lookup_A_B_tiny(gpu_ctx):
+    _say("lookup A/B tiny")
+    t = synth.tiny()
+    gpu_ctx.set_tile(t)
+    A = gpu_ctx.lookup_A(out=True)
+    B = gpu_ctx.lookup_B(out=True)
+    Ar = O.lookup_A(t)
+    Br = O.lookup_B(t)
+    assert np.array_equal(A, A.T)
+    assert np.max(np.abs(A - Ar)) <= 1e-13 * np.max(np.abs(Ar))
+    assert np.max(np.abs(B - Br)) <= 1e-13 * np.max(np.abs(Br))
+
+
+def test_lookup_A_B_small(gpu_ctx, small_tile, small_ref):
+    _say("lookup A/B small n=1024 m=576")
+    gpu_ctx.set_tile(small_tile)
+    A = gpu_ctx.lookup_A(out=True)
+    B = gpu_ctx.lookup_B(out=True)
+    assert np.array_equal(A, A.T)
+    assert np.max(np.abs(A - small_ref["A"])) <= 1e-13 * np.max(np.abs(small_ref["A"]))
+    assert np.max(np.abs(B - small_ref["B"])) <= 1e-13 * np.max(np.abs(small_ref["B"]))
+
+
+def test_bisect_at_oracle_inputs(gpu_ctx, small_tile, small_ref):
+    _say("bisect U-mode at oracle (L, P)")
+    t, r = small_tile, small_ref
+    gpu_ctx.set_tile(t)
+    gpu_ctx.set_QL(r["Q"], r["L"])
+    gpu_ctx.set_P(r["P"])
+    K, U, ev = gpu_ctx.bisect_kappa()
+    # same decision path (evaluation count) and kappa up to the last bits of
+    # 10**root (device exp10 vs host pow)
+    same = (ev == r["evals"]) & (np.abs(K / r["K"] - 1.0) <= 1e-14)
+    assert same.mean() >= 0.99, f"kappa equal in {same.mean():.3f} of pixels"
+    assert np.max(np.abs(U - r["U"])) <= 1e-12
+    assert np.all(U < t.US_max)
+    # T, S, H at identical inputs
+    T = gpu_ctx.build_T()
+    Tr = O.build_T(r["Q"], r["P"], r["L"], K)
+    assert np.max(np.abs(T - Tr)) <= 1e-10 * np.max(np.abs(Tr))
+    S = gpu_ctx.build_S()
+    Sr = O.build_S(Tr, t.Ndiag)
+    assert np.max(np.abs(S - Sr) / np.abs(Sr)) <= 1e-10
+    H = gpu_ctx.build_image()
+    Hr = O.build_image(Tr, t.I_i)
+    assert np.max(np.abs(H - Hr)) <= 1e-10 * np.max(np.abs(Hr))
+
+
+def test_bisect_S_mode(gpu_ctx, small_tile, small_ref):
+    _say("bisect S-mode at oracle (L, P)")
+    t, r = small_tile, small_ref
+    gpu_ctx.set_tile(t)
+    gpu_ctx.set_QL(r["Q"], r["L"])
+    gpu_ctx.set_P(r["P"])
+    Smax = 2.0 * np.median(r["S"])
+    K, S, ev = gpu_ctx.bisect_kappa(UnotS=False, US=Smax, tol=1e-3 * Smax)
+    Kr, Sr, evr = O.bisect_kappa(r["L"], r["P2"], t.C_a, Smax, t.kappa_min, t.kappa_max,
+                                 1e-3 * Smax, 53, UnotS=False, Ndiag=t.Ndiag)
+    same = (ev == evr) & (np.abs(K / Kr - 1.0) <= 1e-14)
+    assert same.mean() >= 0.99, f"kappa equal in {same.mean():.3f} of pixels"
+    assert np.max(np.abs(S - Sr) / Sr) <= 1e-10
+    U = gpu_ctx.build_U()
+    Ur = O.build_U(r["L"], r["P2"], t.C_a, K)
+    assert np.max(np.abs(U - Ur)) <= 1e-12
+
+
+def _tridiag(d, e):
+    return np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+
+
+@pytest.mark.parametrize("kind,n", [("random", 1), ("random", 2), ("random", 7), ("random", 100),
+                                    ("random", 1000), ("clustered", 513), ("graded", 800),
+                                    ("wilkinson", 201), ("zero_e", 64)])
+def test_tridiag_eig(gpu_ctx, kind, n):
+    _say(f"tridiag_eig {kind} n={n}")
+    rng = np.random.default_rng(n)
+    if kind == "random":
+        d, e = rng.standard_normal(n), rng.standard_normal(max(n - 1, 0))
+    elif kind == "clustered":
+        d = np.concatenate([np.full(n // 2, 1.0), rng.uniform(0, 1e-12, n - n // 2)])
+        e = rng.uniform(0, 1e-9, n - 1)
+    elif kind == "graded":
+        d = 10.0 ** rng.uniform(-16, 0, n)
+        e = 10.0 ** rng.uniform(-17, -1, n - 1)
+    elif kind == "wilkinson":
+        m = (n - 1) // 2
+        d = np.abs(np.arange(-m, m + 1, dtype=float))
+        e = np.ones(n - 1)
+    else:
+        d, e = rng.standard_normal(n), np.zeros(n - 1)
+    lam, Z = gpu_ctx.tridiag_eig(d, e)
+    Tm = _tridiag(d, e)
+    nrm = max(np.max(np.abs(Tm)), 1e-300)
+    ref = np.linalg.eigvalsh(Tm)
+    assert np.all(np.diff(lam) >= 0)
+    assert np.max(np.abs(lam - ref)) <= 1e3 * n * EPS * nrm
+    assert np.max(np.abs(Z.T @ Z - np.eye(n))) <= 1e3 * n * EPS
+    assert np.max(np.abs(Tm @ Z - Z * lam)) <= 1e3 * n * EPS * nrm
+
+
+def test_tridiagonalize_small(gpu_ctx, small_tile, small_ref):
+    _say("tridiagonalize small")
+    gpu_ctx.set_tile(small_tile)
+    gpu_ctx.set_A(small_ref["A"])
+    d, e = gpu_ctx.tridiagonalize()
+    lam = np.linalg.eigvalsh(_tridiag(d, e))
+    ref = np.sort(small_ref["L"])
+    n = small_tile.n
+    nrm = np.max(np.abs(small_ref["A"]))
+    assert np.max(np.abs(lam - ref)) <= 1e3 * n * EPS * nrm
+
+
+@pytest.mark.parametrize("n", [5, 130, 700])
+def test_eigen_random_spd(gpu_ctx, n):
+    _say(f"eigen_A random n={n}")
+    rng = np.random.default_rng(3 * n)
+    X = rng.standard_normal((n, n))
+    A = np.asfortranarray(X @ X.T / n)
+    t = synth.tiny()
+    t.n, t.m = n, 4
+    t.x_i, t.y_i = np.zeros(n), np.zeros(n)
+    t.exp_i = np.ones(n, dtype=np.int32)
+    t.X_a, t.Y_a = np.zeros(4), np.zeros(4)
+    t.Ndiag = np.ones(n)
+    gpu_ctx.set_tile(t)
+    gpu_ctx.set_A(A)
+    Q, L, cond = gpu_ctx.eigen_A()
+    nrm = np.max(np.abs(A))
+    assert np.all(np.diff(L) <= 0)
+    assert np.max(np.abs(A @ Q - Q * L)) <= 1e3 * n * EPS * nrm
+    assert np.max(np.abs(Q.T @ Q - np.eye(n))) <= 1e3 * n * EPS
+    _, Lr, _ = O.eigen_A(A)
+    assert np.max(np.abs(L - Lr)) <= 1e3 * n * EPS * nrm
+
+
+def test_eigen_small(gpu_ctx, small_tile, small_ref):
+    _say("eigen_A small n=1024")
+    gpu_ctx.set_tile(small_tile)
+    A = small_ref["A"]
+    gpu_ctx.set_A(A)
+    Q, L, cond = gpu_ctx.eigen_A()
+    n = small_tile.n
+    nrm = np.max(np.abs(A))
+    assert np.max(np.abs(A @ Q - Q * L)) <= 1e3 * n * EPS * nrm
+    assert np.max(np.abs(Q.T @ Q - np.eye(n))) <= 1e3 * n * EPS
+    assert np.max(np.abs(L - small_ref["L"])) <= 1e3 * n * EPS * nrm
+
+
+def test_solve_small_end_to_end(gpu_ctx, small_tile, small_ref):
+    """Whole path on the GPU against the oracle (LAPACK eigensolver): T, U, S,
+    H compared through eigenvector-free quantities."""
+    _say("solve small end-to-end")
+    t, r = small_tile, small_ref
+    out = gpu_ctx.solve(t)
+    assert np.all(out["U"] < t.US_max)
+    assert np.max(np.abs(out["U"] - r["U"])) <= 1e-11
+    # kappa: same bisection decisions in almost every pixel
+    close = np.abs(out["K"] / r["K"] - 1.0) <= 1e-12
+    assert close.mean() >= 0.95
+    # leakage identity from T itself: U = (C + T^T A T + B^T T) / C
+    A, B, T = r["A"], r["B"], out["T"]
+    Uid = (t.C_a + np.einsum("ia,ia->a", T, A @ T) + np.einsum("ia,ia->a", B, T)) / t.C_a
+    assert np.max(np.abs(Uid - out["U"])) <= 1e-10
+    same = close
+    relT = np.max(np.abs(T[:, same] - r["T"][:, same])) / np.max(np.abs(r["T"]))
+    assert relT <= 1e-8, relT
+    assert np.max(np.abs(out["S"] - r["S"]) / r["S"]) <= 1e-8

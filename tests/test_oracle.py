@@ -1,0 +1,244 @@
+"""Pins the CPU oracle (oracle/) and the host-side set-up (imcom_amd/tile.py)
+against closed forms, brute force and the mathematics, so that a dropped
+term, a wrong sign, a wrong index or a transposed operand fails here.
+CPU only."""
+import numpy as np
+import pytest
+
+from imcom_amd import synth, tile as TL
+from oracle import oracle as O
+
+EPS = 2.0 ** -53
+
+
+# ---------------------------------------------------------------- interpolation
+def _poly_table(n1, n2, cx, cy):
+    i = np.arange(1, n1 + 1, dtype=np.float64)
+    j = np.arange(1, n2 + 1, dtype=np.float64)
+    px = np.polynomial.polynomial.polyval(i, cx)
+    py = np.polynomial.polynomial.polyval(j, cy)
+    return np.asfortranarray(np.outer(px, py))
+
+
+@pytest.mark.parametrize("deg", [0, 3, 7])
+def test_interp_reproduces_polynomials(deg):
+    """8-point stencils reproduce any degree<=7 polynomial in each variable
+    (imcom_interp_lookup -> polin2 -> polint is exact for them)."""
+    rng = np.random.default_rng(deg)
+    n1, n2 = 40, 36
+    # scale coefficients so the polynomial stays O(1) on the table
+    cx = rng.standard_normal(deg + 1) / (20.0 ** np.arange(deg + 1))
+    cy = rng.standard_normal(deg + 1) / (18.0 ** np.arange(deg + 1))
+    T = _poly_table(n1, n2, cx, cy)
+    for _ in range(200):
+        xa = rng.uniform(-14.0, 14.0)
+        ya = rng.uniform(-12.0, 12.0)
+        v = O.interp_lookup(T, xa, ya)
+        iarg, jarg = n1 // 2 + 1 + xa, n2 // 2 + 1 + ya
+        ref = np.polynomial.polynomial.polyval(iarg, cx) * np.polynomial.polynomial.polyval(jarg, cy)
+        assert abs(v - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+def test_interp_node_values_and_outside():
+    rng = np.random.default_rng(1)
+    T = np.asfortranarray(rng.standard_normal((30, 30)))
+    # integer offsets land on nodes: exact table values (7-point stencil)
+    for dx, dy in [(0, 0), (3, -5), (-14, 10), (14, 14)]:
+        assert O.interp_lookup(T, dx, dy) == T[30 // 2 + dx, 30 // 2 + dy]
+    # outside the table -> 0
+    assert O.interp_lookup(T, 100.0, 0.0) == 0.0
+    assert O.interp_lookup(T, -15.5, 0.0) == 0.0
+
+
+def test_interp_edge_clipped_stencil():
+    """Near the edge the stencil is clipped to the table; a cubic is still
+    reproduced by the (>=5-point) clipped stencil."""
+    rng = np.random.default_rng(2)
+    cx = rng.standard_normal(4) / 10.0 ** np.arange(4)
+    cy = rng.standard_normal(4) / 10.0 ** np.arange(4)
+    T = _poly_table(20, 20, cx, cy)
+    for xa in [-9.2, -9.7, 8.3, 8.9]:
+        ya = 0.37
+        v = O.interp_lookup(T, xa, ya)
+        ref = np.polynomial.polynomial.polyval(11 + xa, cx) * np.polynomial.polynomial.polyval(11 + ya, cy)
+        assert abs(v - ref) <= 1e-11 * max(1.0, abs(ref))
+
+
+def test_neville_matches_lagrange_on_random_tables():
+    """The Neville tableau restatement equals an independent Lagrange
+    evaluation of the same stencil polynomial."""
+    rng = np.random.default_rng(3)
+    T = np.asfortranarray(rng.standard_normal((50, 50)))
+    for _ in range(100):
+        xa, ya = rng.uniform(-20, 20, 2)
+        iarg, jarg = 26 + xa, 26 + ya
+        i0, j0 = int(np.floor(iarg)) - 3, int(np.floor(jarg)) - 3
+        nx = np.arange(i0, i0 + 8, dtype=float)
+        ny = np.arange(j0, j0 + 8, dtype=float)
+        def w(nodes, x):
+            out = np.ones(8)
+            for k in range(8):
+                for l in range(8):
+                    if l != k:
+                        out[k] *= (x - nodes[l]) / (nodes[k] - nodes[l])
+            return out
+        ref = w(nx, iarg) @ T[i0 - 1:i0 + 7, j0 - 1:j0 + 7] @ w(ny, jarg)
+        assert abs(O.interp_lookup(T, xa, ya) - ref) <= 1e-12 * np.max(np.abs(T))
+
+
+# ------------------------------------------------------------ lookup tables
+def _gauss_G(npix, sigma, psfscale, shift=(0.0, 0.0)):
+    c = npix // 2
+    s = sigma / psfscale
+    i = np.arange(npix) - c - shift[0] / psfscale
+    j = np.arange(npix) - c - shift[1] / psfscale
+    return np.asfortranarray(np.exp(-0.5 * (i[:, None] ** 2 + j[None, :] ** 2) / s ** 2))
+
+
+def test_tables_gaussian_closed_form():
+    """Alookup is the autocorrelation s^2/(4 pi sig^2) exp(-D^2/(4 sig^2)),
+    Blookup is -2 x the Gamma-G cross-correlation, C = sum Gamma^2."""
+    ps, sg, sG = 0.25, 1.2, 1.5
+    n1pad = 256
+    G = TL.embed_psf(_gauss_G(128, sg, ps), n1pad)
+    Gam = TL.embed_psf(_gauss_G(128, sG, ps), n1pad)
+    Gt, Gamt = TL.ft_r2c(G), TL.ft_r2c(Gam)
+    Al = TL.build_Alookup([Gt], 3)[:, :, 0]
+    Bl = TL.build_Blookup(Gamt, [Gt], 3)[:, :, 0]
+    for D in [(0.0, 0.0), (0.37, -1.21), (2.5, 3.1), (-4.0, 0.66)]:
+        xa, ya = 3 * D[0] / ps, 3 * D[1] / ps
+        r2 = D[0] ** 2 + D[1] ** 2
+        a = O.interp_lookup(Al, xa, ya)
+        a_ref = ps ** 2 / (4 * np.pi * sg ** 2) * np.exp(-r2 / (4 * sg ** 2))
+        assert abs(a - a_ref) <= 1e-10 * a_ref
+        b = O.interp_lookup(Bl, xa, ya)
+        v = sg ** 2 + sG ** 2
+        b_ref = -2 * ps ** 2 / (2 * np.pi * v) * np.exp(-r2 / (2 * v))
+        assert abs(b - b_ref) <= 1e-10 * abs(b_ref)
+    C = TL.calc_C(Gamt)
+    assert abs(C - ps ** 2 / (4 * np.pi * sG ** 2)) <= 1e-12 * C
+
+
+def test_table_orientation_shifted_psf():
+    """For G_j displaced by +s the pair table peaks at D = +s: the table is
+    sum_y G_i(y) G_j(y + D) (conj(Gt_i) Gt_j, imcom_matrix.f90:405-409)."""
+    ps = 0.25
+    s = (0.75, -0.5)
+    Gi = TL.embed_psf(_gauss_G(128, 1.2, ps), 256)
+    Gj = TL.embed_psf(_gauss_G(128, 1.2, ps, shift=s), 256)
+    Al = TL.build_Alookup([TL.ft_r2c(Gi), TL.ft_r2c(Gj)], 3)
+    t12 = Al[:, :, 1]  # ial = 1 + 2*1/2 = 2 -> pair (1,2)
+    peak = O.interp_lookup(t12, 3 * s[0] / ps, 3 * s[1] / ps)
+    off = O.interp_lookup(t12, -3 * s[0] / ps, -3 * s[1] / ps)
+    assert peak > off
+    assert abs(peak - ps ** 2 / (4 * np.pi * 1.2 ** 2)) <= 1e-9 * peak
+
+
+def test_pad_size_and_ndiag():
+    x = np.array([0.0, 15.3])
+    X = np.array([2.0, 13.0])
+    assert TL.pad_size(x, x, X, X, 0.25, 0.25, 128, 128) == 256
+    assert TL.pad_size(x, x, X, X, 0.05, 0.05, 128, 128) == 2 * 306
+    Nd = TL.build_Ndiag(10, [1.0, 2.0, 3.0])
+    assert list(Nd) == [1, 1, 1, 2, 2, 2, 3, 3, 3, 3]
+    assert TL.test_psfconst(["a", "a"], [0.0, 0.0]) == 0
+    assert TL.test_psfconst(["a", "a"], [10.0, 20.0]) == 1
+    assert TL.test_psfconst(["a", "b"], [10.0, 20.0]) == 0
+
+
+# ------------------------------------------------------------- solve path
+@pytest.fixture(scope="module")
+def tiny_solution():
+    t = synth.tiny(nexp=2, npix_in=8, nout=8)
+    return t, O.solve(t)
+
+
+def test_lookup_A_symmetric_and_matches_interp(tiny_solution):
+    t, r = tiny_solution
+    A = r["A"]
+    assert np.array_equal(A, A.T)
+    i, j = 3, 77
+    D = (3 * (t.x_i[j] - t.x_i[i]) / 0.25, 3 * (t.y_i[j] - t.y_i[i]) / 0.25)
+    assert A[i, j] == O.interp_lookup(t.Alookup[:, :, 0], *D)
+    a = 5
+    Db = (3 * (t.x_i[i] - t.X_a[a]) / 0.25, 3 * (t.y_i[i] - t.Y_a[a]) / 0.25)
+    assert r["B"][i, a] == O.interp_lookup(t.Blookup[:, :, 0], *Db)
+
+
+def test_eigen_A_lapack(tiny_solution):
+    t, r = tiny_solution
+    A, Q, L = r["A"], r["Q"], r["L"]
+    n = t.n
+    assert np.all(np.diff(L) <= 0)
+    assert np.max(np.abs(A @ Q - Q * L)) <= 100 * n * EPS * np.max(np.abs(A))
+    assert np.max(np.abs(Q.T @ Q - np.eye(n))) <= 100 * n * EPS
+    assert np.max(np.abs(np.sort(L) - np.linalg.eigvalsh(A))) <= 100 * n * EPS * np.max(np.abs(A))
+    assert r["cond"] == L.max() / L.min()
+
+
+def test_P_brute_force(tiny_solution):
+    t, r = tiny_solution
+    Q, B, P = r["Q"], r["B"], r["P"]
+    a, i = 3, 17
+    assert abs(P[i, a] - sum(Q[k, i] * B[k, a] for k in range(t.n))) <= 1e-14 * np.abs(Q[:, i]) @ np.abs(B[:, a])
+    assert np.array_equal(r["P2"], P * P)
+
+
+def test_U_S_identities(tiny_solution):
+    """U(k) from the eigen form equals (C + T^T A T + B^T T)/C with T from the
+    direct solve T = -1/2 (A + k I)^-1 B; with uniform noise S(k) equals
+    sum_i T_i^2 N_i."""
+    t, r = tiny_solution
+    A, B, L, P2 = r["A"], r["B"], r["L"], r["P2"]
+    n = t.n
+    for a, k in [(0, 1e-6), (9, 3e-5), (40, 1e-3)]:
+        Tcol = -0.5 * np.linalg.solve(A + k * np.eye(n), B[:, a])
+        Uid = (t.C_a + Tcol @ A @ Tcol + B[:, a] @ Tcol) / t.C_a
+        U = O.U_from_kappa(L, P2[:, a], t.C_a, k)
+        assert abs(U - Uid) <= 1e-9 * max(abs(Uid), 1e-6) + 1e-13
+        Nd = np.full(n, 1e-6)
+        S = O.S_from_kappa(L, P2[:, a], Nd, k)
+        assert abs(S - np.sum(Tcol ** 2 * Nd)) <= 1e-8 * S
+
+
+def test_U_monotone_in_kappa(tiny_solution):
+    t, r = tiny_solution
+    ks = np.logspace(-12, 2, 60)
+    U = [O.U_from_kappa(r["L"], r["P2"][:, 4], t.C_a, k) for k in ks]
+    assert np.all(np.diff(U) > -1e-15)
+
+
+def test_bisection_bracket_invariant(tiny_solution):
+    """Every pixel: U(K) < U_max, and either U_max - U(K) < tol or the next
+    step up reaches U_max (the bisection found the largest such K), or
+    K = kappa_min when even kappa_min leaks too much."""
+    t, r = tiny_solution
+    span = np.log10(t.kappa_max) - np.log10(t.kappa_min)
+    for a in range(t.m):
+        K = r["K"][a]
+        U = O.U_from_kappa(r["L"], r["P2"][:, a], t.C_a, K)
+        if K == t.kappa_min and U >= t.US_max:
+            continue
+        assert U < t.US_max
+        assert U == r["U"][a]
+        if t.US_max - U >= t.US_tol:
+            h = span / 2.0 ** (r["evals"][a] - 1)
+            assert O.U_from_kappa(r["L"], r["P2"][:, a], t.C_a, K * 10 ** (2 * h)) >= t.US_max
+
+
+def test_T_direct_solve_and_maps(tiny_solution):
+    t, r = tiny_solution
+    A, B, T = r["A"], r["B"], r["T"]
+    n = t.n
+    for a in [0, 13, t.m - 1]:
+        Td = -0.5 * np.linalg.solve(A + r["K"][a] * np.eye(n), B[:, a])
+        assert np.max(np.abs(Td - T[:, a])) <= 1e-8 * np.max(np.abs(Td))
+    assert np.allclose(r["H"], T.T @ t.I_i, rtol=1e-13, atol=1e-18)
+    assert np.allclose(r["S"], (T * T * t.Ndiag[:, None]).sum(0), rtol=1e-13)
+
+
+def test_small_workload_reaches_leakage_target():
+    t = synth.small()
+    r = O.solve(t)
+    assert np.all(r["U"] < 1e-6)
+    assert t.n == 1024 and t.m == 576
