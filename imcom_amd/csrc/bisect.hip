@@ -29,7 +29,8 @@ __global__ void bisect_kernel(int64_t n, int64_t m, const double* __restrict__ L
                               const double* __restrict__ Nd, double C_a, double US_max,
                               double tol, double kmin, double kmax, int maxnits, int UnotS,
                               const float* __restrict__ bracket, double* __restrict__ K_out,
-                              double* __restrict__ US_out, int32_t* __restrict__ evals_out) {
+                              double* __restrict__ US_out, int32_t* __restrict__ evals_out,
+                              int32_t* __restrict__ hits_out) {
   __shared__ double red[16];
   const int64_t a = blockIdx.x;
   if (a >= m) return;
@@ -75,6 +76,7 @@ __global__ void bisect_kernel(int64_t n, int64_t m, const double* __restrict__ L
 
   int evals = 0;
   double k_root, out;
+  bool use_full = true;
   // U: start from kappa_min and search upward; S: from kappa_max downward
   const double kstart = UnotS ? kmin : kmax;
   const double kend = UnotS ? kmax : kmin;
@@ -85,7 +87,6 @@ __global__ void bisect_kernel(int64_t n, int64_t m, const double* __restrict__ L
     k_root = kstart;
   } else {
     double lo = log10(kstart), hi = log10(kend);
-    bool use_full = true;
     if (bracket != nullptr) {
       // learned bracket [blo, bhi] (log10 kappa), oriented like (lo, hi)
       double blo = (double)bracket[2 * a], bhi = (double)bracket[2 * a + 1];
@@ -111,7 +112,6 @@ __global__ void bisect_kernel(int64_t n, int64_t m, const double* __restrict__ L
         }
       }
     }
-    (void)use_full;
     double root = lo, dx = hi - lo;
     for (int it = 1; it <= maxnits; ++it) {
       dx = 0.5 * dx;
@@ -131,20 +131,21 @@ __global__ void bisect_kernel(int64_t n, int64_t m, const double* __restrict__ L
     K_out[a] = k_root;
     US_out[a] = out;
     if (evals_out) evals_out[a] = evals;
+    if (hits_out) hits_out[a] = use_full ? 0 : 1;
   }
 }
 
 hipError_t launch_bisect(hipStream_t st, int64_t n, int64_t m, const double* L, const double* P,
                          int64_t ldp, const double* Nd, double C_a, double US_max, double tol,
                          double kmin, double kmax, int maxnits, int UnotS, const float* bracket,
-                         double* K, double* US, int32_t* evals) {
+                         double* K, double* US, int32_t* evals, int32_t* hits) {
   int bs = 64;
   while (bs < 1024 && (n + bs - 1) / bs > 16) bs *= 2;
   const int64_t per = (n + bs - 1) / bs;
   dim3 grid((unsigned)m), block(bs);
 #define IMCOM_BISECT(PV)                                                                     \
   hipLaunchKernelGGL(bisect_kernel<PV>, grid, block, 0, st, n, m, L, P, ldp, Nd, C_a, US_max, \
-                     tol, kmin, kmax, maxnits, UnotS, bracket, K, US, evals)
+                     tol, kmin, kmax, maxnits, UnotS, bracket, K, US, evals, hits)
   if (per <= 4)
     IMCOM_BISECT(4);
   else if (per <= 8)

@@ -14,99 +14,7 @@
 
 using namespace imcom;
 
-namespace {
-
-constexpr int kNB = 32;    // tridiagonal-reduction panel width
-constexpr int kNBB = 128;  // back-transform block
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t cap = 0;
-  hipError_t reserve(size_t bytes) {
-    if (bytes <= cap && p) return hipSuccess;
-    if (p) {
-      hipError_t e = hipFree(p);
-      p = nullptr;
-      cap = 0;
-      if (e != hipSuccess) return e;
-    }
-    if (bytes == 0) bytes = 8;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e == hipSuccess) cap = bytes;
-    return e;
-  }
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(p);
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    cap = 0;
-  }
-};
-
-}  // namespace
-
-struct imcom_gpu_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  int64_t n = 0, m = 0;
-  double C_a = 0.0;
-  std::vector<int32_t> h_exp;
-  // tables
-  int psfconst = 1, nexppsf = 1, n1 = 0, n2 = 0;
-  bool have_tables = false;
-  // stage flags
-  bool have_tile = false, have_A = false, have_B = false, have_QL = false, have_P = false,
-       have_K = false, have_T = false;
-  // inputs
-  DevBuf x, y, ex, Xa, Ya, Nd, Atab, Btab, I;
-  // stages
-  DevBuf A, B, Q, L, P, K, US, T, W, H, S, U, evals, bracket;
-  // eigensolver workspace
-  DevBuf sW, sVW, sWV, sy, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb, sTb,
-      sY, sY2, slam, sZ, stile;
-  DcWork* dc = nullptr;
-  DevBuf split_work;
-};
-
-static int set_err(imcom_gpu_ctx* c, int code, const std::string& msg) {
-  if (c) c->err = msg;
-  return code;
-}
-
-static int hip_err(imcom_gpu_ctx* c, hipError_t e, const char* where) {
-  if (e == hipSuccess) return IMCOM_OK;
-  const int code = (e == hipErrorOutOfMemory) ? IMCOM_ENOMEM : IMCOM_EHIP;
-  return set_err(c, code, std::string(where) + ": " + hipGetErrorString(e));
-}
-
-#define CK(expr, where)                                   \
-  do {                                                    \
-    hipError_t _e = (expr);                               \
-    if (_e != hipSuccess) return hip_err(ctx, _e, where); \
-  } while (0)
-
-#define NEED(cond, code, msg) \
-  do {                        \
-    if (!(cond)) return set_err(ctx, code, msg); \
-  } while (0)
-
-static hipError_t upload(imcom_gpu_ctx* ctx, DevBuf& b, const void* h, size_t bytes) {
-  hipError_t e = b.reserve(bytes);
-  if (e != hipSuccess) return e;
-  if (bytes == 0) return hipSuccess;
-  return hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, ctx->stream);
-}
-
-static hipError_t download(imcom_gpu_ctx* ctx, void* h, const DevBuf& b, size_t bytes) {
-  if (!h || bytes == 0) return hipSuccess;
-  hipError_t e = hipMemcpyAsync(h, b.p, bytes, hipMemcpyDeviceToHost, ctx->stream);
-  if (e != hipSuccess) return e;
-  return hipStreamSynchronize(ctx->stream);
-}
+#include "ctx.h"
 
 // Q(:, j) = Z(:, n-1-j), L(j) = lam(n-1-j): ascending -> descending
 __global__ void reverse_cols_kernel(int64_t n, const double* __restrict__ Z, double* __restrict__ Q,
@@ -147,6 +55,7 @@ int imcom_gpu_init(imcom_gpu_ctx** out, int device) {
     return IMCOM_EHIP;
   }
   ctx->dc = dc_create();
+  ctx->prof.st = ctx->stream;
   *out = ctx;
   return IMCOM_OK;
 }
@@ -159,12 +68,13 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
                     &ctx->Atab,  &ctx->Btab, &ctx->I,      &ctx->A,     &ctx->B,    &ctx->Q,
                     &ctx->L,     &ctx->P,    &ctx->K,      &ctx->US,    &ctx->T,    &ctx->W,
                     &ctx->H,     &ctx->S,    &ctx->U,      &ctx->evals, &ctx->bracket,
-                    &ctx->sW,    &ctx->sVW,  &ctx->sWV,    &ctx->sy,    &ctx->swpre, &ctx->svtmp,
+                    &ctx->sW,    &ctx->sVW,  &ctx->sWV,    &ctx->sy,    &ctx->sycp, &ctx->swpre, &ctx->svtmp,
                     &ctx->sd,    &ctx->se,   &ctx->stau,   &ctx->spart, &ctx->spart2, &ctx->ss,
                     &ctx->sscal, &ctx->sVb,  &ctx->sSb,    &ctx->sTb,   &ctx->sY,   &ctx->sY2,
                     &ctx->slam,  &ctx->sZ,   &ctx->stile,  &ctx->split_work};
   for (DevBuf* b : bufs) b->release();
   dc_destroy(ctx->dc);
+  ctx->prof.destroy();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -253,6 +163,7 @@ int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   const int64_t n = ctx->n;
   CK(ctx->A.reserve((size_t)n * n * 8), "alloc A");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
+  StageTimer tm(ctx, PS_LOOKUP_A);
   CK(launch_lookup_A(ctx->stream, ctx->device, n, ctx->x.as<double>(), ctx->y.as<double>(),
                      ctx->ex.as<int32_t>(), ctx->psfconst, ctx->Atab.as<double>(),
                      (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
@@ -283,6 +194,7 @@ int imcom_gpu_lookup_B(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->B.reserve((size_t)n * m * 8), "alloc B");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
+  StageTimer tm(ctx, PS_LOOKUP_B);
   CK(launch_lookup_B(ctx->stream, ctx->device, n, m, ctx->x.as<double>(), ctx->y.as<double>(),
                      ctx->ex.as<int32_t>(), ctx->Xa.as<double>(), ctx->Ya.as<double>(),
                      ctx->psfconst, ctx->Btab.as<double>(), (int64_t)ctx->n1 * ctx->n2, ctx->n1,
@@ -308,7 +220,9 @@ static hipError_t reserve_eigen(imcom_gpu_ctx* ctx, int64_t n, SyTile& t) {
   IMCOM_HIP_TRY(ctx->sW.reserve(n * kNB * d));
   IMCOM_HIP_TRY(ctx->sVW.reserve(n * 2 * kNB * d));
   IMCOM_HIP_TRY(ctx->sWV.reserve(n * 2 * kNB * d));
-  IMCOM_HIP_TRY(ctx->sy.reserve(n * d));
+  const int64_t nbr = (n + 63) / 64;
+  IMCOM_HIP_TRY(ctx->sy.reserve((size_t)nbr * n * d));
+  IMCOM_HIP_TRY(ctx->sycp.reserve((size_t)((nbr + 7) / 8) * n * d));
   IMCOM_HIP_TRY(ctx->swpre.reserve(n * d));
   IMCOM_HIP_TRY(ctx->svtmp.reserve(n * d));
   IMCOM_HIP_TRY(ctx->sd.reserve(n * d));
@@ -333,7 +247,8 @@ static hipError_t reserve_eigen(imcom_gpu_ctx* ctx, int64_t n, SyTile& t) {
   t.W = ctx->sW.as<double>();
   t.VW = ctx->sVW.as<double>();
   t.WV = ctx->sWV.as<double>();
-  t.y = ctx->sy.as<double>();
+  t.ypart = ctx->sy.as<double>();
+  t.ycpart = ctx->sycp.as<double>();
   t.wpre = ctx->swpre.as<double>();
   t.vtmp = ctx->svtmp.as<double>();
   t.d = ctx->sd.as<double>();
@@ -364,17 +279,26 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
   std::vector<SyTile> host(1, t);
   const SyTile* dt = ctx->stile.as<SyTile>();
   // 1. A = H T H^T (reflectors left in A's lower part)
-  CK(sytrd_run(ctx->stream, dt, 1, n, host, kNB), "tridiagonal reduction");
+  {
+    StageTimer tm(ctx, PS_SYTRD);
+    CK(sytrd_run(ctx->stream, dt, 1, n, host, kNB, &ctx->prof), "tridiagonal reduction");
+  }
   // 2. T = Z diag(lam) Z^T, ascending
-  CK(dc_solve(ctx->stream, ctx->dc, n, ctx->sd.as<double>(), ctx->se.as<double>(),
-              ctx->slam.as<double>(), ctx->sZ.as<double>()),
-     "divide-and-conquer");
-  // 3. descending order into Q, L (imcom_matrix.f90:656-663)
-  CK(launch_reverse_cols(ctx->stream, n, ctx->sZ.as<double>(), ctx->Q.as<double>(),
-                         ctx->slam.as<double>(), ctx->L.as<double>()),
-     "reverse");
-  // 4. Q = H Q
-  CK(ormtr_run(ctx->stream, dt, 1, n, host, kNBB), "back-transform");
+  {
+    StageTimer tm(ctx, PS_STEDC);
+    CK(dc_solve(ctx->stream, ctx->dc, n, ctx->sd.as<double>(), ctx->se.as<double>(),
+                ctx->slam.as<double>(), ctx->sZ.as<double>()),
+       "divide-and-conquer");
+  }
+  {
+    StageTimer tm(ctx, PS_ORMTR);
+    // 3. descending order into Q, L (imcom_matrix.f90:656-663)
+    CK(launch_reverse_cols(ctx->stream, n, ctx->sZ.as<double>(), ctx->Q.as<double>(),
+                           ctx->slam.as<double>(), ctx->L.as<double>()),
+       "reverse");
+    // 4. Q = H Q
+    CK(ormtr_run(ctx->stream, dt, 1, n, host, kNBB), "back-transform");
+  }
   ctx->have_A = false;  // A was overwritten by the reduction
   ctx->have_QL = true;
   ctx->have_P = ctx->have_K = ctx->have_T = false;
@@ -403,7 +327,8 @@ int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out) {
   CK(hipMemcpyAsync(ctx->stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
      "upload tile descriptor");
   std::vector<SyTile> host(1, t);
-  CK(sytrd_run(ctx->stream, ctx->stile.as<SyTile>(), 1, n, host, kNB), "tridiagonal reduction");
+  CK(sytrd_run(ctx->stream, ctx->stile.as<SyTile>(), 1, n, host, kNB, &ctx->prof),
+     "tridiagonal reduction");
   ctx->have_A = false;
   CK(download(ctx, d_out, ctx->sd, (size_t)n * 8), "download d");
   if (n > 1) CK(download(ctx, e_out, ctx->se, (size_t)(n - 1) * 8), "download e");
@@ -467,6 +392,7 @@ int imcom_gpu_build_P(imcom_gpu_ctx* ctx, int precision, double* P_out) {
   CK(hipSetDevice(ctx->device), "hipSetDevice");
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->P.reserve((size_t)n * m * 8), "alloc P");
+  StageTimer tm(ctx, PS_BUILD_P);
   int rc = gemm_prec(ctx, precision, true, (int)n, (int)m, (int)n, 1.0, ctx->Q.as<double>(), n,
                      ctx->B.as<double>(), n, ctx->P.as<double>(), n, "P = Q^T B");
   if (rc) return rc;
@@ -504,10 +430,11 @@ int imcom_gpu_bisect_kappa(imcom_gpu_ctx* ctx, int UnotS, double US_max, double 
     CK(upload(ctx, ctx->bracket, bracket, (size_t)m * 2 * sizeof(float)), "upload bracket");
     dbr = ctx->bracket.as<float>();
   }
+  StageTimer tm(ctx, PS_BISECT);
   CK(launch_bisect(ctx->stream, n, m, ctx->L.as<double>(), ctx->P.as<double>(), n,
                    ctx->Nd.as<double>(), ctx->C_a, US_max, tol, k_min, k_max, maxnits,
                    UnotS ? 1 : 0, dbr, ctx->K.as<double>(), ctx->US.as<double>(),
-                   ctx->evals.as<int32_t>()),
+                   ctx->evals.as<int32_t>(), nullptr),
      "bisect");
   ctx->have_K = true;
   ctx->have_T = false;
@@ -525,6 +452,7 @@ int imcom_gpu_build_T(imcom_gpu_ctx* ctx, int precision, double* T_out) {
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->W.reserve((size_t)n * m * 8), "alloc W");
   CK(ctx->T.reserve((size_t)n * m * 8), "alloc T");
+  StageTimer tm(ctx, PS_BUILD_T);
   CK(launch_scale_W(ctx->stream, n, m, ctx->P.as<double>(), n, ctx->L.as<double>(),
                     ctx->K.as<double>(), ctx->W.as<double>(), n),
      "W = P / (L + K)");
@@ -541,6 +469,7 @@ int imcom_gpu_build_S(imcom_gpu_ctx* ctx, double* S_out) {
   NEED(ctx->have_T, IMCOM_ESTATE, "build_S: T not built");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
   CK(ctx->S.reserve(ctx->m * 8), "alloc S");
+  StageTimer tm(ctx, PS_MAPS);
   CK(launch_maps(ctx->stream, ctx->n, ctx->m, ctx->T.as<double>(), ctx->n, nullptr,
                  ctx->Nd.as<double>(), nullptr, ctx->S.as<double>()),
      "build_S");
@@ -553,6 +482,7 @@ int imcom_gpu_build_U(imcom_gpu_ctx* ctx, double* U_out) {
   NEED(ctx->have_K && ctx->have_P && ctx->have_QL, IMCOM_ESTATE, "build_U: needs L, P and K");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
   CK(ctx->U.reserve(ctx->m * 8), "alloc U");
+  StageTimer tm(ctx, PS_MAPS);
   CK(launch_U_from_K(ctx->stream, ctx->n, ctx->m, ctx->L.as<double>(), ctx->P.as<double>(), ctx->n,
                      ctx->C_a, ctx->K.as<double>(), ctx->U.as<double>()),
      "build_U");
@@ -567,10 +497,33 @@ int imcom_gpu_build_image(imcom_gpu_ctx* ctx, const double* I_i, double* H_out) 
   CK(hipSetDevice(ctx->device), "hipSetDevice");
   CK(upload(ctx, ctx->I, I_i, ctx->n * 8), "upload I_i");
   CK(ctx->H.reserve(ctx->m * 8), "alloc H");
+  StageTimer tm(ctx, PS_MAPS);
   CK(launch_maps(ctx->stream, ctx->n, ctx->m, ctx->T.as<double>(), ctx->n, ctx->I.as<double>(),
                  nullptr, ctx->H.as<double>(), nullptr),
      "build_image");
   CK(download(ctx, H_out, ctx->H, ctx->m * 8), "download H");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_set_profiling(imcom_gpu_ctx* ctx, int level) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(level >= 0 && level <= 2, IMCOM_EARG, "profiling level must be 0, 1 or 2");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(ctx->prof.collect(), "collect stage times");
+  ctx->prof.level = level;
+  ctx->prof.reset();
+  return IMCOM_OK;
+}
+
+int imcom_gpu_stage_times(imcom_gpu_ctx* ctx, double* ms_out, int64_t* counts_out, int nslots) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ms_out && nslots > 0, IMCOM_EARG, "stage_times: NULL output");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(ctx->prof.collect(), "collect stage times");
+  for (int i = 0; i < nslots; ++i) {
+    ms_out[i] = i < PS_NSLOTS ? ctx->prof.ms[i] : 0.0;
+    if (counts_out) counts_out[i] = i < PS_NSLOTS ? ctx->prof.count[i] : 0;
+  }
   return IMCOM_OK;
 }
 

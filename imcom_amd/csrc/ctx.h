@@ -1,0 +1,126 @@
+// Context of the C ABI (shared by the capi*.hip translation units).
+#pragma once
+#include "../../include/imcom_gpu.h"
+#include "common.h"
+#include "kernels.h"
+
+#include <string>
+#include <vector>
+
+using namespace imcom;
+
+namespace imcom_ctx_detail {
+
+constexpr int kNB = 32;    // tridiagonal-reduction panel width
+constexpr int kNBB = 128;  // back-transform block
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t bytes) {
+    if (bytes <= cap && p) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      cap = 0;
+      if (e != hipSuccess) return e;
+    }
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+}  // namespace imcom_ctx_detail
+using imcom_ctx_detail::DevBuf;
+using imcom_ctx_detail::kNB;
+using imcom_ctx_detail::kNBB;
+
+struct imcom_gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int64_t n = 0, m = 0;
+  double C_a = 0.0;
+  std::vector<int32_t> h_exp;
+  // tables
+  int psfconst = 1, nexppsf = 1, n1 = 0, n2 = 0;
+  bool have_tables = false;
+  // stage flags
+  bool have_tile = false, have_A = false, have_B = false, have_QL = false, have_P = false,
+       have_K = false, have_T = false;
+  // inputs
+  DevBuf x, y, ex, Xa, Ya, Nd, Atab, Btab, I;
+  // stages
+  DevBuf A, B, Q, L, P, K, US, T, W, H, S, U, evals, bracket;
+  // eigensolver workspace
+  DevBuf sW, sVW, sWV, sy, sycp, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb, sTb,
+      sY, sY2, slam, sZ, stile;
+  DcWork* dc = nullptr;
+  DevBuf split_work;
+  Prof prof;
+  // learned kappa bracket
+  DevBuf mlp_params, mlp_feat, mlp_bracket, mlp_bins, mlp_hits, mlp_X, mlp_Y, mlp_perm, mlp_loss,
+      mlp_grad, mlp_m, mlp_v, mlp_out;
+  DevBuf ms_Xn, ms_H1, ms_dZ1, ms_H2, ms_dZ2, ms_dY;
+  bool have_mlp = false;
+  int mlp_step = 0;
+};
+
+// stage timing helper: events recorded only when profiling is on
+struct StageTimer {
+  imcom_gpu_ctx* c;
+  int slot;
+  size_t tok = 0;
+  StageTimer(imcom_gpu_ctx* ctx_, int slot_) : c(ctx_), slot(slot_) {
+    (void)c->prof.begin(slot, 1, &tok);
+  }
+  ~StageTimer() { (void)c->prof.end(slot, tok); }
+};
+
+inline int set_err(imcom_gpu_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+inline int hip_err(imcom_gpu_ctx* c, hipError_t e, const char* where) {
+  if (e == hipSuccess) return IMCOM_OK;
+  const int code = (e == hipErrorOutOfMemory) ? IMCOM_ENOMEM : IMCOM_EHIP;
+  return set_err(c, code, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+#define CK(expr, where)                                   \
+  do {                                                    \
+    hipError_t _e = (expr);                               \
+    if (_e != hipSuccess) return hip_err(ctx, _e, where); \
+  } while (0)
+
+#define NEED(cond, code, msg) \
+  do {                        \
+    if (!(cond)) return set_err(ctx, code, msg); \
+  } while (0)
+
+inline hipError_t upload(imcom_gpu_ctx* ctx, DevBuf& b, const void* h, size_t bytes) {
+  hipError_t e = b.reserve(bytes);
+  if (e != hipSuccess) return e;
+  if (bytes == 0) return hipSuccess;
+  return hipMemcpyAsync(b.p, h, bytes, hipMemcpyHostToDevice, ctx->stream);
+}
+
+inline hipError_t download(imcom_gpu_ctx* ctx, void* h, const DevBuf& b, size_t bytes) {
+  if (!h || bytes == 0) return hipSuccess;
+  hipError_t e = hipMemcpyAsync(h, b.p, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(ctx->stream);
+}
+

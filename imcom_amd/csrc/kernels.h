@@ -7,6 +7,40 @@
 
 namespace imcom {
 
+// ---- profiling --------------------------------------------------------------
+// Device-time accounting with HIP events on the context's stream.  Slots
+// name the stages of the solve path; per-launch timing of the tridiagonal
+// reduction's matrix-vector kernel is kept in its own slot (level 2).
+enum ProfSlot {
+  PS_LOOKUP_A = 0,
+  PS_SYTRD,
+  PS_STEDC,
+  PS_ORMTR,
+  PS_LOOKUP_B,
+  PS_BUILD_P,
+  PS_BISECT,
+  PS_BUILD_T,
+  PS_MAPS,
+  PS_SYMV,      // sy_symv launches only (level 2)
+  PS_SYR2K,     // trailing-update GEMMs only (level 2)
+  PS_NSLOTS
+};
+
+struct Prof {
+  int level = 0;
+  hipStream_t st = nullptr;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> pending;  // (slot, index of the first event)
+  double ms[PS_NSLOTS] = {0};
+  int64_t count[PS_NSLOTS] = {0};
+  hipError_t begin(int slot, int need_level, size_t* token);
+  hipError_t end(int slot, size_t token);
+  hipError_t collect();
+  void reset();
+  void destroy();
+};
+
 // ---- GEMM -----------------------------------------------------------------
 struct GemmDesc {
   int M, N, K;
@@ -46,7 +80,7 @@ hipError_t launch_lookup_B(hipStream_t st, int dev, int64_t n, int64_t m, const 
 hipError_t launch_bisect(hipStream_t st, int64_t n, int64_t m, const double* L, const double* P,
                          int64_t ldp, const double* Nd, double C_a, double US_max, double tol,
                          double kmin, double kmax, int maxnits, int UnotS, const float* bracket,
-                         double* K, double* US, int32_t* evals);
+                         double* K, double* US, int32_t* evals, int32_t* hits);
 hipError_t launch_scale_W(hipStream_t st, int64_t n, int64_t m, const double* P, int64_t ldp,
                           const double* L, const double* K, double* W, int64_t ldw);
 hipError_t launch_maps(hipStream_t st, int64_t n, int64_t m, const double* T, int64_t ldt,
@@ -64,7 +98,8 @@ struct SyTile {
   double* W;      // n x NB
   double* VW;     // n x 2NB  [V W]
   double* WV;     // n x 2NB  [W V]
-  double* y;      // n
+  double* ypart;  // ceil(n/64) x n   row partials of A22 v
+  double* ycpart; // ceil(n/512) x n  column partials of A22 v
   double* wpre;   // n
   double* vtmp;   // n
   double* d;      // n     tridiagonal diagonal
@@ -83,7 +118,7 @@ struct SyTile {
 };
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                     const std::vector<SyTile>& host, int nb);
+                     const std::vector<SyTile>& host, int nb, Prof* prof);
 hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nbb);
 
@@ -94,6 +129,22 @@ DcWork* dc_create();
 void dc_destroy(DcWork* w);
 hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const double* e,
                     double* lam, double* Z);
+
+// ---- learned kappa bracket ---------------------------------------------------
+struct MlpScratch {
+  float *Xn, *H1, *dZ1, *H2, *dZ2, *dY;
+};
+int mlp_nparams();
+int mlp_ntrain();
+hipError_t launch_features(hipStream_t st, int64_t n, int64_t m, const double* L, int8_t* bins,
+                           const double* P, int64_t ldp, double C_a, double target, float* X);
+hipError_t launch_mlp_infer(hipStream_t st, const float* params, const float* X, int64_t N,
+                            float zsig, float* out, int as_bracket);
+hipError_t mlp_gradient(hipStream_t st, const float* params, const float* X, const float* Y,
+                        const int* perm, int64_t off, int64_t B, MlpScratch& s, float* grad,
+                        float* loss);
+hipError_t launch_adam(hipStream_t st, float* p, const float* g, float* m, float* v, int n,
+                       float lr, int step);
 
 // misc
 hipError_t launch_reverse_cols(hipStream_t st, int64_t n, const double* Zin, double* Qout,

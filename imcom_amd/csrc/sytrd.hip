@@ -10,13 +10,15 @@
 // and w_c = tau (A22 v - V (W^T v) - W (V^T v)) - 1/2 tau^2 (w^T v) v is
 // accumulated; after the panel the trailing matrix takes one rank-2NB update
 // A22 -= V W^T + W V^T on the FP64 matrix cores.  The full symmetric trailing
-// matrix is kept, so the matrix-vector product reads whole columns
-// (coalesced, one wavefront per column).
+// matrix is kept in memory; the matrix-vector product reads only its lower
+// triangle, in 64x64 blocks, each block serving its rows and its columns.
 //
 // Three launches per column:
-//   sy_symv      reflector from the column's norm partials, y = A22 v and
-//                the 2jj dot products V^T v, W^T v (one wavefront per column)
-//   sy_wpre      w_pre = tau (y - V sW - W sV) and partials of w_pre^T v
+//   sy_symv      reflector from the column's norm partials, partial sums of
+//                y = A22 v over the lower triangle, the 2jj dot products
+//                V^T v, W^T v
+//   sy_wpre      y from its partials, w_pre = tau (y - V sW - W sV) and
+//                partials of w_pre^T v
 //   sy_wfin      w = w_pre + alpha v; rank-2(jj+1) update of the next column
 //                and its norm partials
 // All sums are fixed-order (partials per workgroup, reduced in workgroup
@@ -71,21 +73,15 @@ __global__ __launch_bounds__(RB) void sy_colupd(const SyTile* __restrict__ tiles
   if (threadIdx.x == 0) t.part[blockIdx.x] = tot;
 }
 
-// Reflector for column c, y = A22 v, and V^T v, W^T v.
-// Columns handled (one wavefront each): j in [c+1, n) of A22, then jj columns
-// of V (A's panel columns), then jj columns of W.
-__global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
-                                               int jj, int nparts) {
-  const SyTile t = tiles[blockIdx.z];
-  const int64_t n = t.n;
-  if (c >= n - 1) return;
-  double* colc = t.A + (int64_t)c * n;
-  // reduce the norm partials (every workgroup, same order)
+// Householder reflector of column c from its norm partials (LAPACK dlarfg
+// conventions): beta = -sign(alpha) |x|, tau = (beta - alpha)/beta, and
+// v = [1, x(2:)/(alpha - beta)].
+__device__ __forceinline__ void sy_reflector(const SyTile& t, int64_t c, int nparts, double& tau,
+                                             double& beta, double& scal) {
   double xn2 = 0.0;
   for (int q = 0; q < nparts; ++q) xn2 += t.part[q];
-  const double alpha = colc[c + 1];
+  const double alpha = t.A[(c + 1) + c * t.n];
   const double xnorm = sqrt(xn2);
-  double tau, beta, scal;
   if (xnorm == 0.0) {
     tau = 0.0;
     beta = alpha;
@@ -95,32 +91,86 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
     tau = (beta - alpha) / beta;
     scal = 1.0 / (alpha - beta);
   }
-  const int lane = threadIdx.x & 63;
-  const int64_t gw = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-  const int64_t m22 = n - c - 1;  // rows/cols of A22
-  const int64_t ncol = m22 + 2 * jj;
-  for (int64_t q = gw; q < ncol; q += nw) {
-    const double* X;
-    if (q < m22)
-      X = t.A + (c + 1 + q) * n;
-    else if (q < m22 + jj)
-      X = t.A + (int64_t)(k + (q - m22)) * n;
-    else
-      X = t.W + (q - m22 - jj) * n;
-    // dot over rows [c+1, n): v_{c+1} = 1, v_i = scal * colc[i]
-    double s = 0.0;
-    for (int64_t i = c + 2 + lane; i < n; i += 64) s += X[i] * colc[i];
-    s = wave_sum(s);
-    if (lane == 0) {
-      const double dot = X[c + 1] + scal * s;
-      if (q < m22)
-        t.y[c + 1 + q] = dot;
-      else
-        t.s[q - m22] = dot;  // s[0:jj) = V^T v, s[jj:2jj) = W^T v
+}
+
+constexpr int SB = 64;   // symmetric matrix-vector block
+constexpr int SEG = 8;   // blocks of one block-column per workgroup
+
+// y = A22 v reading only the lower triangle of A22 (order s = n-c-1, the
+// trailing matrix below/right of column c).  Workgroup (seg, J) walks blocks
+// I in [J + seg*SEG, J + (seg+1)*SEG) of block-column J: block (I,J) gives the
+// rows of block I the product A_IJ v_J (written to ypart[J][rows]) and, for
+// I > J, gives the columns of block J the product A_IJ^T v_I (accumulated over
+// the segment and written to ycpart[seg][cols]).  sy_wpre sums the partials
+// in a fixed order.  Workgroups in row y = nbr compute the 2jj dot products
+// V^T v and W^T v, one wavefront each.
+__global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
+                                               int jj, int nparts) {
+  __shared__ double vJ[SB];
+  __shared__ double red[4][SB + 1];
+  const SyTile t = tiles[blockIdx.z];
+  const int64_t n = t.n;
+  if (c >= n - 1) return;
+  const double* __restrict__ colc = t.A + (int64_t)c * n;
+  double tau, beta, scal;
+  sy_reflector(t, c, nparts, tau, beta, scal);
+  const int64_t s = n - c - 1;
+  const int nbr = (int)((s + SB - 1) / SB);
+  const int J = blockIdx.y;
+  const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+  if (J >= nbr) {
+    const int q = blockIdx.x * 4 + ty;
+    if (J == nbr && q < 2 * jj) {
+      const double* X = q < jj ? t.A + (int64_t)(k + q) * n : t.W + (int64_t)(q - jj) * n;
+      double acc = 0.0;
+      for (int64_t i = c + 2 + tx; i < n; i += 64) acc += X[i] * colc[i];
+      acc = wave_sum(acc);
+      if (tx == 0) t.s[q] = X[c + 1] + scal * acc;  // s[0:jj) = V^T v, s[jj:2jj) = W^T v
     }
+    return;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  const int I0 = J + blockIdx.x * SEG;
+  if (I0 >= nbr) return;
+  const int I1 = min(I0 + SEG, nbr);
+  // v in A22-local indexing: v(0) = 1, v(r) = scal * A(c+1+r, c)
+  if (tid < SB) {
+    const int64_t r = (int64_t)J * SB + tid;
+    vJ[tid] = (r < s) ? (r == 0 ? 1.0 : scal * colc[c + 1 + r]) : 0.0;
+  }
+  __syncthreads();
+  const double* __restrict__ A22 = t.A + (c + 1) + (int64_t)(c + 1) * n;
+  double colacc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) colacc[q] = 0.0;
+  for (int I = I0; I < I1; ++I) {
+    const int64_t r = (int64_t)I * SB + tx;
+    const bool rv = r < s;
+    double a[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int64_t col = (int64_t)J * SB + ty + 4 * q;
+      a[q] = (rv && col < s) ? A22[r + col * n] : 0.0;
+    }
+    double rowp = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) rowp += a[q] * vJ[ty + 4 * q];
+    red[ty][tx] = rowp;
+    if (I > J) {
+      const double vI = rv ? (r == 0 ? 1.0 : scal * colc[c + 1 + r]) : 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) colacc[q] += a[q] * vI;
+    }
+    __syncthreads();
+    if (ty == 0 && rv) t.ypart[(int64_t)J * n + r] = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const double cs = wave_sum(colacc[q]);
+    const int64_t col = (int64_t)J * SB + ty + 4 * q;
+    if (tx == 0 && col < s) t.ycpart[(int64_t)blockIdx.x * n + col] = cs;
+  }
+  if (blockIdx.x == 0 && J == 0 && tid == 0) {
     t.d[c] = colc[c];
     t.e[c] = beta;
     t.tau[c] = tau;
@@ -128,7 +178,7 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
   }
 }
 
-// Store v into A's column c (unit at c+1) after sy_symv has consumed it, and
+// y from the partials of sy_symv, v into vtmp, and
 // w_pre = tau (y - V sW - W sV) with partials of w_pre^T v.
 __global__ __launch_bounds__(RB) void sy_wpre(const SyTile* __restrict__ tiles, int k, int c,
                                               int jj) {
@@ -141,9 +191,16 @@ __global__ __launch_bounds__(RB) void sy_wpre(const SyTile* __restrict__ tiles, 
   const double scal = t.scal[0];
   double contrib = 0.0;
   if (i < n) {
-    double* colc = t.A + (int64_t)c * n;
+    const double* colc = t.A + (int64_t)c * n;
     const double v = (i == c + 1) ? 1.0 : colc[i] * scal;
-    double acc = t.y[i];
+    const int64_t s = n - c - 1;
+    const int nbr = (int)((s + SB - 1) / SB);
+    const int64_t r = i - (c + 1);
+    const int R = (int)(r / SB);
+    double acc = 0.0;
+    for (int J = 0; J <= R; ++J) acc += t.ypart[(int64_t)J * n + r];
+    const int nseg = (nbr - R + SEG - 1) / SEG;
+    for (int sg = 0; sg < nseg; ++sg) acc += t.ycpart[(int64_t)sg * n + r];
     for (int p = 0; p < jj; ++p) {
       const double Vip = t.A[i + (int64_t)(k + p) * n];
       const double Wip = t.W[i + (int64_t)p * n];
@@ -275,8 +332,9 @@ __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ til
 }
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                     const std::vector<SyTile>& host, int nb) {
+                     const std::vector<SyTile>& host, int nb, Prof* prof) {
   const int rbl = ceil_div(nmax, RB);
+  size_t tok = 0;
   for (int64_t k = 0; k < nmax - 1; k += nb) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
     // first column of the panel: norm partials only (jj = 0)
@@ -285,10 +343,12 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
     for (int jj = 0; jj < pb; ++jj) {
       const int64_t c = k + jj;
       const int nparts = ceil_div(nmax - c, RB);
-      const int64_t ncol = (nmax - c - 1) + 2 * jj;
-      const int symv_blocks = (int)std::min<int64_t>(std::max<int64_t>(ceil_div(ncol, 4), 1), 2048);
-      hipLaunchKernelGGL(sy_symv, dim3(symv_blocks, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
-                         (int)c, jj, nparts);
+      const int nbr = ceil_div(nmax - c - 1, SB);
+      const int gx = std::max(ceil_div(nbr, SEG), ceil_div(2 * jj, 4));
+      if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYMV, 2, &tok));
+      hipLaunchKernelGGL(sy_symv, dim3(std::max(gx, 1), nbr + 1, ntiles), dim3(256), 0, st,
+                         d_tiles, (int)k, (int)c, jj, nparts);
+      if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
       const int nb2 = std::max(ceil_div(nmax - c - 1, RB), 1);
       hipLaunchKernelGGL(sy_wpre, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
                          jj);
@@ -301,6 +361,7 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
     if (r0 < nmax) {
       hipLaunchKernelGGL(sy_pack, dim3(ceil_div(nmax - r0, 256), nb, ntiles), dim3(256), 0, st,
                          d_tiles, (int)k, nb, r0);
+      if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYR2K, 2, &tok));
       for (int b = 0; b < ntiles; ++b) {
         const SyTile& h = host[b];
         if (r0 >= h.n) continue;
@@ -308,6 +369,7 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
         IMCOM_HIP_TRY(dgemm(st, false, true, (int)mm, (int)mm, 2 * nb, -1.0, h.VW + r0, h.n,
                             h.WV + r0, h.n, 1.0, h.A + r0 + r0 * h.n, h.n));
       }
+      if (prof) IMCOM_HIP_TRY(prof->end(PS_SYR2K, tok));
     }
   }
   hipLaunchKernelGGL(sy_last_diag, dim3(1, 1, ntiles), dim3(1), 0, st, d_tiles);

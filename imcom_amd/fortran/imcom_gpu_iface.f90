@@ -1,0 +1,159 @@
+! ISO_C_BINDING interface to libimcom_hip.so (include/imcom_gpu.h) for the
+! reference's Fortran driver.  A replacement imcom_matrix / imcom_bisect pair
+! calls these from the bodies of imcom_lookup_A, imcom_eigen_A,
+! imcom_lookup_B, imcom_build_P, imcom_bisect_kappa, imcom_build_T/S/U and
+! imcom_build_image, keeping every public procedure name and argument, so
+! imcom.f90 compiles unchanged against them.  Sizes are int64, arrays are
+! passed as assumed-size real(c_double) (Fortran column-major, as the C side
+! expects); a C status other than 0 is reported as "IMCOM ERROR: <message>"
+! followed by STOP, as the reference does.
+MODULE imcom_gpu_iface
+  USE, INTRINSIC :: iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: IMCOM_PREC_FP64 = 0, IMCOM_PREC_FP32X3 = 1, &
+                               IMCOM_PREC_BF16X6 = 2
+
+  interface
+    integer(c_int) function imcom_gpu_init(ctx, device) bind(C, name="imcom_gpu_init")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: ctx
+      integer(c_int), value :: device
+    end function
+    subroutine imcom_gpu_finalize(ctx) bind(C, name="imcom_gpu_finalize")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end subroutine
+    type(c_ptr) function imcom_gpu_last_error(ctx) bind(C, name="imcom_gpu_last_error")
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function imcom_gpu_set_tile(ctx, n, m, x_i, y_i, exp_i, X_a, Y_a, Ndiag, C_a) &
+        bind(C, name="imcom_gpu_set_tile")
+      import :: c_int, c_ptr, c_int64_t, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n, m
+      real(c_double), intent(in) :: x_i(*), y_i(*), X_a(*), Y_a(*), Ndiag(*)
+      integer(c_int32_t), intent(in) :: exp_i(*)
+      real(c_double), value :: C_a
+    end function
+    integer(c_int) function imcom_gpu_set_tables(ctx, psfconst, nexppsf, n1, n2, Alookup, Blookup) &
+        bind(C, name="imcom_gpu_set_tables")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: psfconst, nexppsf, n1, n2
+      real(c_double), intent(in) :: Alookup(*), Blookup(*)
+    end function
+    integer(c_int) function imcom_gpu_lookup_A(ctx, npoly, npad, psfxscale, psfyscale, A_out) &
+        bind(C, name="imcom_gpu_lookup_A")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npoly, npad
+      real(c_double), value :: psfxscale, psfyscale
+      real(c_double), intent(out) :: A_out(*)
+    end function
+    integer(c_int) function imcom_gpu_set_A(ctx, A) bind(C, name="imcom_gpu_set_A")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: A(*)
+    end function
+    integer(c_int) function imcom_gpu_eigen_A(ctx, Q_out, L_out, condition) &
+        bind(C, name="imcom_gpu_eigen_A")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: Q_out(*), L_out(*)
+      real(c_double), intent(out) :: condition
+    end function
+    integer(c_int) function imcom_gpu_set_QL(ctx, Q, L) bind(C, name="imcom_gpu_set_QL")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: Q(*), L(*)
+    end function
+    integer(c_int) function imcom_gpu_lookup_B(ctx, npoly, npad, psfxscale, psfyscale, B_out) &
+        bind(C, name="imcom_gpu_lookup_B")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: npoly, npad
+      real(c_double), value :: psfxscale, psfyscale
+      real(c_double), intent(out) :: B_out(*)
+    end function
+    integer(c_int) function imcom_gpu_set_B(ctx, B) bind(C, name="imcom_gpu_set_B")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: B(*)
+    end function
+    integer(c_int) function imcom_gpu_build_P(ctx, precision, P_out) bind(C, name="imcom_gpu_build_P")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+      real(c_double), intent(out) :: P_out(*)
+    end function
+    integer(c_int) function imcom_gpu_set_P(ctx, P) bind(C, name="imcom_gpu_set_P")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: P(*)
+    end function
+    integer(c_int) function imcom_gpu_bisect_kappa(ctx, UnotS, US_max, k_min, k_max, tol, maxnits, &
+        bracket, K_out, US_out, evals_out) bind(C, name="imcom_gpu_bisect_kappa")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: UnotS, maxnits
+      real(c_double), value :: US_max, k_min, k_max, tol
+      type(c_ptr), value :: bracket
+      real(c_double), intent(out) :: K_out(*), US_out(*)
+      integer(c_int32_t), intent(out) :: evals_out(*)
+    end function
+    integer(c_int) function imcom_gpu_build_T(ctx, precision, T_out) bind(C, name="imcom_gpu_build_T")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision
+      real(c_double), intent(out) :: T_out(*)
+    end function
+    integer(c_int) function imcom_gpu_build_S(ctx, S_out) bind(C, name="imcom_gpu_build_S")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: S_out(*)
+    end function
+    integer(c_int) function imcom_gpu_build_U(ctx, U_out) bind(C, name="imcom_gpu_build_U")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: U_out(*)
+    end function
+    integer(c_int) function imcom_gpu_build_image(ctx, I_i, H_out) bind(C, name="imcom_gpu_build_image")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: I_i(*)
+      real(c_double), intent(out) :: H_out(*)
+    end function
+  end interface
+
+CONTAINS
+
+  ! Reference-style error stop: print "IMCOM ERROR: <library message>" and stop.
+  SUBROUTINE imcom_gpu_check(ctx, status, where)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    integer :: i
+    interface
+      integer(c_size_t) function c_strlen(s) bind(C, name="strlen")
+        import :: c_size_t, c_ptr
+        type(c_ptr), value :: s
+      end function
+    end interface
+    type(c_ptr) :: p
+    integer(c_size_t) :: len
+    if (status.eq.0) return
+    p = imcom_gpu_last_error(ctx)
+    len = c_strlen(p)
+    call c_f_pointer(p, msg, [len])
+    write(*, FMT='(A)', ADVANCE='NO') "IMCOM ERROR: "//where//": "
+    do i = 1, int(len)
+      write(*, FMT='(A)', ADVANCE='NO') msg(i)
+    end do
+    write(*, *)
+    stop
+  END SUBROUTINE imcom_gpu_check
+
+END MODULE imcom_gpu_iface

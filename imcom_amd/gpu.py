@@ -30,8 +30,14 @@ EXPORTS = [
     "imcom_gpu_eigen_A", "imcom_gpu_tridiagonalize", "imcom_gpu_tridiag_eig",
     "imcom_gpu_set_QL", "imcom_gpu_lookup_B", "imcom_gpu_set_B", "imcom_gpu_build_P",
     "imcom_gpu_set_P", "imcom_gpu_bisect_kappa", "imcom_gpu_build_T", "imcom_gpu_build_S",
-    "imcom_gpu_build_U", "imcom_gpu_build_image",
+    "imcom_gpu_build_U", "imcom_gpu_build_image", "imcom_gpu_set_profiling",
+    "imcom_gpu_stage_times", "imcom_gpu_mlp_nparams", "imcom_gpu_mlp_set_params",
+    "imcom_gpu_mlp_get_params", "imcom_gpu_kappa_features", "imcom_gpu_mlp_infer",
+    "imcom_gpu_mlp_grad", "imcom_gpu_mlp_train", "imcom_gpu_bisect_kappa_learned",
 ]
+
+STAGE_SLOTS = ["lookup_A", "sytrd", "stedc", "ormtr", "lookup_B", "build_P", "bisect",
+               "build_T", "maps", "symv", "syr2k"]
 
 
 class ImcomError(RuntimeError):
@@ -80,6 +86,18 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_build_S": (_int, [_vp, _dp]),
         "imcom_gpu_build_U": (_int, [_vp, _dp]),
         "imcom_gpu_build_image": (_int, [_vp, _dp, _dp]),
+        "imcom_gpu_set_profiling": (_int, [_vp, _int]),
+        "imcom_gpu_stage_times": (_int, [_vp, _dp, ctypes.POINTER(_i64), _int]),
+        "imcom_gpu_mlp_nparams": (_int, []),
+        "imcom_gpu_mlp_set_params": (_int, [_vp, _fp]),
+        "imcom_gpu_mlp_get_params": (_int, [_vp, _fp]),
+        "imcom_gpu_kappa_features": (_int, [_vp, _dbl, _fp]),
+        "imcom_gpu_mlp_infer": (_int, [_vp, _fp, _i64, _fp]),
+        "imcom_gpu_mlp_grad": (_int, [_vp, _fp, _fp, _i64, _fp, _fp]),
+        "imcom_gpu_mlp_train": (_int, [_vp, _fp, _fp, _i64, _int, _int, ctypes.c_float,
+                                       ctypes.c_uint64, _fp]),
+        "imcom_gpu_bisect_kappa_learned": (_int, [_vp, _int, _dbl, _dbl, _dbl, _dbl, _int,
+                                                  ctypes.c_float, _dp, _dp, _ip, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -161,10 +179,12 @@ class ImcomGPU:
         A = _mat(A)
         self._ck(self.lib.imcom_gpu_set_A(self.h, _ptr(A)), "set_A")
 
-    def eigen_A(self, out=True):
+    def eigen_A(self, out=True, out_Q=None, out_L=None):
         n = self.n
-        Q = np.empty((n, n), order="F") if out else None
-        L = np.empty(n) if out else None
+        out_Q = out if out_Q is None else out_Q
+        out_L = out if out_L is None else out_L
+        Q = np.empty((n, n), order="F") if out_Q else None
+        L = np.empty(n) if out_L else None
         cond = ctypes.c_double(0.0)
         self._ck(self.lib.imcom_gpu_eigen_A(self.h, _ptr(Q), _ptr(L), ctypes.byref(cond)),
                  "eigen_A")
@@ -249,6 +269,80 @@ class ImcomGPU:
         H = np.empty(self.m)
         self._ck(self.lib.imcom_gpu_build_image(self.h, _ptr(I_i), _ptr(H)), "build_image")
         return H
+
+    def set_profiling(self, level):
+        self._ck(self.lib.imcom_gpu_set_profiling(self.h, level), "set_profiling")
+
+    def stage_times(self):
+        k = len(STAGE_SLOTS)
+        ms = np.zeros(k)
+        cnt = np.zeros(k, dtype=np.int64)
+        self._ck(self.lib.imcom_gpu_stage_times(self.h, _ptr(ms), cnt.ctypes.data_as(ctypes.POINTER(_i64)), k),
+                 "stage_times")
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(STAGE_SLOTS)}
+
+    # ---- learned kappa bracket --------------------------------------------
+    def mlp_nparams(self):
+        return int(self.lib.imcom_gpu_mlp_nparams())
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.shape[0] == self.mlp_nparams()
+        self._ck(self.lib.imcom_gpu_mlp_set_params(self.h, _ptr(p, _fp)), "mlp_set_params")
+
+    def mlp_get_params(self):
+        p = np.empty(self.mlp_nparams(), dtype=np.float32)
+        self._ck(self.lib.imcom_gpu_mlp_get_params(self.h, _ptr(p, _fp)), "mlp_get_params")
+        return p
+
+    def kappa_features(self, US_target=None):
+        t = self.tile
+        US_target = t.US_max if US_target is None else US_target
+        X = np.empty((self.m, 32), dtype=np.float32)
+        self._ck(self.lib.imcom_gpu_kappa_features(self.h, US_target, _ptr(X, _fp)), "kappa_features")
+        return X
+
+    def mlp_infer(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        out = np.empty((X.shape[0], 2), dtype=np.float32)
+        self._ck(self.lib.imcom_gpu_mlp_infer(self.h, _ptr(X, _fp), X.shape[0], _ptr(out, _fp)),
+                 "mlp_infer")
+        return out
+
+    def mlp_grad(self, X, y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        g = np.empty(6402, dtype=np.float32)
+        loss = ctypes.c_float(0.0)
+        self._ck(self.lib.imcom_gpu_mlp_grad(self.h, _ptr(X, _fp), _ptr(y, _fp), X.shape[0],
+                                             _ptr(g, _fp), ctypes.byref(loss)), "mlp_grad")
+        return g, float(loss.value)
+
+    def mlp_train(self, X, y, epochs=10, batch=1024, lr=1e-3, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        losses = np.zeros(max(epochs, 1), dtype=np.float32)
+        self._ck(self.lib.imcom_gpu_mlp_train(self.h, _ptr(X, _fp), _ptr(y, _fp), X.shape[0], epochs,
+                                              batch, lr, seed, _ptr(losses, _fp)), "mlp_train")
+        return losses[:epochs]
+
+    def bisect_kappa_learned(self, zsig=3.0, UnotS=None, US=None, k_min=None, k_max=None,
+                             tol=None, maxnits=None):
+        t = self.tile
+        UnotS = t.USB if UnotS is None else UnotS
+        US = t.US_max if US is None else US
+        k_min = t.kappa_min if k_min is None else k_min
+        k_max = t.kappa_max if k_max is None else k_max
+        tol = t.US_tol if tol is None else tol
+        maxnits = t.maxNbis if maxnits is None else maxnits
+        m = self.m
+        K, USa = np.empty(m), np.empty(m)
+        ev, hits = np.empty(m, dtype=np.int32), np.empty(m, dtype=np.int32)
+        self._ck(self.lib.imcom_gpu_bisect_kappa_learned(self.h, 1 if UnotS else 0, US, k_min, k_max,
+                                                         tol, maxnits, zsig, _ptr(K), _ptr(USa),
+                                                         _ptr(ev, _ip), _ptr(hits, _ip)),
+                 "bisect_kappa_learned")
+        return K, USa, ev, hits
 
     def synchronize(self):
         self._ck(self.lib.imcom_gpu_synchronize(self.h), "synchronize")

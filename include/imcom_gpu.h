@@ -116,6 +116,43 @@ int imcom_gpu_build_U(imcom_gpu_ctx* ctx, double* U_out);
 /* imcom_build_image  (imcom_matrix.f90:754-790): H = T^T I. */
 int imcom_gpu_build_image(imcom_gpu_ctx* ctx, const double* I_i, double* H_out);
 
+/* Learned kappa bracket.  No reference counterpart: the reference always
+ * bisects the full [kappa_min, kappa_max] range (imcom_bisect.f90:174-267).
+ * A 32-64-64-2 GELU MLP (FP32) reads 32 features per output pixel (12
+ * log eigenvalue quantiles, 19 log-binned energies of the pixel's column of
+ * P, log of the U or S target) and predicts mu and log sigma of log10 kappa.
+ * Parameter block: imcom_gpu_mlp_nparams() floats (weights, then the
+ * feature and target standardisation).  imcom_gpu_mlp_train runs Adam on the
+ * mean Gaussian negative log-likelihood over minibatches of (X: N*32,
+ * y: N log10 kappa); loss_out gets one mean loss per epoch.
+ * imcom_gpu_bisect_kappa_learned is imcom_gpu_bisect_kappa started from the
+ * bracket mu +- zsig*sigma; hits_out (m, may be NULL) is 1 where the bracket
+ * held the root and 0 where the pixel fell back to the full range. */
+int imcom_gpu_mlp_nparams(void);
+int imcom_gpu_mlp_set_params(imcom_gpu_ctx* ctx, const float* params);
+int imcom_gpu_mlp_get_params(imcom_gpu_ctx* ctx, float* params);
+int imcom_gpu_kappa_features(imcom_gpu_ctx* ctx, double US_target, float* X_out);
+int imcom_gpu_mlp_infer(imcom_gpu_ctx* ctx, const float* X, int64_t N, float* out);
+int imcom_gpu_mlp_grad(imcom_gpu_ctx* ctx, const float* X, const float* y, int64_t N,
+                       float* grad_out, float* loss_out);
+int imcom_gpu_mlp_train(imcom_gpu_ctx* ctx, const float* X, const float* y, int64_t N, int epochs,
+                        int batch, float lr, uint64_t seed, float* loss_out);
+int imcom_gpu_bisect_kappa_learned(imcom_gpu_ctx* ctx, int UnotS, double US_max, double k_min,
+                                   double k_max, double tol, int maxnits, float zsig,
+                                   double* K_out, double* US_out, int32_t* evals_out,
+                                   int32_t* hits_out);
+
+/* Device-time accounting (no reference counterpart; the reference prints
+ * progress lines only).  level 0: off; 1: per-stage HIP-event times; 2: also
+ * every launch of the tridiagonal reduction's matrix-vector kernel and its
+ * trailing-update GEMMs.  Setting the level resets the counters.
+ * imcom_gpu_stage_times fills ms_out[i] / counts_out[i] for the slots
+ *   0 lookup_A, 1 tridiagonal reduction, 2 divide-and-conquer,
+ *   3 back-transform, 4 lookup_B, 5 build_P, 6 bisect_kappa, 7 build_T,
+ *   8 maps (S, U, H), 9 matrix-vector launches, 10 trailing updates. */
+int imcom_gpu_set_profiling(imcom_gpu_ctx* ctx, int level);
+int imcom_gpu_stage_times(imcom_gpu_ctx* ctx, double* ms_out, int64_t* counts_out, int nslots);
+
 #ifdef __cplusplus
 }
 #endif

@@ -135,6 +135,24 @@ void or_lookup_A(int64_t n, const double *x_i, const double *y_i, const int32_t 
     for (int64_t j = 0; j < i; ++j) A[i + j * n] = A[j + i * n];
 }
 
+/* Rows [i0, i1) of imcom_lookup_A's upper triangle (j >= i) into A (n x n):
+ * a bounded sample of the A build, for CPU timing. */
+void or_lookup_A_rows(int64_t n, int64_t i0, int64_t i1, const double *x_i, const double *y_i,
+                      const int32_t *exp_i, int psfconst, int n1al, int n2al,
+                      const double *Alookup, int npad, int npoly, double psfxscale,
+                      double psfyscale, double *A) {
+  int64_t tsz = (int64_t)n1al * n2al;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t i = i0; i < i1; ++i) {
+    for (int64_t j = i; j < n; ++j) {
+      int ial = psfconst == 1 ? 1 : exp_i[i] + exp_i[j] * (exp_i[j] - 1) / 2;
+      double Delx = (double)npad * (x_i[j] - x_i[i]) / psfxscale;
+      double Dely = (double)npad * (y_i[j] - y_i[i]) / psfyscale;
+      A[i + j * n] = or_interp_lookup(n1al, n2al, Alookup + (ial - 1) * tsz, Delx, Dely, npoly);
+    }
+  }
+}
+
 /* imcom_matrix.f90:288-357.  B is n x m. */
 void or_lookup_B(int64_t n, int64_t m, const double *x_i, const double *y_i,
                  const int32_t *exp_i, const double *X_a, const double *Y_a, int psfconst,

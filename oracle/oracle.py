@@ -205,6 +205,80 @@ def build_S(T, Ndiag):
     return S
 
 
+def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, cols_P=128,
+                     pix_bisect=48, pix_T=8):
+    """Time the oracle's solve path on bounded samples of one tile and scale
+    each stage to the whole tile (the CPU-baseline leg of bench.py):
+
+      lookup_A   rows [0, rows_A) of the upper triangle, scaled by entry count
+      lookup_B   the first rows_B input pixels x all m outputs
+      eigen_A    DSYEVR on the leading n_eig x n_eig block of A, scaled by n^3
+      build_P    DGEMM Q^T B[:, :cols_P] at full n, scaled by m
+      bisect     pix_bisect pixels at full n, scaled by m
+      build_T    pix_T DGEMVs at full n, scaled by m
+
+    L and P (the tile's eigenvalues and projections, any source) feed the
+    bisection sample.  Returns (seconds for the whole tile, dict of stages).
+    """
+    import time
+    n, m = tile.n, tile.m
+    stages = {}
+    tab = _f(tile.Alookup)
+    ex = np.ascontiguousarray(tile.exp_i, dtype=np.int32)
+    x, y = np.ascontiguousarray(tile.x_i), np.ascontiguousarray(tile.y_i)
+    rows_A = min(rows_A, n)
+    A = np.zeros((n, n), order="F")
+    t0 = time.perf_counter()
+    lib().or_lookup_A_rows(ctypes.c_int64(n), ctypes.c_int64(0), ctypes.c_int64(rows_A), _p(x),
+                           _p(y), ex.ctypes.data_as(_ip), ctypes.c_int(tile.psfconst),
+                           ctypes.c_int(tab.shape[0]), ctypes.c_int(tab.shape[1]), _p(tab),
+                           ctypes.c_int(tile.npad), ctypes.c_int(tile.npoly),
+                           ctypes.c_double(tile.psfxscale), ctypes.c_double(tile.psfyscale), _p(A))
+    dt = time.perf_counter() - t0
+    done = rows_A * n - rows_A * (rows_A - 1) / 2
+    stages["lookup_A"] = dt * (n * (n + 1) / 2) / done
+    # leading block of A for the eigen sample (computed, not timed)
+    ne = min(n_eig, n)
+    sub = tile.__class__(**{**tile.__dict__})
+    sub.n = ne
+    sub.x_i, sub.y_i, sub.exp_i = tile.x_i[:ne], tile.y_i[:ne], tile.exp_i[:ne]
+    Ae = lookup_A(sub)
+    del A
+    rb = min(rows_B, n)
+    subB = tile.__class__(**{**tile.__dict__})
+    subB.n = rb
+    subB.x_i, subB.y_i, subB.exp_i = tile.x_i[:rb], tile.y_i[:rb], tile.exp_i[:rb]
+    t0 = time.perf_counter()
+    lookup_B(subB)
+    stages["lookup_B"] = (time.perf_counter() - t0) * n / rb
+    t0 = time.perf_counter()
+    eigen_A(Ae)
+    stages["eigen_A"] = (time.perf_counter() - t0) * (n / ne) ** 3
+    rng = np.random.default_rng(0)
+    Qr = np.asfortranarray(rng.standard_normal((n, n)))
+    cp = min(cols_P, m)
+    Bs = np.asfortranarray(rng.standard_normal((n, cp)))
+    t0 = time.perf_counter()
+    build_P(Qr, Bs)
+    stages["build_P"] = (time.perf_counter() - t0) * m / cp
+    if L is None or P is None:
+        L = np.sort(rng.uniform(1e-17, 0.2, n))[::-1].copy()
+        P = rng.standard_normal((n, m)) * 1e-3
+    pb = min(pix_bisect, m)
+    P2s = np.asfortranarray(np.asarray(P)[:, :pb] ** 2)
+    t0 = time.perf_counter()
+    K, _, _ = bisect_kappa(L, P2s, tile.C_a, tile.US_max, tile.kappa_min, tile.kappa_max,
+                           tile.US_tol, tile.maxNbis, tile.USB, tile.Ndiag)
+    stages["bisect"] = (time.perf_counter() - t0) * m / pb
+    pt = min(pix_T, m)
+    t0 = time.perf_counter()
+    T = build_T(Qr, np.asfortranarray(np.asarray(P)[:, :pt]), L, K[:pt] if pt <= pb else np.full(pt, K[0]))
+    build_S(T, tile.Ndiag)
+    build_image(T, tile.I_i)
+    stages["build_T"] = (time.perf_counter() - t0) * m / pt
+    return float(sum(stages.values())), stages
+
+
 def solve(tile, Q=None, L=None):
     """The whole reference solve path on one tile (imcom.f90:151-169).
     Returns a dict of every stage's output."""

@@ -1,0 +1,107 @@
+"""Learned kappa bracket on the GPU: forward and gradients of the hand-written
+MLP kernels against the numpy oracle, the feature kernel against numpy on the
+oracle's (L, P), the forced-miss fallback, and what a trained network saves.
+
+Tolerances: FP32 kernels vs the FP64 oracle -- forward outputs within 2e-4
+(absolute, outputs are O(1)), gradients within 2e-3 of the oracle's norm,
+features within 1e-4 (log10 units)."""
+import numpy as np
+import pytest
+
+from imcom_amd import kappa_net, synth
+from oracle import mlp_oracle as M
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(X, y, seed):
+    p = kappa_net.init_params(X, y, seed).astype(np.float64)
+    rng = np.random.default_rng(seed + 11)
+    p[:M.N_TRAIN] += rng.standard_normal(M.N_TRAIN) * 0.05
+    return p.astype(np.float32)
+
+
+def test_mlp_forward_and_grad_match_oracle(gpu_ctx):
+    print("[case] mlp forward/grad vs oracle", flush=True)
+    rng = np.random.default_rng(3)
+    X = (rng.standard_normal((700, 32)) * 2).astype(np.float32)
+    y = (rng.standard_normal(700) - 4).astype(np.float32)
+    p = _params(X, y, 4)
+    gpu_ctx.mlp_set_params(p)
+    out = gpu_ctx.mlp_infer(X)
+    ref, _ = M.forward(p.astype(np.float64), X.astype(np.float64))
+    assert np.max(np.abs(out - ref)) <= 2e-4
+    g, loss = gpu_ctx.mlp_grad(X, y)
+    gr = M.grad(p.astype(np.float64), X.astype(np.float64), y.astype(np.float64))
+    assert abs(loss - M.loss(p.astype(np.float64), X, y)) <= 1e-4 * max(1.0, abs(loss))
+    assert np.linalg.norm(g - gr) <= 2e-3 * np.linalg.norm(gr)
+    # every block of the layout carries gradient
+    for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_W2, M.OFF_B2), (M.OFF_W3, M.OFF_B3), (M.OFF_B3, M.N_TRAIN)):
+        assert np.linalg.norm(g[a:b] - gr[a:b]) <= 5e-3 * np.linalg.norm(gr[a:b]) + 1e-7
+
+
+def test_features_match_numpy(gpu_ctx):
+    print("[case] kappa features", flush=True)
+    t = synth.small()
+    r = O.solve(t)
+    gpu_ctx.set_tile(t)
+    gpu_ctx.set_QL(r["Q"], r["L"])
+    gpu_ctx.set_P(r["P"])
+    X = gpu_ctx.kappa_features(1e-6)
+    L, P2, C = r["L"], r["P2"], t.C_a
+    n = t.n
+    qidx = np.rint(np.arange(12) * (n - 1) / 11.0).astype(int)
+    ref_q = np.log10(np.abs(L[qidx]) / C + 1e-300)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = -np.log10(L / L[0])
+    b = np.where(L > 0, np.clip(np.floor(np.nan_to_num(d, nan=18.0)), 0, 18), 18).astype(int)
+    E = np.zeros((19, t.m))
+    np.add.at(E, b, P2)
+    ref_e = np.log10(E / C + 1e-30).T
+    assert np.max(np.abs(X[:, :12] - ref_q[None, :])) <= 1e-4
+    assert np.max(np.abs(X[:, 12:31] - ref_e)) <= 1e-4
+    assert np.all(np.abs(X[:, 31] + 6.0) <= 1e-6)
+
+
+def test_forced_miss_falls_back_to_full_range(gpu_ctx):
+    """A bracket far from every root is rejected by the bisection, which then
+    runs the reference's full-range search: same kappa, same U."""
+    print("[case] forced bracket miss", flush=True)
+    t = synth.small()
+    gpu_ctx.set_tile(t)
+    gpu_ctx.lookup_A(t.npoly, t.npad)
+    gpu_ctx.eigen_A(out=False)
+    gpu_ctx.lookup_B(t.npoly, t.npad)
+    gpu_ctx.build_P()
+    K0, U0, e0 = gpu_ctx.bisect_kappa()
+    X = gpu_ctx.kappa_features(t.US_max)
+    p = kappa_net.init_params(X, np.log10(K0).astype(np.float32), 0)
+    p[-2] += 8.0  # ymean moved 8 decades away from every root
+    gpu_ctx.mlp_set_params(p)
+    K1, U1, e1, hits = gpu_ctx.bisect_kappa_learned(zsig=0.5)
+    assert hits.sum() == 0
+    assert np.array_equal(K1, K0) and np.array_equal(U1, U0)
+    assert np.all(e1 >= e0) and np.all(e1 <= e0 + 2)
+
+
+def test_trained_bracket_saves_evaluations(gpu_ctx):
+    """Train on bisection results of seeded variants (PSF sigma in [1, 2],
+    U target log-uniform in [1e-8, 1e-4]) and compare on held-out tiles:
+    fewer U evaluations per pixel, and every pixel still passes the same
+    leakage test."""
+    print("[case] trained bracket gain", flush=True)
+    X, y = kappa_net.make_dataset(gpu_ctx, ntiles=10, targets_per_tile=6, seed0=500)
+    p, losses = kappa_net.train(gpu_ctx, X, y, epochs=60, batch=1024, lr=3e-3, seed=1)
+    print(f"[mlp] samples={len(y)} loss first={losses[0]:.3f} last={losses[-1]:.3f}", flush=True)
+    assert losses[-1] < losses[0]
+    gains = []
+    for s, U_t in ((9001, 1e-6), (9002, 3e-7), (9003, 2e-5)):
+        t = kappa_net.variant_tile(s)
+        gpu_ctx.mlp_set_params(p)
+        c = kappa_net.compare(gpu_ctx, t, U_t)
+        print(f"[mlp] tile {s} U_t={U_t:g}: {c}", flush=True)
+        assert c["all_U_ok_plain"] and c["all_U_ok_learned"]
+        gains.append(c["evals_plain"] - c["evals_learned"])
+        assert c["hit_rate"] >= 0.8
+    assert np.mean(gains) > 2.0
