@@ -68,11 +68,14 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
                     &ctx->Atab,  &ctx->Btab, &ctx->I,      &ctx->A,     &ctx->B,    &ctx->Q,
                     &ctx->L,     &ctx->P,    &ctx->K,      &ctx->US,    &ctx->T,    &ctx->W,
                     &ctx->H,     &ctx->S,    &ctx->U,      &ctx->evals, &ctx->bracket,
-                    &ctx->sW,    &ctx->sVW,  &ctx->sWV,    &ctx->sy,    &ctx->sycp, &ctx->swpre, &ctx->svtmp,
-                    &ctx->sd,    &ctx->se,   &ctx->stau,   &ctx->spart, &ctx->spart2, &ctx->ss,
-                    &ctx->sscal, &ctx->sVb,  &ctx->sSb,    &ctx->sTb,   &ctx->sY,   &ctx->sY2,
-                    &ctx->slam,  &ctx->sZ,   &ctx->stile,  &ctx->split_work};
+                    &ctx->split_work, &ctx->mlp_params, &ctx->mlp_feat, &ctx->mlp_bracket,
+                    &ctx->mlp_bins, &ctx->mlp_hits, &ctx->mlp_X, &ctx->mlp_Y, &ctx->mlp_perm,
+                    &ctx->mlp_loss, &ctx->mlp_grad, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_out,
+                    &ctx->ms_Xn, &ctx->ms_H1, &ctx->ms_dZ1, &ctx->ms_H2, &ctx->ms_dZ2,
+                    &ctx->ms_dY};
   for (DevBuf* b : bufs) b->release();
+  ctx->ew.release();
+  imcom_batch_free(ctx);
   dc_destroy(ctx->dc);
   ctx->prof.destroy();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -216,55 +219,9 @@ int imcom_gpu_set_B(imcom_gpu_ctx* ctx, const double* B) {
 }
 
 static hipError_t reserve_eigen(imcom_gpu_ctx* ctx, int64_t n, SyTile& t) {
-  const size_t d = 8;
-  IMCOM_HIP_TRY(ctx->sW.reserve(n * kNB * d));
-  IMCOM_HIP_TRY(ctx->sVW.reserve(n * 2 * kNB * d));
-  IMCOM_HIP_TRY(ctx->sWV.reserve(n * 2 * kNB * d));
-  const int64_t nbr = (n + 63) / 64;
-  IMCOM_HIP_TRY(ctx->sy.reserve((size_t)nbr * n * d));
-  IMCOM_HIP_TRY(ctx->sycp.reserve((size_t)((nbr + 7) / 8) * n * d));
-  IMCOM_HIP_TRY(ctx->swpre.reserve(n * d));
-  IMCOM_HIP_TRY(ctx->svtmp.reserve(n * d));
-  IMCOM_HIP_TRY(ctx->sd.reserve(n * d));
-  IMCOM_HIP_TRY(ctx->se.reserve(n * d));
-  IMCOM_HIP_TRY(ctx->stau.reserve(n * d));
-  IMCOM_HIP_TRY(ctx->spart.reserve((n / 256 + 2) * d));
-  IMCOM_HIP_TRY(ctx->spart2.reserve((n / 256 + 2) * d));
-  IMCOM_HIP_TRY(ctx->ss.reserve(2 * kNB * d));
-  IMCOM_HIP_TRY(ctx->sscal.reserve(d));
-  IMCOM_HIP_TRY(ctx->sVb.reserve((size_t)std::max<int64_t>(n - 1, 1) * kNBB * d));
-  IMCOM_HIP_TRY(ctx->sSb.reserve(kNBB * kNBB * d));
-  IMCOM_HIP_TRY(ctx->sTb.reserve(kNBB * kNBB * d));
-  IMCOM_HIP_TRY(ctx->sY.reserve(kNBB * n * d));
-  IMCOM_HIP_TRY(ctx->sY2.reserve(kNBB * n * d));
-  IMCOM_HIP_TRY(ctx->slam.reserve(n * d));
-  IMCOM_HIP_TRY(ctx->sZ.reserve((size_t)n * n * d));
-  IMCOM_HIP_TRY(ctx->stile.reserve(sizeof(SyTile)));
-  IMCOM_HIP_TRY(ctx->Q.reserve((size_t)n * n * d));
-  IMCOM_HIP_TRY(ctx->L.reserve(n * d));
-  t.n = n;
-  t.A = ctx->A.as<double>();
-  t.W = ctx->sW.as<double>();
-  t.VW = ctx->sVW.as<double>();
-  t.WV = ctx->sWV.as<double>();
-  t.ypart = ctx->sy.as<double>();
-  t.ycpart = ctx->sycp.as<double>();
-  t.wpre = ctx->swpre.as<double>();
-  t.vtmp = ctx->svtmp.as<double>();
-  t.d = ctx->sd.as<double>();
-  t.e = ctx->se.as<double>();
-  t.tau = ctx->stau.as<double>();
-  t.part = ctx->spart.as<double>();
-  t.part2 = ctx->spart2.as<double>();
-  t.s = ctx->ss.as<double>();
-  t.scal = ctx->sscal.as<double>();
-  t.Vb = ctx->sVb.as<double>();
-  t.Sb = ctx->sSb.as<double>();
-  t.Tb = ctx->sTb.as<double>();
-  t.Y = ctx->sY.as<double>();
-  t.Y2 = ctx->sY2.as<double>();
-  t.Z = ctx->Q.as<double>();
-  return hipSuccess;
+  IMCOM_HIP_TRY(ctx->Q.reserve((size_t)n * n * 8));
+  IMCOM_HIP_TRY(ctx->L.reserve((size_t)n * 8));
+  return ctx->ew.reserve(n, ctx->nb, ctx->nbb, ctx->A.as<double>(), ctx->Q.as<double>(), t);
 }
 
 int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* condition) {
@@ -274,30 +231,30 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
   const int64_t n = ctx->n;
   SyTile t;
   CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
-  CK(hipMemcpyAsync(ctx->stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
+  CK(hipMemcpyAsync(ctx->ew.stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
      "upload tile descriptor");
   std::vector<SyTile> host(1, t);
-  const SyTile* dt = ctx->stile.as<SyTile>();
+  const SyTile* dt = ctx->ew.stile.as<SyTile>();
   // 1. A = H T H^T (reflectors left in A's lower part)
   {
     StageTimer tm(ctx, PS_SYTRD);
-    CK(sytrd_run(ctx->stream, dt, 1, n, host, kNB, &ctx->prof), "tridiagonal reduction");
+    CK(sytrd_run(ctx->stream, dt, 1, n, host, ctx->nb, &ctx->prof), "tridiagonal reduction");
   }
   // 2. T = Z diag(lam) Z^T, ascending
   {
     StageTimer tm(ctx, PS_STEDC);
-    CK(dc_solve(ctx->stream, ctx->dc, n, ctx->sd.as<double>(), ctx->se.as<double>(),
-                ctx->slam.as<double>(), ctx->sZ.as<double>()),
+    CK(dc_solve(ctx->stream, ctx->dc, n, ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(),
+                ctx->ew.slam.as<double>(), ctx->ew.sZ.as<double>()),
        "divide-and-conquer");
   }
   {
     StageTimer tm(ctx, PS_ORMTR);
     // 3. descending order into Q, L (imcom_matrix.f90:656-663)
-    CK(launch_reverse_cols(ctx->stream, n, ctx->sZ.as<double>(), ctx->Q.as<double>(),
-                           ctx->slam.as<double>(), ctx->L.as<double>()),
+    CK(launch_reverse_cols(ctx->stream, n, ctx->ew.sZ.as<double>(), ctx->Q.as<double>(),
+                           ctx->ew.slam.as<double>(), ctx->L.as<double>()),
        "reverse");
     // 4. Q = H Q
-    CK(ormtr_run(ctx->stream, dt, 1, n, host, kNBB), "back-transform");
+    CK(ormtr_run(ctx->stream, dt, 1, n, host, ctx->nbb), "back-transform");
   }
   ctx->have_A = false;  // A was overwritten by the reduction
   ctx->have_QL = true;
@@ -324,14 +281,14 @@ int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out) {
   const int64_t n = ctx->n;
   SyTile t;
   CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
-  CK(hipMemcpyAsync(ctx->stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
+  CK(hipMemcpyAsync(ctx->ew.stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
      "upload tile descriptor");
   std::vector<SyTile> host(1, t);
-  CK(sytrd_run(ctx->stream, ctx->stile.as<SyTile>(), 1, n, host, kNB, &ctx->prof),
+  CK(sytrd_run(ctx->stream, ctx->ew.stile.as<SyTile>(), 1, n, host, ctx->nb, &ctx->prof),
      "tridiagonal reduction");
   ctx->have_A = false;
-  CK(download(ctx, d_out, ctx->sd, (size_t)n * 8), "download d");
-  if (n > 1) CK(download(ctx, e_out, ctx->se, (size_t)(n - 1) * 8), "download e");
+  CK(download(ctx, d_out, ctx->ew.sd, (size_t)n * 8), "download d");
+  if (n > 1) CK(download(ctx, e_out, ctx->ew.se, (size_t)(n - 1) * 8), "download e");
   CK(hipStreamSynchronize(ctx->stream), "sync");
   return IMCOM_OK;
 }
@@ -341,16 +298,16 @@ int imcom_gpu_tridiag_eig(imcom_gpu_ctx* ctx, int64_t n, const double* d, const 
   if (!ctx) return IMCOM_EARG;
   NEED(n >= 1 && n <= 32768 && d && (e || n == 1), IMCOM_EARG, "tridiag_eig: bad arguments");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
-  CK(upload(ctx, ctx->sd, d, (size_t)n * 8), "upload d");
-  CK(ctx->se.reserve((size_t)n * 8), "alloc e");
-  if (n > 1) CK(hipMemcpyAsync(ctx->se.p, e, (size_t)(n - 1) * 8, hipMemcpyHostToDevice, ctx->stream), "upload e");
-  CK(ctx->slam.reserve((size_t)n * 8), "alloc lam");
-  CK(ctx->sZ.reserve((size_t)n * n * 8), "alloc Z");
-  CK(dc_solve(ctx->stream, ctx->dc, n, ctx->sd.as<double>(), ctx->se.as<double>(),
-              ctx->slam.as<double>(), ctx->sZ.as<double>()),
+  CK(upload(ctx, ctx->ew.sd, d, (size_t)n * 8), "upload d");
+  CK(ctx->ew.se.reserve((size_t)n * 8), "alloc e");
+  if (n > 1) CK(hipMemcpyAsync(ctx->ew.se.p, e, (size_t)(n - 1) * 8, hipMemcpyHostToDevice, ctx->stream), "upload e");
+  CK(ctx->ew.slam.reserve((size_t)n * 8), "alloc lam");
+  CK(ctx->ew.sZ.reserve((size_t)n * n * 8), "alloc Z");
+  CK(dc_solve(ctx->stream, ctx->dc, n, ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(),
+              ctx->ew.slam.as<double>(), ctx->ew.sZ.as<double>()),
      "divide-and-conquer");
-  CK(download(ctx, lam_out, ctx->slam, (size_t)n * 8), "download lam");
-  CK(download(ctx, Z_out, ctx->sZ, (size_t)n * n * 8), "download Z");
+  CK(download(ctx, lam_out, ctx->ew.slam, (size_t)n * 8), "download lam");
+  CK(download(ctx, Z_out, ctx->ew.sZ, (size_t)n * n * 8), "download Z");
   CK(hipStreamSynchronize(ctx->stream), "sync");
   return IMCOM_OK;
 }

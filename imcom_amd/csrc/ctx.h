@@ -46,12 +46,75 @@ using imcom_ctx_detail::DevBuf;
 using imcom_ctx_detail::kNB;
 using imcom_ctx_detail::kNBB;
 
+// Device workspace of one tile's eigendecomposition (tridiagonal reduction,
+// divide-and-conquer output, back-transform), sized for n, panel width nb and
+// back-transform block nbb.
+struct EigWork {
+  DevBuf sW, sVW, sWV, sy, sycp, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb,
+      sTb, sY, sY2, slam, sZ, stile;
+  hipError_t reserve(int64_t n, int nb, int nbb, double* A, double* Z, SyTile& t) {
+    const size_t d = 8;
+    IMCOM_HIP_TRY(sW.reserve(n * nb * d));
+    IMCOM_HIP_TRY(sVW.reserve(n * 2 * nb * d));
+    IMCOM_HIP_TRY(sWV.reserve(n * 2 * nb * d));
+    const int64_t nbr = (n + 63) / 64;
+    IMCOM_HIP_TRY(sy.reserve((size_t)nbr * n * d));
+    IMCOM_HIP_TRY(sycp.reserve((size_t)((nbr + 7) / 8) * n * d));
+    IMCOM_HIP_TRY(swpre.reserve(n * d));
+    IMCOM_HIP_TRY(svtmp.reserve(n * d));
+    IMCOM_HIP_TRY(sd.reserve(n * d));
+    IMCOM_HIP_TRY(se.reserve(n * d));
+    IMCOM_HIP_TRY(stau.reserve(n * d));
+    IMCOM_HIP_TRY(spart.reserve((n / 256 + 2) * d));
+    IMCOM_HIP_TRY(spart2.reserve((n / 256 + 2) * d));
+    IMCOM_HIP_TRY(ss.reserve(2 * nb * d));
+    IMCOM_HIP_TRY(sscal.reserve(d));
+    IMCOM_HIP_TRY(sVb.reserve((size_t)(n > 1 ? n - 1 : 1) * nbb * d));
+    IMCOM_HIP_TRY(sSb.reserve((size_t)nbb * nbb * d));
+    IMCOM_HIP_TRY(sTb.reserve((size_t)nbb * nbb * d));
+    IMCOM_HIP_TRY(sY.reserve((size_t)nbb * n * d));
+    IMCOM_HIP_TRY(sY2.reserve((size_t)nbb * n * d));
+    IMCOM_HIP_TRY(slam.reserve(n * d));
+    IMCOM_HIP_TRY(sZ.reserve((size_t)n * n * d));
+    IMCOM_HIP_TRY(stile.reserve(sizeof(SyTile)));
+    t.n = n;
+    t.A = A;
+    t.W = sW.as<double>();
+    t.VW = sVW.as<double>();
+    t.WV = sWV.as<double>();
+    t.ypart = sy.as<double>();
+    t.ycpart = sycp.as<double>();
+    t.wpre = swpre.as<double>();
+    t.vtmp = svtmp.as<double>();
+    t.d = sd.as<double>();
+    t.e = se.as<double>();
+    t.tau = stau.as<double>();
+    t.part = spart.as<double>();
+    t.part2 = spart2.as<double>();
+    t.s = ss.as<double>();
+    t.scal = sscal.as<double>();
+    t.Vb = sVb.as<double>();
+    t.Sb = sSb.as<double>();
+    t.Tb = sTb.as<double>();
+    t.Y = sY.as<double>();
+    t.Y2 = sY2.as<double>();
+    t.Z = Z;
+    return hipSuccess;
+  }
+  void release() {
+    DevBuf* b[] = {&sW, &sVW, &sWV, &sy, &sycp, &swpre, &svtmp, &sd, &se, &stau, &spart,
+                   &spart2, &ss, &sscal, &sVb, &sSb, &sTb, &sY, &sY2, &slam, &sZ, &stile};
+    for (DevBuf* p : b) p->release();
+  }
+};
+
 struct imcom_gpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
   int64_t n = 0, m = 0;
   double C_a = 0.0;
+  int nb = kNB, nbb = kNBB;  // eigensolver tuning (imcom_gpu_set_tuning)
   std::vector<int32_t> h_exp;
   // tables
   int psfconst = 1, nexppsf = 1, n1 = 0, n2 = 0;
@@ -64,8 +127,7 @@ struct imcom_gpu_ctx {
   // stages
   DevBuf A, B, Q, L, P, K, US, T, W, H, S, U, evals, bracket;
   // eigensolver workspace
-  DevBuf sW, sVW, sWV, sy, sycp, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb, sTb,
-      sY, sY2, slam, sZ, stile;
+  EigWork ew;
   DcWork* dc = nullptr;
   DevBuf split_work;
   Prof prof;
@@ -75,7 +137,11 @@ struct imcom_gpu_ctx {
   DevBuf ms_Xn, ms_H1, ms_dZ1, ms_H2, ms_dZ2, ms_dY;
   bool have_mlp = false;
   int mlp_step = 0;
+  // batched tiles
+  struct BatchState* batch = nullptr;
 };
+
+void imcom_batch_free(imcom_gpu_ctx* ctx);
 
 // stage timing helper: events recorded only when profiling is on
 struct StageTimer {

@@ -217,3 +217,50 @@ hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_des
 }
 
 }  // namespace imcom
+
+namespace imcom {
+
+int GroupedPlan::add(bool ta, bool tb, const std::vector<GemmDesc>& groups) {
+  Launch L{(int)descs.size(), (int)prefix.size(), 0, 0, ta, tb};
+  prefix.push_back(0);
+  for (const GemmDesc& g : groups) {
+    if (g.M <= 0 || g.N <= 0) continue;
+    descs.push_back(g);
+    L.blocks += gemm_tiles64(g.M, g.N);
+    prefix.push_back(L.blocks);
+    L.ngroups += 1;
+  }
+  launches.push_back(L);
+  return (int)launches.size() - 1;
+}
+
+hipError_t GroupedPlan::upload(hipStream_t st) {
+  release();
+  if (!descs.empty()) {
+    IMCOM_HIP_TRY(hipMalloc(&d_descs, descs.size() * sizeof(GemmDesc)));
+    IMCOM_HIP_TRY(hipMemcpyAsync(d_descs, descs.data(), descs.size() * sizeof(GemmDesc),
+                                 hipMemcpyHostToDevice, st));
+  }
+  if (!prefix.empty()) {
+    IMCOM_HIP_TRY(hipMalloc(&d_prefix, prefix.size() * sizeof(int)));
+    IMCOM_HIP_TRY(hipMemcpyAsync(d_prefix, prefix.data(), prefix.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, st));
+  }
+  return hipStreamSynchronize(st);
+}
+
+hipError_t GroupedPlan::run(hipStream_t st, int i) const {
+  const Launch& L = launches[i];
+  if (L.ngroups == 0) return hipSuccess;
+  return dgemm_grouped(st, L.ta, L.tb, d_descs + L.desc_off, d_prefix + L.prefix_off, L.ngroups,
+                       L.blocks);
+}
+
+void GroupedPlan::release() {
+  if (d_descs) (void)hipFree(d_descs);
+  if (d_prefix) (void)hipFree(d_prefix);
+  d_descs = nullptr;
+  d_prefix = nullptr;
+}
+
+}  // namespace imcom

@@ -44,6 +44,9 @@ static hipError_t ensure_invden(int dev) {
   return hipSuccess;
 }
 
+// Loads the weight table on `dev` (idempotent; call before any graph capture).
+hipError_t interp_init(int dev) { return ensure_invden(dev); }
+
 // Lagrange weights at 1-based table coordinate `arg` for nodes lo..lo+cnt-1.
 __device__ __forceinline__ void lagrange8(double arg, int lo, int cnt, double w[8]) {
   double d[8];

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <vector>
 
 namespace imcom {
@@ -60,6 +61,25 @@ int gemm_tiles64(int M, int N);
 hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
                          const int* d_prefix, int ngroups, int total_blocks);
 
+// A sequence of grouped GEMM launches whose descriptors are uploaded once
+// (before any graph capture) and replayed by index.
+struct GroupedPlan {
+  struct Launch {
+    int desc_off, prefix_off, ngroups, blocks;
+    bool ta, tb;
+  };
+  std::vector<GemmDesc> descs;
+  std::vector<int> prefix;
+  std::vector<Launch> launches;
+  GemmDesc* d_descs = nullptr;
+  int* d_prefix = nullptr;
+  // append one launch made of `groups` (empty groups are dropped); returns its index
+  int add(bool ta, bool tb, const std::vector<GemmDesc>& groups);
+  hipError_t upload(hipStream_t st);  // (re)allocates and copies; synchronises
+  hipError_t run(hipStream_t st, int launch) const;
+  void release();
+};
+
 // Split-precision GEMM (FP32 x3 or BF16 x6 products on MFMA with FP32
 // accumulation and FP64 recombination); scheme 1 = FP32x3, 2 = BF16x6.
 hipError_t gemm_split(hipStream_t st, int scheme, bool ta, int M, int N, int K, double alpha,
@@ -68,6 +88,7 @@ hipError_t gemm_split(hipStream_t st, int scheme, bool ta, int M, int N, int K, 
 size_t gemm_split_work_bytes(int scheme, int M, int N, int K);
 
 // ---- (a) interpolation ----------------------------------------------------
+hipError_t interp_init(int dev);
 hipError_t launch_lookup_A(hipStream_t st, int dev, int64_t n, const double* x, const double* y,
                            const int32_t* ex, int psfconst, const double* tabs, int64_t tstride,
                            int n1, int n2, int npm, int npad, double sx, double sy, double* A);
@@ -122,6 +143,17 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
 hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nbb);
 
+struct SyPlan {
+  GroupedPlan gp;
+  std::vector<int> syr2k;            // grouped launch per panel
+  std::vector<std::array<int, 4>> bt;  // grouped launches per back-transform block
+};
+void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb, SyPlan& plan);
+hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
+                             int nb, const SyPlan& plan, Prof* prof);
+hipError_t ormtr_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
+                             int nbb, const SyPlan& plan);
+
 // Divide-and-conquer symmetric tridiagonal eigensolver (ascending
 // eigenvalues, eigenvectors in Z, ld n).  `ws` is the workspace object.
 struct DcWork;
@@ -129,6 +161,9 @@ DcWork* dc_create();
 void dc_destroy(DcWork* w);
 hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const double* e,
                     double* lam, double* Z);
+hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t* ns,
+                            const double* const* d_t, const double* const* e_t,
+                            double* const* lam_t, double* const* Z_t);
 
 // ---- learned kappa bracket ---------------------------------------------------
 struct MlpScratch {

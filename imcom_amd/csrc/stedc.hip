@@ -35,7 +35,10 @@
 namespace imcom {
 
 struct DcMerge {
-  int off, n1, n2;
+  int off, n1, n2;   // off: offset in the concatenated vectors of all tiles
+  double* Z;         // this merge's tile eigenvector matrix
+  int64_t ldz;       // its leading dimension (the tile's n)
+  int loff;          // offset of the merge inside its tile
   double rho;    // |beta|
   double sigma;  // sign(beta)
   int64_t u_off;   // offset of this merge's k x k U block
@@ -49,24 +52,25 @@ struct DcCount {
 };
 
 struct DcWork {
-  int64_t n = 0;
+  int64_t n = 0;    // capacity of the concatenated vectors
+  int64_t nn = 0;   // capacity of the matrix workspaces (sum of n_t^2)
   double *D = nullptr, *ds = nullptr, *zs = nullptr, *dl = nullptr, *wz = nullptr,
          *dfd = nullptr, *rc = nullptr, *rs = nullptr, *tau = nullptr, *lam = nullptr,
-         *zhat = nullptr, *U = nullptr, *Zg = nullptr, *Zo = nullptr, *scale = nullptr;
+         *zhat = nullptr, *U = nullptr, *Zg = nullptr, *Zo = nullptr, *scale = nullptr,
+         *dcat = nullptr, *ecat = nullptr;
   int *cs = nullptr, *ndcol = nullptr, *dfcol = nullptr, *rp = nullptr, *rn = nullptr,
       *org = nullptr, *d_prefix = nullptr;
   DcMerge* d_merges = nullptr;
   DcCount* d_counts = nullptr;
   GemmDesc* d_descs = nullptr;
-  int cap_merges = 0;
   DcCount* h_counts = nullptr;  // pinned
 };
 
 static void dc_free(DcWork* w) {
   void* ptrs[] = {w->D,   w->ds,    w->zs,    w->dl,       w->wz,       w->dfd,    w->rc,
                   w->rs,  w->tau,   w->lam,   w->zhat,     w->U,        w->Zg,     w->Zo,
-                  w->scale, w->cs,  w->ndcol, w->dfcol,    w->rp,       w->rn,     w->org,
-                  w->d_prefix, w->d_merges, w->d_counts, w->d_descs};
+                  w->scale, w->dcat, w->ecat, w->cs,  w->ndcol, w->dfcol,    w->rp,
+                  w->rn,     w->org, w->d_prefix, w->d_merges, w->d_counts, w->d_descs};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (w->h_counts) (void)hipHostFree(w->h_counts);
@@ -80,39 +84,29 @@ void dc_destroy(DcWork* w) {
   delete w;
 }
 
-static hipError_t dc_reserve(DcWork* w, int64_t n) {
-  if (w->n >= n && w->cap_merges >= n) return hipSuccess;
+static hipError_t dc_reserve(DcWork* w, int64_t n, int64_t nn) {
+  if (w->n >= n && w->nn >= nn) return hipSuccess;
+  n = std::max(n, w->n);
+  nn = std::max(nn, w->nn);
   dc_free(w);
   const size_t nd = (size_t)n * sizeof(double), ni = (size_t)n * sizeof(int);
-  const size_t nn = (size_t)n * (size_t)n * sizeof(double);
-  IMCOM_HIP_TRY(hipMalloc(&w->D, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->ds, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->zs, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->dl, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->wz, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->dfd, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->rc, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->rs, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->tau, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->lam, nd));
-  IMCOM_HIP_TRY(hipMalloc(&w->zhat, nd));
+  const size_t nnb = (size_t)nn * sizeof(double);
+  double** dv[] = {&w->D, &w->ds, &w->zs, &w->dl, &w->wz, &w->dfd, &w->rc, &w->rs,
+                   &w->tau, &w->lam, &w->zhat, &w->dcat, &w->ecat};
+  for (double** q : dv) IMCOM_HIP_TRY(hipMalloc(q, nd));
   IMCOM_HIP_TRY(hipMalloc(&w->scale, sizeof(double) * 2));
-  IMCOM_HIP_TRY(hipMalloc(&w->U, nn));
-  IMCOM_HIP_TRY(hipMalloc(&w->Zg, nn));
-  IMCOM_HIP_TRY(hipMalloc(&w->Zo, nn));
-  IMCOM_HIP_TRY(hipMalloc(&w->cs, ni));
-  IMCOM_HIP_TRY(hipMalloc(&w->ndcol, ni));
-  IMCOM_HIP_TRY(hipMalloc(&w->dfcol, ni));
-  IMCOM_HIP_TRY(hipMalloc(&w->rp, ni));
-  IMCOM_HIP_TRY(hipMalloc(&w->rn, ni));
-  IMCOM_HIP_TRY(hipMalloc(&w->org, ni));
+  IMCOM_HIP_TRY(hipMalloc(&w->U, nnb));
+  IMCOM_HIP_TRY(hipMalloc(&w->Zg, nnb));
+  IMCOM_HIP_TRY(hipMalloc(&w->Zo, nnb));
+  int** iv[] = {&w->cs, &w->ndcol, &w->dfcol, &w->rp, &w->rn, &w->org};
+  for (int** q : iv) IMCOM_HIP_TRY(hipMalloc(q, ni));
   IMCOM_HIP_TRY(hipMalloc(&w->d_prefix, ni + sizeof(int)));
   IMCOM_HIP_TRY(hipMalloc(&w->d_merges, (size_t)n * sizeof(DcMerge)));
   IMCOM_HIP_TRY(hipMalloc(&w->d_counts, (size_t)n * sizeof(DcCount)));
   IMCOM_HIP_TRY(hipMalloc(&w->d_descs, (size_t)n * sizeof(GemmDesc)));
   IMCOM_HIP_TRY(hipHostMalloc(&w->h_counts, (size_t)n * sizeof(DcCount)));
   w->n = n;
-  w->cap_merges = (int)n;
+  w->nn = nn;
   return hipSuccess;
 }
 
@@ -152,10 +146,8 @@ __global__ void dc_leaves_kernel(int64_t n, const double* __restrict__ d,
 }
 
 __global__ void dc_identity_kernel(int64_t n, double* __restrict__ Z) {
-  const int64_t j = blockIdx.y;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    Z[i + j * n] = (i == j) ? 1.0 : 0.0;
+  const int64_t j = blockIdx.x;
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) Z[i + j * n] = (i == j) ? 1.0 : 0.0;
 }
 
 __device__ __forceinline__ int find_merge(const DcMerge* __restrict__ mg, int nm, int64_t g) {
@@ -201,7 +193,7 @@ __global__ void dc_sort_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm
         hi = mid;
     }
     rank = q + lo;
-    z = Z[(m.off + m.n1 - 1) + g * n] * r2;
+    z = m.Z[(m.loff + m.n1 - 1) + (int64_t)(m.loff + q) * m.ldz] * r2;
   } else {
     // rank in D1: count of D1 <= v
     int lo = 0, hi = m.n1;
@@ -213,7 +205,7 @@ __global__ void dc_sort_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm
         hi = mid;
     }
     rank = (q - m.n1) + lo;
-    z = m.sigma * Z[(m.off + m.n1) + g * n] * r2;
+    z = m.sigma * m.Z[(m.loff + m.n1) + (int64_t)(m.loff + q) * m.ldz] * r2;
   }
   ds[m.off + rank] = v;
   zs[m.off + rank] = z;
@@ -339,8 +331,9 @@ __global__ void dc_rotate_kernel(int64_t n, const DcMerge* __restrict__ mg, int 
   const int nr = counts[mi].nrot;
   const int o = m.off;
   for (int r = 0; r < nr; ++r) {
-    double* x = Z + g + (int64_t)(o + rp[o + r]) * n;
-    double* y = Z + g + (int64_t)(o + rn[o + r]) * n;
+    const int64_t row = m.loff + (g - m.off);
+    double* x = m.Z + row + (int64_t)(m.loff + rp[o + r]) * m.ldz;
+    double* y = m.Z + row + (int64_t)(m.loff + rn[o + r]) * m.ldz;
     const double c = rc[o + r], s = rs[o + r];
     const double xv = *x, yv = *y;
     *x = c * xv + s * yv;
@@ -556,7 +549,7 @@ __global__ void dc_gather_kernel(int64_t n, const DcMerge* __restrict__ mg, int 
                                  const DcCount* __restrict__ counts, const int* __restrict__ ndcol,
                                  const int* __restrict__ dfcol, const double* __restrict__ Z,
                                  double* __restrict__ Zg) {
-  const int64_t g = blockIdx.y;
+  const int64_t g = blockIdx.x;
   const int mi = find_merge(mg, nm, g);
   if (mi < 0) return;
   const DcMerge m = mg[mi];
@@ -564,10 +557,9 @@ __global__ void dc_gather_kernel(int64_t n, const DcMerge* __restrict__ mg, int 
   const int k = counts[mi].k;
   const int q = (int)(g - m.off);
   const int src = (q < k) ? ndcol[m.off + q] : dfcol[m.off + (q - k)];
-  const double* zc = Z + m.off + (int64_t)(m.off + src) * n;
+  const double* zc = m.Z + m.loff + (int64_t)(m.loff + src) * m.ldz;
   double* out = Zg + m.zg_off + (int64_t)q * K;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < K; r += gridDim.x * blockDim.x)
-    out[r] = zc[r];
+  for (int r = threadIdx.x; r < K; r += blockDim.x) out[r] = zc[r];
 }
 
 // ---- step 8: place eigenpairs in ascending order ----------------------------
@@ -578,7 +570,7 @@ __global__ void dc_place_kernel(int64_t n, const DcMerge* __restrict__ mg, int n
                                 const double* __restrict__ dfd, const double* __restrict__ Zg,
                                 const double* __restrict__ Zo, double* __restrict__ D,
                                 double* __restrict__ Z) {
-  const int64_t g = blockIdx.y;
+  const int64_t g = blockIdx.x;
   const int mi = find_merge(mg, nm, g);
   if (mi < 0) return;
   const DcMerge m = mg[mi];
@@ -617,10 +609,9 @@ __global__ void dc_place_kernel(int64_t n, const DcMerge* __restrict__ mg, int n
     pos = jd + lo;
     src = Zg + m.zg_off + (int64_t)q * K;
   }
-  double* dst = Z + m.off + (int64_t)(m.off + pos) * n;
-  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < K; r += gridDim.x * blockDim.x)
-    dst[r] = src[r];
-  if (blockIdx.x == 0 && threadIdx.x == 0) D[m.off + pos] = val;
+  double* dst = m.Z + m.loff + (int64_t)(m.loff + pos) * m.ldz;
+  for (int r = threadIdx.x; r < K; r += blockDim.x) dst[r] = src[r];
+  if (threadIdx.x == 0) D[m.off + pos] = val;
 }
 
 __global__ void dc_unscale_kernel(int64_t n, const double* __restrict__ D,
@@ -631,71 +622,95 @@ __global__ void dc_unscale_kernel(int64_t n, const double* __restrict__ D,
 
 // ---- driver -----------------------------------------------------------------
 struct Node {
-  int off, size, height;
+  int tile, off, size, height;
 };
 
-static int build_tree(int off, int size, std::vector<std::vector<Node>>& levels) {
+static int build_tree(int tile, int off, int size, std::vector<std::vector<Node>>& levels) {
   if (size <= 1) return 0;
   const int n1 = (size + 1) / 2;
-  const int h1 = build_tree(off, n1, levels);
-  const int h2 = build_tree(off + n1, size - n1, levels);
+  const int h1 = build_tree(tile, off, n1, levels);
+  const int h2 = build_tree(tile, off + n1, size - n1, levels);
   const int h = std::max(h1, h2) + 1;
   if ((int)levels.size() < h) levels.resize(h);
-  levels[h - 1].push_back(Node{off, size, h});
+  levels[h - 1].push_back(Node{tile, off, size, h});
   return h;
 }
 
-hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const double* e,
-                    double* lam, double* Z) {
-  if (n <= 0) return hipSuccess;
-  IMCOM_HIP_TRY(dc_reserve(w, n));
-  hipLaunchKernelGGL(dc_scale_kernel, dim3(1), dim3(1024), 0, st, n, d, e, w->scale);
-  hipLaunchKernelGGL(dc_leaves_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, d, e, w->scale,
-                     w->D);
-  hipLaunchKernelGGL(dc_identity_kernel, dim3(std::min<int64_t>(ceil_div(n, 256), 64), (unsigned)n),
-                     dim3(256), 0, st, n, Z);
-  std::vector<std::vector<Node>> levels;
-  build_tree(0, (int)n, levels);
-  // host copy of e for the coupling signs/magnitudes
-  std::vector<double> he(std::max<int64_t>(n - 1, 1));
-  double hscale = 1.0;
-  if (n > 1) {
-    IMCOM_HIP_TRY(hipMemcpyAsync(he.data(), e, (n - 1) * sizeof(double), hipMemcpyDeviceToHost, st));
-    IMCOM_HIP_TRY(hipMemcpyAsync(&hscale, w->scale, sizeof(double), hipMemcpyDeviceToHost, st));
-    IMCOM_HIP_TRY(hipStreamSynchronize(st));
+// Tiles' tridiagonals (d_t: n_t, e_t: n_t - 1, device) solved together: their
+// vectors are concatenated (with zero coupling between tiles), each tile keeps
+// its own balanced merge tree, and merges of equal height across all tiles
+// share the launches of a level.  lam_t (n_t) ascending, Z_t (n_t x n_t).
+hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t* ns,
+                            const double* const* d_t, const double* const* e_t,
+                            double* const* lam_t, double* const* Z_t) {
+  std::vector<int64_t> base(ntiles + 1, 0);
+  int64_t nn = 0;
+  for (int t = 0; t < ntiles; ++t) {
+    base[t + 1] = base[t] + ns[t];
+    nn += ns[t] * ns[t];
   }
+  const int64_t N = base[ntiles];
+  if (N <= 0) return hipSuccess;
+  IMCOM_HIP_TRY(dc_reserve(w, N, nn));
+  IMCOM_HIP_TRY(hipMemsetAsync(w->ecat, 0, (size_t)N * sizeof(double), st));
+  for (int t = 0; t < ntiles; ++t) {
+    IMCOM_HIP_TRY(hipMemcpyAsync(w->dcat + base[t], d_t[t], ns[t] * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st));
+    if (ns[t] > 1)
+      IMCOM_HIP_TRY(hipMemcpyAsync(w->ecat + base[t], e_t[t], (ns[t] - 1) * sizeof(double),
+                                   hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(dc_identity_kernel, dim3((unsigned)ns[t]), dim3(256), 0, st, ns[t], Z_t[t]);
+  }
+  hipLaunchKernelGGL(dc_scale_kernel, dim3(1), dim3(1024), 0, st, N, w->dcat, w->ecat, w->scale);
+  hipLaunchKernelGGL(dc_leaves_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->dcat, w->ecat,
+                     w->scale, w->D);
+  std::vector<std::vector<Node>> levels;
+  for (int t = 0; t < ntiles; ++t) build_tree(t, 0, (int)ns[t], levels);
+  std::vector<double> he(N, 0.0);
+  double hscale = 1.0;
+  IMCOM_HIP_TRY(hipMemcpyAsync(he.data(), w->ecat, N * sizeof(double), hipMemcpyDeviceToHost, st));
+  IMCOM_HIP_TRY(hipMemcpyAsync(&hscale, w->scale, sizeof(double), hipMemcpyDeviceToHost, st));
+  IMCOM_HIP_TRY(hipStreamSynchronize(st));
   std::vector<DcMerge> hm;
   std::vector<int> rprefix, bprefix;
   std::vector<GemmDesc> descs;
   for (size_t L = 0; L < levels.size(); ++L) {
     std::vector<Node>& lv = levels[L];
-    std::sort(lv.begin(), lv.end(), [](const Node& a, const Node& b) { return a.off < b.off; });
+    std::sort(lv.begin(), lv.end(), [&](const Node& a, const Node& b) {
+      return base[a.tile] + a.off < base[b.tile] + b.off;
+    });
     const int nm = (int)lv.size();
     hm.resize(nm);
     int64_t zoff = 0;
+    int maxK = 0;
     for (int i = 0; i < nm; ++i) {
-      const int n1 = (lv[i].size + 1) / 2;
-      const double beta = he[lv[i].off + n1 - 1] / hscale;
-      hm[i].off = lv[i].off;
+      const Node& nd = lv[i];
+      const int n1 = (nd.size + 1) / 2;
+      const int64_t g = base[nd.tile] + nd.off;
+      const double beta = he[g + n1 - 1] / hscale;
+      hm[i].off = (int)g;
       hm[i].n1 = n1;
-      hm[i].n2 = lv[i].size - n1;
+      hm[i].n2 = nd.size - n1;
+      hm[i].Z = Z_t[nd.tile];
+      hm[i].ldz = ns[nd.tile];
+      hm[i].loff = nd.off;
       hm[i].rho = fabs(beta);
       hm[i].sigma = beta < 0.0 ? -1.0 : 1.0;
       hm[i].zg_off = zoff;
       hm[i].u_off = 0;
       hm[i].root_off = 0;
-      zoff += (int64_t)lv[i].size * lv[i].size;
+      zoff += (int64_t)nd.size * nd.size;
+      maxK = std::max(maxK, nd.size);
     }
     IMCOM_HIP_TRY(hipMemcpyAsync(w->d_merges, hm.data(), nm * sizeof(DcMerge), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(dc_sort_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, w->d_merges, nm,
-                       w->D, Z, w->ds, w->zs, w->cs);
+    hipLaunchKernelGGL(dc_sort_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->d_merges, nm,
+                       w->D, (const double*)nullptr, w->ds, w->zs, w->cs);
     hipLaunchKernelGGL(dc_deflate_kernel, dim3(ceil_div(nm, 64)), dim3(64), 0, st, w->d_merges, nm,
                        w->ds, w->zs, w->cs, w->dl, w->wz, w->ndcol, w->dfd, w->dfcol, w->rp, w->rn,
                        w->rc, w->rs, w->d_counts);
     IMCOM_HIP_TRY(hipMemcpyAsync(w->h_counts, w->d_counts, nm * sizeof(DcCount),
                                  hipMemcpyDeviceToHost, st));
     IMCOM_HIP_TRY(hipStreamSynchronize(st));
-    // per-merge root prefix and U offsets
     rprefix.assign(nm + 1, 0);
     int64_t uoff = 0;
     bool anyrot = false;
@@ -712,8 +727,8 @@ hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const
     IMCOM_HIP_TRY(hipMemcpyAsync(w->d_prefix, rprefix.data(), (nm + 1) * sizeof(int),
                                  hipMemcpyHostToDevice, st));
     if (anyrot)
-      hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, w->d_merges,
-                         nm, w->d_counts, w->rp, w->rn, w->rc, w->rs, Z);
+      hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->d_merges,
+                         nm, w->d_counts, w->rp, w->rn, w->rc, w->rs, (double*)nullptr);
     if (total > 0) {
       hipLaunchKernelGGL(dc_secular_kernel, dim3(ceil_div(total, 64)), dim3(64), 0, st, w->d_merges,
                          w->d_counts, w->d_prefix, nm, total, w->dl, w->wz, w->org, w->tau, w->lam);
@@ -722,10 +737,9 @@ hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const
       hipLaunchKernelGGL(dc_vec_kernel, dim3(total), dim3(256), 0, st, w->d_merges, w->d_counts,
                          w->d_prefix, nm, w->dl, w->org, w->tau, w->zhat, w->U);
     }
-    int maxK = 0;
-    for (int i = 0; i < nm; ++i) maxK = std::max(maxK, lv[i].size);
-    hipLaunchKernelGGL(dc_gather_kernel, dim3(std::max(1, std::min(ceil_div(maxK, 256), 16)), (unsigned)n),
-                       dim3(256), 0, st, n, w->d_merges, nm, w->d_counts, w->ndcol, w->dfcol, Z, w->Zg);
+    const int tpb = std::min(256, std::max(64, (maxK + 63) / 64 * 64));
+    hipLaunchKernelGGL(dc_gather_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, w->d_merges, nm,
+                       w->d_counts, w->ndcol, w->dfcol, (const double*)nullptr, w->Zg);
     // grouped GEMM: Zo_m (K x k) = Zg_m(:, 0:k) (K x k) * U_m (k x k)
     descs.clear();
     bprefix.assign(1, 0);
@@ -740,20 +754,26 @@ hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const
     if (ng > 0) {
       IMCOM_HIP_TRY(hipMemcpyAsync(w->d_descs, descs.data(), ng * sizeof(GemmDesc),
                                    hipMemcpyHostToDevice, st));
-      // prefix goes after the root prefix (both small; reuse the same buffer region)
       IMCOM_HIP_TRY(hipMemcpyAsync(w->rp, bprefix.data(), (ng + 1) * sizeof(int),
                                    hipMemcpyHostToDevice, st));
       IMCOM_HIP_TRY(dgemm_grouped(st, false, false, w->d_descs, w->rp, ng, bprefix.back()));
     }
-    hipLaunchKernelGGL(dc_place_kernel, dim3(std::max(1, std::min(ceil_div(maxK, 256), 16)), (unsigned)n),
-                       dim3(256), 0, st, n, w->d_merges, nm, w->d_counts, w->lam, w->dfd, w->Zg,
-                       w->Zo, w->D, Z);
+    hipLaunchKernelGGL(dc_place_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, w->d_merges, nm,
+                       w->d_counts, w->lam, w->dfd, w->Zg, w->Zo, w->D, (double*)nullptr);
     // host vectors (hm, descs, prefixes) must outlive the async copies
     IMCOM_HIP_TRY(hipStreamSynchronize(st));
   }
-  hipLaunchKernelGGL(dc_unscale_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, st, n, w->D, w->scale,
-                     lam);
+  hipLaunchKernelGGL(dc_unscale_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->D, w->scale,
+                     w->lam);
+  for (int t = 0; t < ntiles; ++t)
+    IMCOM_HIP_TRY(hipMemcpyAsync(lam_t[t], w->lam + base[t], ns[t] * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st));
   return hipGetLastError();
+}
+
+hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const double* e,
+                    double* lam, double* Z) {
+  return dc_solve_batched(st, w, 1, &n, &d, &e, &lam, &Z);
 }
 
 }  // namespace imcom

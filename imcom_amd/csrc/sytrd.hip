@@ -416,3 +416,106 @@ hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
 }
 
 }  // namespace imcom
+
+namespace imcom {
+
+// ---- batched tiles ------------------------------------------------------------
+// The per-panel trailing update and the four GEMMs of each back-transform
+// block become one grouped launch across the tiles; the descriptors are
+// built and uploaded once per batch shape.
+void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb, SyPlan& plan) {
+  plan.gp = GroupedPlan();
+  plan.syr2k.clear();
+  plan.bt.clear();
+  for (int64_t k = 0; k < nmax - 1; k += nb) {
+    const int64_t r0 = k + nb;
+    std::vector<GemmDesc> g;
+    for (const SyTile& h : host) {
+      if (r0 >= h.n) continue;
+      const int mm = (int)(h.n - r0);
+      g.push_back(GemmDesc{mm, mm, 2 * nb, -1.0, 1.0, h.VW + r0, h.n, h.WV + r0, h.n,
+                           h.A + r0 + r0 * h.n, h.n});
+    }
+    plan.syr2k.push_back(plan.gp.add(false, true, g));
+  }
+  if (nmax > 1) {
+    const int64_t nblk = (nmax - 1 + nbb - 1) / nbb;
+    for (int64_t bi = nblk - 1; bi >= 0; --bi) {
+      const int kb = (int)(bi * nbb);
+      std::vector<GemmDesc> gs, gy, gy2, gz;
+      for (const SyTile& h : host) {
+        if (kb >= h.n - 1) continue;
+        const int rows = (int)(h.n - kb - 1);
+        const int n = (int)h.n;
+        double* Zr = h.Z + kb + 1;
+        gs.push_back(GemmDesc{nbb, nbb, rows, 1.0, 0.0, h.Vb, rows, h.Vb, rows, h.Sb, nbb});
+        gy.push_back(GemmDesc{nbb, n, rows, 1.0, 0.0, h.Vb, rows, Zr, h.n, h.Y, nbb});
+        gy2.push_back(GemmDesc{nbb, n, nbb, 1.0, 0.0, h.Tb, nbb, h.Y, nbb, h.Y2, nbb});
+        gz.push_back(GemmDesc{rows, n, nbb, -1.0, 1.0, h.Vb, rows, h.Y2, nbb, Zr, h.n});
+      }
+      std::array<int, 4> ids;
+      ids[0] = plan.gp.add(true, false, gs);
+      ids[1] = plan.gp.add(true, false, gy);
+      ids[2] = plan.gp.add(false, false, gy2);
+      ids[3] = plan.gp.add(false, false, gz);
+      plan.bt.push_back(ids);
+    }
+  }
+}
+
+hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
+                             int nb, const SyPlan& plan, Prof* prof) {
+  size_t tok = 0;
+  int panel = 0;
+  for (int64_t k = 0; k < nmax - 1; k += nb, ++panel) {
+    const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
+    hipLaunchKernelGGL(sy_colupd, dim3(ceil_div(nmax - k, RB), 1, ntiles), dim3(RB), 0, st,
+                       d_tiles, (int)k, (int)k, 0);
+    for (int jj = 0; jj < pb; ++jj) {
+      const int64_t c = k + jj;
+      const int nparts = ceil_div(nmax - c, RB);
+      const int nbr = ceil_div(nmax - c - 1, SB);
+      const int gx = std::max(ceil_div(nbr, SEG), ceil_div(2 * jj, 4));
+      if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYMV, 2, &tok));
+      hipLaunchKernelGGL(sy_symv, dim3(std::max(gx, 1), nbr + 1, ntiles), dim3(256), 0, st,
+                         d_tiles, (int)k, (int)c, jj, nparts);
+      if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
+      const int nb2 = std::max(ceil_div(nmax - c - 1, RB), 1);
+      hipLaunchKernelGGL(sy_wpre, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
+                         jj);
+      hipLaunchKernelGGL(sy_wfin, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
+                         jj, nb2, (jj + 1 < pb) ? 1 : 0);
+    }
+    const int64_t r0 = k + nb;
+    if (r0 < nmax) {
+      hipLaunchKernelGGL(sy_pack, dim3(ceil_div(nmax - r0, 256), nb, ntiles), dim3(256), 0, st,
+                         d_tiles, (int)k, nb, r0);
+      if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYR2K, 2, &tok));
+      IMCOM_HIP_TRY(plan.gp.run(st, plan.syr2k[panel]));
+      if (prof) IMCOM_HIP_TRY(prof->end(PS_SYR2K, tok));
+    }
+  }
+  hipLaunchKernelGGL(sy_last_diag, dim3(1, 1, ntiles), dim3(1), 0, st, d_tiles);
+  return hipGetLastError();
+}
+
+hipError_t ormtr_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
+                             int nbb, const SyPlan& plan) {
+  if (nmax <= 1) return hipSuccess;
+  const int64_t nblk = (nmax - 1 + nbb - 1) / nbb;
+  int idx = 0;
+  for (int64_t bi = nblk - 1; bi >= 0; --bi, ++idx) {
+    const int kb = (int)(bi * nbb);
+    const std::array<int, 4>& ids = plan.bt[idx];
+    hipLaunchKernelGGL(bt_gather, dim3(ceil_div(nmax - kb - 1, 256), nbb, ntiles), dim3(256), 0,
+                       st, d_tiles, kb, nbb);
+    IMCOM_HIP_TRY(plan.gp.run(st, ids[0]));
+    hipLaunchKernelGGL(bt_build_T, dim3(1, 1, ntiles), dim3(256), 0, st, d_tiles, kb, nbb);
+    IMCOM_HIP_TRY(plan.gp.run(st, ids[1]));
+    IMCOM_HIP_TRY(plan.gp.run(st, ids[2]));
+    IMCOM_HIP_TRY(plan.gp.run(st, ids[3]));
+  }
+  return hipGetLastError();
+}
+
+}  // namespace imcom

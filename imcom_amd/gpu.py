@@ -34,7 +34,26 @@ EXPORTS = [
     "imcom_gpu_stage_times", "imcom_gpu_mlp_nparams", "imcom_gpu_mlp_set_params",
     "imcom_gpu_mlp_get_params", "imcom_gpu_kappa_features", "imcom_gpu_mlp_infer",
     "imcom_gpu_mlp_grad", "imcom_gpu_mlp_train", "imcom_gpu_bisect_kappa_learned",
+    "imcom_gpu_batch_set", "imcom_gpu_batch_solve", "imcom_gpu_batch_result",
+    "imcom_gpu_set_tuning",
 ]
+
+
+class TileDesc(ctypes.Structure):
+    """struct imcom_gpu_tile (include/imcom_gpu.h)."""
+    _fields_ = [("n", ctypes.c_int64), ("m", ctypes.c_int64),
+                ("x_i", ctypes.c_void_p), ("y_i", ctypes.c_void_p), ("exp_i", ctypes.c_void_p),
+                ("X_a", ctypes.c_void_p), ("Y_a", ctypes.c_void_p), ("Ndiag", ctypes.c_void_p),
+                ("I_i", ctypes.c_void_p),
+                ("C_a", ctypes.c_double), ("kappa_min", ctypes.c_double),
+                ("kappa_max", ctypes.c_double), ("US_max", ctypes.c_double),
+                ("US_tol", ctypes.c_double),
+                ("UnotS", ctypes.c_int), ("maxnits", ctypes.c_int),
+                ("psfconst", ctypes.c_int), ("nexppsf", ctypes.c_int), ("n1", ctypes.c_int),
+                ("n2", ctypes.c_int),
+                ("Alookup", ctypes.c_void_p), ("Blookup", ctypes.c_void_p),
+                ("npoly", ctypes.c_int), ("npad", ctypes.c_int),
+                ("psfxscale", ctypes.c_double), ("psfyscale", ctypes.c_double)]
 
 STAGE_SLOTS = ["lookup_A", "sytrd", "stedc", "ormtr", "lookup_B", "build_P", "bisect",
                "build_T", "maps", "symv", "syr2k"]
@@ -98,6 +117,10 @@ def load_library(path: str = LIB_PATH):
                                        ctypes.c_uint64, _fp]),
         "imcom_gpu_bisect_kappa_learned": (_int, [_vp, _int, _dbl, _dbl, _dbl, _dbl, _int,
                                                   ctypes.c_float, _dp, _dp, _ip, _ip]),
+        "imcom_gpu_batch_set": (_int, [_vp, _int, ctypes.POINTER(TileDesc)]),
+        "imcom_gpu_batch_solve": (_int, [_vp, _int, _int, _int, ctypes.c_float, _int]),
+        "imcom_gpu_batch_result": (_int, [_vp, _int, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
+        "imcom_gpu_set_tuning": (_int, [_vp, _int, _int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -280,6 +303,50 @@ class ImcomGPU:
         self._ck(self.lib.imcom_gpu_stage_times(self.h, _ptr(ms), cnt.ctypes.data_as(ctypes.POINTER(_i64)), k),
                  "stage_times")
         return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(STAGE_SLOTS)}
+
+    def set_tuning(self, nb=32, nbb=128):
+        self._ck(self.lib.imcom_gpu_set_tuning(self.h, nb, nbb), "set_tuning")
+
+    # ---- batched tiles ----------------------------------------------------
+    def batch_set(self, tiles):
+        """Upload many tiles (each a tile.Tile) for one batched solve."""
+        self._batch_keep = []
+        descs = (TileDesc * len(tiles))()
+        for d, t in zip(descs, tiles):
+            arrs = dict(x_i=_vec(t.x_i), y_i=_vec(t.y_i), exp_i=_vec(t.exp_i, np.int32),
+                        X_a=_vec(t.X_a), Y_a=_vec(t.Y_a), Ndiag=_vec(t.Ndiag), I_i=_vec(t.I_i),
+                        Alookup=_mat(t.Alookup), Blookup=_mat(t.Blookup))
+            self._batch_keep.append(arrs)
+            for k, a in arrs.items():
+                setattr(d, k, a.ctypes.data)
+            d.n, d.m = t.n, t.m
+            d.C_a, d.kappa_min, d.kappa_max = t.C_a, t.kappa_min, t.kappa_max
+            d.US_max, d.US_tol = t.US_max, t.US_tol
+            d.UnotS, d.maxnits = (1 if t.USB else 0), t.maxNbis
+            d.psfconst, d.nexppsf = t.psfconst, t.nexppsf
+            d.n1, d.n2 = t.Alookup.shape[0], t.Alookup.shape[1]
+            d.npoly, d.npad = t.npoly, t.npad
+            d.psfxscale, d.psfyscale = t.psfxscale, t.psfyscale
+        self._ck(self.lib.imcom_gpu_batch_set(self.h, len(tiles), descs), "batch_set")
+        self.batch_tiles = list(tiles)
+
+    def batch_solve(self, precision_P=IMCOM_PREC_FP64, precision_T=IMCOM_PREC_FP64,
+                    learned=False, zsig=3.0, use_graph=True):
+        self._ck(self.lib.imcom_gpu_batch_solve(self.h, precision_P, precision_T,
+                                                1 if learned else 0, zsig, 1 if use_graph else 0),
+                 "batch_solve")
+
+    def batch_result(self, i, want_T=False):
+        t = self.batch_tiles[i]
+        m = t.m
+        K, US, SU, H = (np.empty(m) for _ in range(4))
+        ev, hits = np.empty(m, dtype=np.int32), np.empty(m, dtype=np.int32)
+        T = np.empty((t.n, m), order="F") if want_T else None
+        self._ck(self.lib.imcom_gpu_batch_result(self.h, i, _ptr(K), _ptr(US), _ptr(SU), _ptr(H),
+                                                 _ptr(ev, _ip), _ptr(hits, _ip), _ptr(T)),
+                 "batch_result")
+        U, S = (US, SU) if t.USB else (SU, US)
+        return dict(K=K, U=U, S=S, H=H, evals=ev, hits=hits, T=T)
 
     # ---- learned kappa bracket --------------------------------------------
     def mlp_nparams(self):

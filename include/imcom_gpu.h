@@ -142,6 +142,42 @@ int imcom_gpu_bisect_kappa_learned(imcom_gpu_ctx* ctx, int UnotS, double US_max,
                                    double* K_out, double* US_out, int32_t* evals_out,
                                    int32_t* hits_out);
 
+/* Batched tiles.  No reference counterpart (the reference solves one tile per
+ * process); each tile's result is what the single-tile entry points give for
+ * it.  A tile carries its own inputs (the fields of imcom_gpu_set_tile and
+ * imcom_gpu_set_tables, the image I_i, the target and bisection controls of
+ * imcom_gpu_bisect_kappa, and the interpolation controls of
+ * imcom_gpu_lookup_A/B).  imcom_gpu_batch_set uploads them;
+ * imcom_gpu_batch_solve runs get_A, get_QL, get_B, get_P, get_T and
+ * build_image for every tile (use_graph: replay the launch sequences as HIP
+ * graphs, captured on the first call); imcom_gpu_batch_result returns one
+ * tile's K, U or S at the root (US_out), the other map (SU_out: S in U mode,
+ * U in S mode), H, evaluation counts, bracket hits and T (any may be NULL). */
+typedef struct imcom_gpu_tile {
+  int64_t n, m;
+  const double *x_i, *y_i;
+  const int32_t* exp_i;
+  const double *X_a, *Y_a, *Ndiag, *I_i;
+  double C_a, kappa_min, kappa_max, US_max, US_tol;
+  int UnotS, maxnits;
+  int psfconst, nexppsf, n1, n2;
+  const double *Alookup, *Blookup;
+  int npoly, npad;
+  double psfxscale, psfyscale;
+} imcom_gpu_tile;
+
+/* Eigensolver tuning picked by the autotuner (imcom_amd/autotune.py): the
+ * reduction panel width nb (16, 32, 64) and the back-transform block nbb
+ * (64, 128, 256); applies to later eigen_A calls and batch_set. */
+int imcom_gpu_set_tuning(imcom_gpu_ctx* ctx, int nb, int nbb);
+
+int imcom_gpu_batch_set(imcom_gpu_ctx* ctx, int ntiles, const imcom_gpu_tile* tiles);
+int imcom_gpu_batch_solve(imcom_gpu_ctx* ctx, int precision_P, int precision_T, int learned,
+                          float zsig, int use_graph);
+int imcom_gpu_batch_result(imcom_gpu_ctx* ctx, int tile, double* K_out, double* US_out,
+                           double* SU_out, double* H_out, int32_t* evals_out, int32_t* hits_out,
+                           double* T_out);
+
 /* Device-time accounting (no reference counterpart; the reference prints
  * progress lines only).  level 0: off; 1: per-stage HIP-event times; 2: also
  * every launch of the tridiagonal reduction's matrix-vector kernel and its
