@@ -65,8 +65,8 @@ struct EigWork {
     IMCOM_HIP_TRY(sd.reserve(n * d));
     IMCOM_HIP_TRY(se.reserve(n * d));
     IMCOM_HIP_TRY(stau.reserve(n * d));
-    IMCOM_HIP_TRY(spart.reserve((n / 256 + 2) * d));
-    IMCOM_HIP_TRY(spart2.reserve((n / 256 + 2) * d));
+    IMCOM_HIP_TRY(spart.reserve(kNormParts * d));
+    IMCOM_HIP_TRY(spart2.reserve(kQParts * d));
     IMCOM_HIP_TRY(ss.reserve(2 * nb * d));
     IMCOM_HIP_TRY(sscal.reserve(d));
     IMCOM_HIP_TRY(sVb.reserve((size_t)(n > 1 ? n - 1 : 1) * nbb * d));
@@ -91,6 +91,7 @@ struct EigWork {
     t.tau = stau.as<double>();
     t.part = spart.as<double>();
     t.part2 = spart2.as<double>();
+    t.qpart = spart2.as<double>();
     t.s = ss.as<double>();
     t.scal = sscal.as<double>();
     t.Vb = sVb.as<double>();

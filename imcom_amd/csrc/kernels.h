@@ -121,6 +121,7 @@ struct SyTile {
   double* WV;     // n x 2NB  [W V]
   double* ypart;  // ceil(n/64) x n   row partials of A22 v
   double* ycpart; // ceil(n/512) x n  column partials of A22 v
+  double* qpart;  // kQParts            per-workgroup partials of v^T A22 v
   double* wpre;   // n
   double* vtmp;   // n
   double* d;      // n     tridiagonal diagonal
@@ -137,6 +138,11 @@ struct SyTile {
   double* Y2;     // NBB x n
   double* Z;      // n x n  eigenvectors (in: of the tridiagonal, out: of A)
 };
+
+// Partial-sum slots of the reduction kernels, sized for the largest tile
+// (n <= 32768) so that tiles of a batch can share launch grids.
+constexpr int kQParts = 40000;
+constexpr int kNormParts = 160;
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nb, Prof* prof);

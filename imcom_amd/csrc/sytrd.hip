@@ -13,14 +13,14 @@
 // matrix is kept in memory; the matrix-vector product reads only its lower
 // triangle, in 64x64 blocks, each block serving its rows and its columns.
 //
-// Three launches per column:
+// Two launches per column:
 //   sy_symv      reflector from the column's norm partials, partial sums of
-//                y = A22 v over the lower triangle, the 2jj dot products
-//                V^T v, W^T v
-//   sy_wpre      y from its partials, w_pre = tau (y - V sW - W sV) and
-//                partials of w_pre^T v
-//   sy_wfin      w = w_pre + alpha v; rank-2(jj+1) update of the next column
-//                and its norm partials
+//                y = A22 v and of v^T A22 v over the lower triangle, the 2jj
+//                dot products V^T v, W^T v
+//   sy_wupd      w = tau (y - V sW - W sV) + alpha v with
+//                alpha = -1/2 tau^2 (v^T A22 v - 2 sV.sW) (the same value as
+//                -1/2 tau w_pre^T v, without a second grid-wide reduction);
+//                rank-2(jj+1) update of the next column and its norm partials
 // All sums are fixed-order (partials per workgroup, reduced in workgroup
 // order), so runs are bitwise reproducible.
 //
@@ -101,7 +101,7 @@ constexpr int SEG = 8;   // blocks of one block-column per workgroup
 // I in [J + seg*SEG, J + (seg+1)*SEG) of block-column J: block (I,J) gives the
 // rows of block I the product A_IJ v_J (written to ypart[J][rows]) and, for
 // I > J, gives the columns of block J the product A_IJ^T v_I (accumulated over
-// the segment and written to ycpart[seg][cols]).  sy_wpre sums the partials
+// the segment and written to ycpart[seg][cols]).  sy_wupd sums the partials
 // in a fixed order.  Workgroups in row y = nbr compute the 2jj dot products
 // V^T v and W^T v, one wavefront each.
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
@@ -118,7 +118,9 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
   const int nbr = (int)((s + SB - 1) / SB);
   const int J = blockIdx.y;
   const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+  const int64_t qslot = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
   if (J >= nbr) {
+    if (tid == 0) t.qpart[qslot] = 0.0;
     const int q = blockIdx.x * 4 + ty;
     if (J == nbr && q < 2 * jj) {
       const double* X = q < jj ? t.A + (int64_t)(k + q) * n : t.W + (int64_t)(q - jj) * n;
@@ -130,7 +132,10 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
     return;
   }
   const int I0 = J + blockIdx.x * SEG;
-  if (I0 >= nbr) return;
+  if (I0 >= nbr) {
+    if (tid == 0) t.qpart[qslot] = 0.0;
+    return;
+  }
   const int I1 = min(I0 + SEG, nbr);
   // v in A22-local indexing: v(0) = 1, v(r) = scal * A(c+1+r, c)
   if (tid < SB) {
@@ -142,6 +147,7 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
   double colacc[16];
 #pragma unroll
   for (int q = 0; q < 16; ++q) colacc[q] = 0.0;
+  double qacc = 0.0;  // this workgroup's share of v^T A22 v
   for (int I = I0; I < I1; ++I) {
     const int64_t r = (int64_t)I * SB + tx;
     const bool rv = r < s;
@@ -155,13 +161,18 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
 #pragma unroll
     for (int q = 0; q < 16; ++q) rowp += a[q] * vJ[ty + 4 * q];
     red[ty][tx] = rowp;
+    const double vI = rv ? (r == 0 ? 1.0 : scal * colc[c + 1 + r]) : 0.0;
     if (I > J) {
-      const double vI = rv ? (r == 0 ? 1.0 : scal * colc[c + 1 + r]) : 0.0;
 #pragma unroll
       for (int q = 0; q < 16; ++q) colacc[q] += a[q] * vI;
     }
     __syncthreads();
-    if (ty == 0 && rv) t.ypart[(int64_t)J * n + r] = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
+    if (ty == 0 && rv) {
+      const double rs = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
+      t.ypart[(int64_t)J * n + r] = rs;
+      // v_I^T A_IJ v_J, counted twice off the diagonal (the mirrored block)
+      qacc += (I == J ? 1.0 : 2.0) * vI * rs;
+    }
     __syncthreads();
   }
 #pragma unroll
@@ -169,6 +180,11 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
     const double cs = wave_sum(colacc[q]);
     const int64_t col = (int64_t)J * SB + ty + 4 * q;
     if (tx == 0 && col < s) t.ycpart[(int64_t)blockIdx.x * n + col] = cs;
+  }
+  {
+    __shared__ double qred[4];
+    const double qt = block_sum(qacc, qred);
+    if (tid == 0) t.qpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = qt;
   }
   if (blockIdx.x == 0 && J == 0 && tid == 0) {
     t.d[c] = colc[c];
@@ -178,82 +194,74 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
   }
 }
 
-// y from the partials of sy_symv, v into vtmp, and
-// w_pre = tau (y - V sW - W sV) with partials of w_pre^T v.
-__global__ __launch_bounds__(RB) void sy_wpre(const SyTile* __restrict__ tiles, int k, int c,
-                                              int jj) {
-  __shared__ double red[RB / 64];
-  const SyTile t = tiles[blockIdx.z];
+// w before its v-correction at row i of the trailing matrix:
+//   tau (y_i - sum_p V(i,p) sW_p - W(i,p) sV_p),
+// y_i summed from sy_symv's partials in a fixed order.
+__device__ __forceinline__ double sy_wrow(const SyTile& t, int k, int64_t c, int jj, double tau,
+                                          int64_t i) {
   const int64_t n = t.n;
-  if (c >= n - 1) return;
-  const int64_t i = c + 1 + (int64_t)blockIdx.x * RB + threadIdx.x;
-  const double tau = t.tau[c];
-  const double scal = t.scal[0];
-  double contrib = 0.0;
-  if (i < n) {
-    const double* colc = t.A + (int64_t)c * n;
-    const double v = (i == c + 1) ? 1.0 : colc[i] * scal;
-    const int64_t s = n - c - 1;
-    const int nbr = (int)((s + SB - 1) / SB);
-    const int64_t r = i - (c + 1);
-    const int R = (int)(r / SB);
-    double acc = 0.0;
-    for (int J = 0; J <= R; ++J) acc += t.ypart[(int64_t)J * n + r];
-    const int nseg = (nbr - R + SEG - 1) / SEG;
-    for (int sg = 0; sg < nseg; ++sg) acc += t.ycpart[(int64_t)sg * n + r];
-    for (int p = 0; p < jj; ++p) {
-      const double Vip = t.A[i + (int64_t)(k + p) * n];
-      const double Wip = t.W[i + (int64_t)p * n];
-      acc -= Vip * t.s[jj + p] + Wip * t.s[p];
-    }
-    const double w = tau * acc;
-    t.wpre[i] = w;
-    t.vtmp[i] = v;
-    contrib = w * v;
+  const int64_t s = n - c - 1;
+  const int nbr = (int)((s + SB - 1) / SB);
+  const int64_t r = i - (c + 1);
+  const int R = (int)(r / SB);
+  double acc = 0.0;
+  for (int J = 0; J <= R; ++J) acc += t.ypart[(int64_t)J * n + r];
+  const int nseg = (nbr - R + SEG - 1) / SEG;
+  for (int sg = 0; sg < nseg; ++sg) acc += t.ycpart[(int64_t)sg * n + r];
+  for (int p = 0; p < jj; ++p) {
+    const double Vip = t.A[i + (int64_t)(k + p) * n];
+    const double Wip = t.W[i + (int64_t)p * n];
+    acc -= Vip * t.s[jj + p] + Wip * t.s[p];
   }
-  const double tot = block_sum(contrib, red);
-  if (threadIdx.x == 0) t.part2[blockIdx.x] = tot;
+  return tau * acc;
 }
 
-// w = w_pre - 1/2 tau (w_pre^T v) v into W(:, jj); v into A(:, c); then (if
-// requested) update column c+1 with reflectors 0..jj and its norm partials.
-__global__ __launch_bounds__(RB) void sy_wfin(const SyTile* __restrict__ tiles, int k, int c,
-                                              int jj, int nparts2, int in_panel_next) {
+// Finish column c and start column c+1 in one pass over rows [c+1, n):
+//   w = tau (y - V sW - W sV) + alpha v,
+//   alpha = -1/2 tau (w_pre^T v) = -1/2 tau^2 (v^T A22 v - 2 sV.sW)
+// (v^T A22 v from sy_symv's per-workgroup partials), stored in W(:, jj); v into
+// A(:, c); then, when column c+1 is in this panel or is the tile's last column,
+// its rank-2(jj+1) update and its norm partials.
+__global__ __launch_bounds__(RB) void sy_wupd(const SyTile* __restrict__ tiles, int k, int c,
+                                              int jj, int nqparts, int in_panel_next) {
   __shared__ double red[RB / 64];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
   if (c >= n - 1) return;
-  // column c+1 is updated here when it is in this panel, or when it is the
-  // tile's last column (no reflector of its own; its diagonal is d(n-1))
   const int upd_next = in_panel_next || (c + 1 == n - 1);
-  double wv = 0.0;
-  for (int q = 0; q < nparts2; ++q) wv += t.part2[q];
   const double tau = t.tau[c];
-  const double alpha = -0.5 * tau * wv;
+  const double scal = t.scal[0];
+  double qs = 0.0;
+  for (int q = threadIdx.x; q < nqparts; q += RB) qs += t.qpart[q];
+  const double vAv = block_sum(qs, red);
+  double svw = 0.0;
+  for (int p = 0; p < jj; ++p) svw += t.s[p] * t.s[jj + p];
+  const double alpha = -0.5 * tau * tau * (vAv - 2.0 * svw);
   const int64_t c1 = c + 1;
   const int64_t i = c1 + (int64_t)blockIdx.x * RB + threadIdx.x;
   double* W = t.W;
   double a = 0.0;
   if (i < n) {
-    const double v = t.vtmp[i];
-    const double w = t.wpre[i] + alpha * v;
-    W[i + (int64_t)jj * n] = w;
-    t.A[i + (int64_t)c * n] = v;
+    const double* colc = t.A + (int64_t)c * n;
+    const double v = (i == c1) ? 1.0 : colc[i] * scal;
+    const double w = sy_wrow(t, k, c, jj, tau, i) + alpha * v;
     if (upd_next) {
-      // row c+1 of V and W for p <= jj (read from the untouched copies)
+      // W(c+1, jj): the same row formula at row c+1, where v = 1
+      const double Wc1 = sy_wrow(t, k, c, jj, tau, c1) + alpha;
       a = t.A[i + c1 * n];
-      double s = 0.0;
+      double sacc = 0.0;
       for (int p = 0; p < jj; ++p) {
         const double* Vp = t.A + (int64_t)(k + p) * n;
         const double* Wp = W + (int64_t)p * n;
-        s += Vp[i] * Wp[c1] + Wp[i] * Vp[c1];
+        sacc += Vp[i] * Wp[c1] + Wp[i] * Vp[c1];
       }
-      // p = jj: V(c1, jj) = 1 and W(c1, jj) = wpre[c1] + alpha
-      const double Wc1 = t.wpre[c1] + alpha;
-      s += v * Wc1 + w * 1.0;
-      a -= s;
-      t.A[i + c1 * n] = a;
+      sacc += v * Wc1 + w * 1.0;  // p = jj: V(c+1, jj) = 1
+      a -= sacc;
     }
+    // each thread reads and writes only its own row of columns c, c+1 and W(:, jj)
+    W[i + (int64_t)jj * n] = w;
+    t.A[i + (int64_t)c * n] = v;
+    if (upd_next) t.A[i + c1 * n] = a;
   }
   if (upd_next) {
     const double sq = (i >= c1 + 2 && i < n) ? a * a : 0.0;
@@ -350,10 +358,9 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
                          d_tiles, (int)k, (int)c, jj, nparts);
       if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
       const int nb2 = std::max(ceil_div(nmax - c - 1, RB), 1);
-      hipLaunchKernelGGL(sy_wpre, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
-                         jj);
-      hipLaunchKernelGGL(sy_wfin, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
-                         jj, nb2, (jj + 1 < pb) ? 1 : 0);
+      const int nq = std::max(gx, 1) * (nbr + 1);  // sy_symv's workgroup count
+      hipLaunchKernelGGL(sy_wupd, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
+                         jj, nq, (jj + 1 < pb) ? 1 : 0);
     }
     (void)rbl;
     // trailing update A22 -= [V W] [W V]^T over rows/cols [k+nb, n)
@@ -481,10 +488,9 @@ hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, 
                          d_tiles, (int)k, (int)c, jj, nparts);
       if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
       const int nb2 = std::max(ceil_div(nmax - c - 1, RB), 1);
-      hipLaunchKernelGGL(sy_wpre, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
-                         jj);
-      hipLaunchKernelGGL(sy_wfin, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
-                         jj, nb2, (jj + 1 < pb) ? 1 : 0);
+      const int nq = std::max(gx, 1) * (nbr + 1);  // sy_symv's workgroup count
+      hipLaunchKernelGGL(sy_wupd, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
+                         jj, nq, (jj + 1 < pb) ? 1 : 0);
     }
     const int64_t r0 = k + nb;
     if (r0 < nmax) {
