@@ -49,7 +49,36 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    demo = build_fortran_demo()
+    if verbose and demo:
+        print("built", demo)
     return LIB
+
+
+FORTRAN = os.path.join(HERE, "fortran")
+DEMO = os.path.join(BUILD, "imcom_gpu_demo")
+
+
+def build_fortran_demo() -> str | None:
+    """Compile the ISO_C_BINDING module and the Fortran driver
+    (fortran/imcom_gpu_demo.f90) against libimcom_hip.so when flang is
+    installed; returns the executable's path or None without flang."""
+    fc = shutil.which("flang") or shutil.which("flang-new")
+    if fc is None:
+        return None
+    mod = os.path.join(BUILD, "fmod")
+    os.makedirs(mod, exist_ok=True)
+    iface = os.path.join(mod, "imcom_gpu_iface.o")
+    steps = [
+        [fc, "-O2", "-J", mod, "-c", os.path.join(FORTRAN, "imcom_gpu_iface.f90"), "-o", iface],
+        [fc, "-O2", "-I", mod, "-J", mod, os.path.join(FORTRAN, "imcom_gpu_demo.f90"), iface,
+         f"-L{HERE}", "-limcom_hip", "-Wl,-rpath,$ORIGIN/..", "-o", DEMO],
+    ]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed: {' '.join(cmd)}\n{r.stderr[-4000:]}")
+    return DEMO
 
 
 if __name__ == "__main__":
