@@ -63,7 +63,21 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "small", "batch"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB 5 times (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nbytes = args.copy_calibration << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "launches": 5}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); tiles shard "
               "across GPUs with no communication, see DESIGN section 7", file=sys.stderr)
@@ -90,15 +104,40 @@ def main():
     learned = weights is not None
 
     if args.workload == "batch":
+        # fast: every tile in one batched solve (lockstep launches, one HIP graph);
+        # plain: the single-tile FP64 path tile after tile, each tile's inputs
+        # resident in its own context before the timed region
         g.batch_set(tiles)
+        singles = []
+        if not args.skip_plain:
+            for t in tiles:
+                gs = ImcomGPU(0)
+                autotune.apply(gs, t.n)
+                gs.set_tile(t)
+                singles.append(gs)
+        Umax_all = np.concatenate([np.full(t.m, t.US_max) for t in tiles])
+
+        def single_step(gs, t):
+            gs.lookup_A(t.npoly, t.npad)
+            gs.eigen_A(out=False)
+            gs.lookup_B(t.npoly, t.npad)
+            gs.build_P()
+            K, U, ev = gs.bisect_kappa()
+            gs.build_T(IMCOM_PREC_FP64, out=False)
+            gs.build_S()
+            gs.build_image()
+            return U, ev
 
         def step(fast):
-            g.batch_solve(precision_T=IMCOM_PREC_BF16X6 if fast else IMCOM_PREC_FP64,
-                          learned=fast and learned, use_graph=True)
+            if not fast:
+                res = [single_step(gs, t) for gs, t in zip(singles, tiles)]
+                return (np.concatenate([r[0] for r in res]), np.concatenate([r[1] for r in res]),
+                        Umax_all)
+            g.batch_solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
             res = [g.batch_result(i) for i in range(len(tiles))]
             U = np.concatenate([r["U"] for r in res])
             ev = np.concatenate([r["evals"] for r in res])
-            return U, ev, np.concatenate([np.full(t.m, t.US_max) for t in tiles])
+            return U, ev, Umax_all
     else:
         g.set_tile(tile)
 
@@ -194,8 +233,12 @@ def main():
 
     cfg = {"workload": (f"{args.workload}: {len(tiles)} tile(s), n={tile.n if len(tiles) == 1 else 'mixed'}"
                         f" input px, sum m={m_total} output px, U_max={tile.US_max}"),
-           "path": ("learned kappa bracket + BF16x6 split GEMM for T" if learned
-                    else "BF16x6 split GEMM for T (no trained bracket weights found)"),
+           "path": (("batched tiles in one HIP graph + " if args.workload == "batch" else "")
+                    + ("learned kappa bracket + BF16x6 split GEMM for T" if learned
+                       else "BF16x6 split GEMM for T (no trained bracket weights found)")
+                    + "; P and the bisection FP64"),
+           "plain_path": ("single-tile FP64 path, tile after tile" if args.workload == "batch"
+                          else "single-tile FP64 path, full-range bisection"),
            "pixels_U_ge_Umax": n_bad, "mean_U_evaluations": round(mean_ev, 2)}
     if plain is not None:
         cfg["plain_fp64_px_per_s"] = round(m_total / plain[0], 2)
@@ -209,6 +252,9 @@ def main():
         "data": "synthetic", "config": cfg, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.workload == "batch":
+        for gs in singles:
+            gs.close()
     g.close()
 
 

@@ -6,9 +6,15 @@ The workloads (n = input pixels, m = output pixels):
                128^2 image at 0.25 px, Gaussian target sigma=1.5, 24x24 output
                at 0.5 px (m=576), noise sigma 1e-3, U_max=1e-6.
 * ``large``    4 exposures of 64x64 (n=16384), Airy PSF (obscuration 0.3,
-               lambda/D = 0.5 px) convolved with the unit pixel on a 256^2
+               lambda/D = 1.0 px) convolved with the unit pixel on a 256^2
                image at 0.25 px, Gaussian target sigma=1.8, 96x96 output at
-               0.5 px (m=9216), U_max=1e-6.
+               0.25 px (m=9216, a 24 px square centred in the 64 px
+               footprint), U_max=1e-6.  Both choices make U <= 1e-6 reachable
+               at every output pixel: at lambda/D = 0.5 px the aliased Airy
+               power leaves a leakage floor near 2e-4, and the Airy wings need
+               about 20 px of inputs around an output pixel to bring the floor
+               below 1e-6 (a 0.5 px output pitch would leave the grid's edge
+               8 px from the footprint's).
 * ``batch``    tiles with 2-6 exposures of 16-32 px squares (n in
                [1024, 6144]), m in [256, 2304], sigma_psf ~ U[1.0, 1.6],
                noise sigma log-uniform in [1e-4, 1e-2].
@@ -112,9 +118,10 @@ def small(seed=20261204, **kw) -> Tile:
 
 
 def large(seed=20261205, **kw) -> Tile:
-    """4 x 64^2 inputs (n=16384), 96^2 output (m=9216), Airy x pixel PSF."""
-    args = dict(nexp=4, npix_in=64, psf_kind="airy", gamma_sigma=1.8, psf_npix=256,
-                nout=96, noise_sigma=1e-3, seed=seed, name="large")
+    """4 x 64^2 inputs (n=16384), 96^2 output at 0.25 px (m=9216), Airy x pixel
+    PSF with lambda/D = 1 input pixel."""
+    args = dict(nexp=4, npix_in=64, psf_kind="airy", lam_over_d=1.0, gamma_sigma=1.8,
+                psf_npix=256, nout=96, outscale=0.25, noise_sigma=1e-3, seed=seed, name="large")
     args.update(kw)
     return make_tile(**args)
 

@@ -4,7 +4,7 @@
 
 Training pairs come from the plain bisection on seeded variants: small tiles
 (4 x 16^2 inputs, PSF sigma in [1, 2]) and, for the large workload, variants of
-the large tile (4 x 64^2, Airy x pixel PSF, lambda/D in [0.45, 0.55]), each at
+the large tile (4 x 64^2, Airy x pixel PSF, lambda/D in [0.9, 1.1]), each at
 several leakage targets log-uniform in [1e-8, 1e-4].  Held-out tiles measure
 the gain (U evaluations per pixel, bracket hit rate) and check that every
 pixel passes the same leakage test with and without the bracket.  Writes
@@ -47,7 +47,7 @@ def main(argv=None):
         if k % 10 == 0:
             print(f"small tile {k}: {sum(len(v) for v in ys)} pairs, {time.time() - t0:.0f}s", flush=True)
     for k in range(a.large_tiles):
-        lod = float(rng.uniform(0.45, 0.55))
+        lod = float(rng.uniform(0.9, 1.1))
         t = synth.large(seed=40000 + k, lam_over_d=lod)
         X, y = kappa_net.bisection_pairs(g, t, 10.0 ** rng.uniform(-8, -4, max(2, a.targets // 2)))
         Xs.append(X)

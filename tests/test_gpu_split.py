@@ -41,10 +41,11 @@ def test_split_P_random(gpu_ctx, scheme, n, m):
 
 @pytest.mark.parametrize("scheme", [IMCOM_PREC_FP32X3, IMCOM_PREC_BF16X6])
 def test_split_T_U_S_error_small_tile(gpu_ctx, scheme):
-    """Max error of T, U, S (and H) of the split schemes against the FP64
-    oracle on the small tile.  P = Q^T B and T = Q W both run split; the
-    bisection runs on the split P.  T is compared at the same kappa (GEMM
-    error only); U, S, H include the effect of the split P on kappa."""
+    """Both GEMMs split (P = Q^T B and T = Q W), the bisection on the split P:
+    max error of T, U, S and H against the FP64 oracle on the small tile.
+    A relative error d in P moves U by about d (U = 1 - (~1)), and T's
+    1/(lambda + kappa) factor amplifies P's error along the small eigenvalues,
+    so the bounds are loose (measured values are printed)."""
     print(f"[case] split T/U/S error scheme={scheme}", flush=True)
     t = synth.small()
     r = O.solve(t)
@@ -56,12 +57,46 @@ def test_split_T_U_S_error_small_tile(gpu_ctx, scheme):
     T = gpu_ctx.build_T(precision=scheme)
     S = gpu_ctx.build_S()
     H = gpu_ctx.build_image()
-    relT = float(np.max(np.abs(T - O.build_T(r["Q"], r["P"], r["L"], K))) / np.max(np.abs(r["T"])))
+    relT = float(np.max(np.abs(T - r["T"])) / np.max(np.abs(r["T"])))
     dU = float(np.max(np.abs(U - r["U"])))
     relS = float(np.max(np.abs(S - r["S"]) / r["S"]))
     relH = float(np.max(np.abs(H - r["H"])) / np.max(np.abs(r["H"])))
+    # leakage of the split T itself, in FP64: U = (C + T^T A T + B^T T) / C
+    A, B = r["A"], r["B"]
+    Utrue = (t.C_a + np.einsum("ia,ia->a", T, A @ T) + np.einsum("ia,ia->a", B, T)) / t.C_a
     print(f"[split-error] scheme={scheme} relT={relT:.3e} absU={dU:.3e} relS={relS:.3e} "
-          f"relH={relH:.3e} kappa_changed={(np.abs(K / r['K'] - 1) > 1e-12).mean():.3f}", flush=True)
-    assert relT <= 1e-4
-    assert dU <= 1e-6
-    assert relS <= 1e-4
+          f"relH={relH:.3e} max U_true={Utrue.max():.6e} "
+          f"kappa_changed={(np.abs(K / r['K'] - 1) > 1e-12).mean():.3f}", flush=True)
+    assert relT <= 1e-2
+    assert dU <= 1e-8
+    assert relS <= 1e-2
+    assert relH <= 1e-2
+    assert np.max(np.abs(Utrue - U)) <= 1e-8
+
+
+@pytest.mark.parametrize("scheme", [IMCOM_PREC_FP32X3, IMCOM_PREC_BF16X6])
+def test_split_T_after_fp64_bisection(gpu_ctx, scheme):
+    """The benchmark's fast path: P and the bisection stay FP64 (so kappa is
+    the FP64 path's), only T = Q W is split.  Every pixel's leakage,
+    recomputed in FP64 from the split T, stays within 1e-10 of the FP64
+    path's U (T's error enters U only at second order plus a kappa-weighted
+    first-order term), so every pixel keeps U <= U_max + US_tol."""
+    print(f"[case] split T after FP64 bisection scheme={scheme}", flush=True)
+    t = synth.small()
+    r = O.solve(t)
+    gpu_ctx.set_tile(t)
+    gpu_ctx.set_QL(r["Q"], r["L"])
+    gpu_ctx.lookup_B(t.npoly, t.npad)
+    gpu_ctx.build_P()
+    K, U, ev = gpu_ctx.bisect_kappa()
+    T64 = gpu_ctx.build_T()
+    T = gpu_ctx.build_T(precision=scheme)
+    A, B = r["A"], r["B"]
+    Utrue = (t.C_a + np.einsum("ia,ia->a", T, A @ T) + np.einsum("ia,ia->a", B, T)) / t.C_a
+    relT = float(np.max(np.abs(T - T64)) / np.max(np.abs(T64)))
+    dU = float(np.max(np.abs(Utrue - U)))
+    print(f"[split-T] scheme={scheme} relT={relT:.3e} max|U_true - U|={dU:.3e} "
+          f"max U_true={Utrue.max():.6e}", flush=True)
+    assert relT <= 1e-5
+    assert dU <= 1e-10
+    assert np.all(Utrue <= t.US_max + t.US_tol)
