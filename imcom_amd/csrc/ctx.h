@@ -68,7 +68,7 @@ struct EigWork {
     IMCOM_HIP_TRY(spart.reserve(kNormParts * d));
     IMCOM_HIP_TRY(spart2.reserve(kQParts * d));
     IMCOM_HIP_TRY(ss.reserve(2 * nb * d));
-    IMCOM_HIP_TRY(sscal.reserve(d));
+    IMCOM_HIP_TRY(sscal.reserve(8 * d));
     IMCOM_HIP_TRY(sVb.reserve((size_t)(n > 1 ? n - 1 : 1) * nbb * d));
     IMCOM_HIP_TRY(sSb.reserve((size_t)nbb * nbb * d));
     IMCOM_HIP_TRY(sTb.reserve((size_t)nbb * nbb * d));

@@ -56,18 +56,29 @@ __global__ __launch_bounds__(256) void dgemm_kernel(GemmDesc one, const GemmDesc
   }
   const int M = g.M, N = g.N, K = g.K;
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
-  const int nt = tiles_m * tiles_n;
-  if (b >= nt) return;
-  // blocks are handed round-robin to the 8 XCDs: give each XCD a contiguous
-  // run of tiles so neighbouring tiles share A rows / B columns in its L2
-  if (!GROUPED && (nt & 7) == 0) b = (b & 7) * (nt >> 3) + (b >> 3);
-  // column-major walk in groups of 8 tile rows
-  const int group = 8 * tiles_n;
-  const int gi0 = b / group;
-  const int first_m = gi0 * 8;
-  const int gm = min(tiles_m - first_m, 8);
-  const int tm = first_m + (b % group) % gm;
-  const int tn = (b % group) / gm;
+  int tm, tn;
+  if (g.lower) {
+    // row-major walk of the tiles with tm >= tn
+    if (b >= tiles_m * (tiles_m + 1) / 2) return;
+    int r = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= b) ++r;
+    while (r * (r + 1) / 2 > b) --r;
+    tm = r;
+    tn = b - r * (r + 1) / 2;
+  } else {
+    const int nt = tiles_m * tiles_n;
+    if (b >= nt) return;
+    // blocks are handed round-robin to the 8 XCDs: give each XCD a contiguous
+    // run of tiles so neighbouring tiles share A rows / B columns in its L2
+    if (!GROUPED && (nt & 7) == 0) b = (b & 7) * (nt >> 3) + (b >> 3);
+    // column-major walk in groups of 8 tile rows
+    const int group = 8 * tiles_n;
+    const int gi0 = b / group;
+    const int first_m = gi0 * 8;
+    const int gm = min(tiles_m - first_m, 8);
+    tm = first_m + (b % group) % gm;
+    tn = (b % group) / gm;
+  }
   const int m0 = tm * BM, n0 = tn * BN;
   const double* __restrict__ A = g.A;
   const double* __restrict__ B = g.B;
@@ -206,7 +217,30 @@ hipError_t dgemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, double a
   return launch_sz<64, 64, false>(st, ta, tb, grid, g, nullptr, nullptr, 0);
 }
 
+hipError_t dgemm_lower(hipStream_t st, bool ta, bool tb, int M, int K, double alpha,
+                       const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                       double* C, int64_t ldc) {
+  if (M <= 0) return hipSuccess;
+  GemmDesc g{M, M, K, alpha, beta, A, lda, B, ldb, C, ldc, 1};
+  const int64_t T128 = (M + 127) / 128;
+  if (T128 * (T128 + 1) / 2 >= 512) {
+    dim3 grid((unsigned)(T128 * (T128 + 1) / 2));
+    return launch_sz<128, 128, false>(st, ta, tb, grid, g, nullptr, nullptr, 0);
+  }
+  const int64_t T64 = (M + 63) / 64;
+  dim3 grid((unsigned)(T64 * (T64 + 1) / 2));
+  return launch_sz<64, 64, false>(st, ta, tb, grid, g, nullptr, nullptr, 0);
+}
+
 int gemm_tiles64(int M, int N) { return ((M + 63) / 64) * ((N + 63) / 64); }
+
+int gemm_tiles64(const GemmDesc& g) {
+  if (g.lower) {
+    const int T = (g.M + 63) / 64;
+    return T * (T + 1) / 2;
+  }
+  return gemm_tiles64(g.M, g.N);
+}
 
 hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
                          const int* d_prefix, int ngroups, int total_blocks) {
@@ -226,7 +260,7 @@ int GroupedPlan::add(bool ta, bool tb, const std::vector<GemmDesc>& groups) {
   for (const GemmDesc& g : groups) {
     if (g.M <= 0 || g.N <= 0) continue;
     descs.push_back(g);
-    L.blocks += gemm_tiles64(g.M, g.N);
+    L.blocks += gemm_tiles64(g);
     prefix.push_back(L.blocks);
     L.ngroups += 1;
   }

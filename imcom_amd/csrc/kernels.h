@@ -52,12 +52,19 @@ struct GemmDesc {
   int64_t ldb;
   double* C;
   int64_t ldc;
+  int lower = 0;  // 1: only the tiles of C on or below its diagonal (M == N)
 };
 
 hipError_t dgemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, double alpha,
                  const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                  double* C, int64_t ldc);
+// C's tiles on or below the diagonal only (M == N): the symmetric trailing
+// update, whose upper triangle is never read.
+hipError_t dgemm_lower(hipStream_t st, bool ta, bool tb, int M, int K, double alpha,
+                       const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
+                       double* C, int64_t ldc);
 int gemm_tiles64(int M, int N);
+int gemm_tiles64(const GemmDesc& g);
 hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
                          const int* d_prefix, int ngroups, int total_blocks);
 
@@ -127,10 +134,10 @@ struct SyTile {
   double* d;      // n     tridiagonal diagonal
   double* e;      // n-1   off-diagonal
   double* tau;    // n-1
-  double* part;   // >= ceil(n/256) partial sums
-  double* part2;  // >= ceil(n/256)
+  double* part;   // kNormParts   per-workgroup partials of a column's squared norm
+  double* part2;  // kQParts
   double* s;      // 2 NB dot products
-  double* scal;   // 1
+  double* scal;   // 5: reflector scale 1/(alpha-beta), tau, alpha, beta, W(c+1, jj)
   double* Vb;     // (n-1) x NBB
   double* Sb;     // NBB x NBB
   double* Tb;     // NBB x NBB
@@ -142,7 +149,7 @@ struct SyTile {
 // Partial-sum slots of the reduction kernels, sized for the largest tile
 // (n <= 32768) so that tiles of a batch can share launch grids.
 constexpr int kQParts = 40000;
-constexpr int kNormParts = 160;
+constexpr int kNormParts = 2048;
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nb, Prof* prof);

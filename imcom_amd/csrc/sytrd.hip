@@ -9,18 +9,24 @@
 // the panel's earlier reflectors (rank-2jj update), its reflector is formed,
 // and w_c = tau (A22 v - V (W^T v) - W (V^T v)) - 1/2 tau^2 (w^T v) v is
 // accumulated; after the panel the trailing matrix takes one rank-2NB update
-// A22 -= V W^T + W V^T on the FP64 matrix cores.  The full symmetric trailing
-// matrix is kept in memory; the matrix-vector product reads only its lower
-// triangle, in 64x64 blocks, each block serving its rows and its columns.
+// A22 -= V W^T + W V^T on the FP64 matrix cores, on the tiles on or below
+// the diagonal only: nothing reads A above its diagonal.  The matrix-vector
+// product reads the lower triangle in 64x64 blocks, each block serving its
+// rows and its columns.
 //
-// Two launches per column:
-//   sy_symv      reflector from the column's norm partials, partial sums of
-//                y = A22 v and of v^T A22 v over the lower triangle, the 2jj
-//                dot products V^T v, W^T v
+// Three launches per column:
+//   sy_symv      partial sums of u = A22 x for the unscaled column x (so it
+//                does not wait for the reflector) and of x^T u over the lower
+//                triangle, the 2jj dot products [V W]^T x
+//   sy_mid       one workgroup: the reflector from the column's norm
+//                partials, v^T A22 v, V^T v, W^T v, alpha and W(c+1, jj)
 //   sy_wupd      w = tau (y - V sW - W sV) + alpha v with
 //                alpha = -1/2 tau^2 (v^T A22 v - 2 sV.sW) (the same value as
-//                -1/2 tau w_pre^T v, without a second grid-wide reduction);
-//                rank-2(jj+1) update of the next column and its norm partials
+//                -1/2 tau w_pre^T v); rank-2(jj+1) update of the next column
+//                and its norm partials
+// Grid-wide sums are left as per-workgroup partials and finished by one
+// workgroup (sy_mid) rather than with device-scope fences, which on a
+// multi-XCD GPU write back the L2 and cost more than the sums themselves.
 // All sums are fixed-order (partials per workgroup, reduced in workgroup
 // order), so runs are bitwise reproducible.
 //
@@ -44,230 +50,276 @@ __device__ __forceinline__ double dlapy2(double x, double y) {
   return w * sqrt(1.0 + q * q);
 }
 
-// Column c update for rows [c, n) with the panel's first `jj` reflectors,
-// plus per-workgroup partials of the squared norm of rows [c+2, n).
-// V aliases A's panel columns (V(i,p) = A(i, k+p)); W is n x NB.
-__global__ __launch_bounds__(RB) void sy_colupd(const SyTile* __restrict__ tiles, int k, int c,
-                                                int jj) {
+// Fixed-order sum of part[0, np) over the whole workgroup (every thread gets it).
+__device__ __forceinline__ double sum_parts(const double* __restrict__ part, int np, double* red) {
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < np; q += blockDim.x) acc += part[q];
+  return block_sum(acc, red);
+}
+
+// First column c of a panel (already up to date): per-workgroup partials of
+// its squared norm over rows [c+2, n), read by sy_mid.
+__global__ __launch_bounds__(RB) void sy_colnorm(const SyTile* __restrict__ tiles, int c) {
   __shared__ double red[RB / 64];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
-  if (c >= n - 1) return;
-  const int64_t i = c + (int64_t)blockIdx.x * RB + threadIdx.x;
-  double a = 0.0;
-  if (i < n) {
-    a = t.A[i + (int64_t)c * n];
-    if (jj > 0) {
-      double s = 0.0;
-      for (int p = 0; p < jj; ++p) {
-        const double* Vp = t.A + (int64_t)(k + p) * n;
-        const double* Wp = t.W + (int64_t)p * n;
-        s += Vp[i] * Wp[c] + Wp[i] * Vp[c];
-      }
-      a -= s;
-      t.A[i + (int64_t)c * n] = a;
-    }
-  }
-  const double sq = (i >= c + 2 && i < n) ? a * a : 0.0;
-  const double tot = block_sum(sq, red);
+  if (c >= n - 1) return;  // this tile is reduced (uniform over its workgroups)
+  const int64_t i = c + 2 + (int64_t)blockIdx.x * RB + threadIdx.x;
+  const double a = (i < n) ? t.A[i + (int64_t)c * n] : 0.0;
+  const double tot = block_sum(a * a, red);
   if (threadIdx.x == 0) t.part[blockIdx.x] = tot;
-}
-
-// Householder reflector of column c from its norm partials (LAPACK dlarfg
-// conventions): beta = -sign(alpha) |x|, tau = (beta - alpha)/beta, and
-// v = [1, x(2:)/(alpha - beta)].
-__device__ __forceinline__ void sy_reflector(const SyTile& t, int64_t c, int nparts, double& tau,
-                                             double& beta, double& scal) {
-  double xn2 = 0.0;
-  for (int q = 0; q < nparts; ++q) xn2 += t.part[q];
-  const double alpha = t.A[(c + 1) + c * t.n];
-  const double xnorm = sqrt(xn2);
-  if (xnorm == 0.0) {
-    tau = 0.0;
-    beta = alpha;
-    scal = 0.0;
-  } else {
-    beta = -copysign(dlapy2(alpha, xnorm), alpha);
-    tau = (beta - alpha) / beta;
-    scal = 1.0 / (alpha - beta);
-  }
 }
 
 constexpr int SB = 64;   // symmetric matrix-vector block
 constexpr int SEG = 8;   // blocks of one block-column per workgroup
 
-// y = A22 v reading only the lower triangle of A22 (order s = n-c-1, the
-// trailing matrix below/right of column c).  Workgroup (seg, J) walks blocks
-// I in [J + seg*SEG, J + (seg+1)*SEG) of block-column J: block (I,J) gives the
-// rows of block I the product A_IJ v_J (written to ypart[J][rows]) and, for
-// I > J, gives the columns of block J the product A_IJ^T v_I (accumulated over
-// the segment and written to ycpart[seg][cols]).  sy_wupd sums the partials
-// in a fixed order.  Workgroups in row y = nbr compute the 2jj dot products
-// V^T v and W^T v, one wavefront each.
+// u = A22 x for the unscaled column x = A(c+1:n, c), reading only the lower
+// triangle (diagonal included) of the stored trailing matrix A22 (order
+// s = n-c-1).  The
+// reflector's v is scal * (x - beta e_0), so A22 v = scal (u - beta A22 e_0)
+// and v^T A22 v = scal^2 (x^T u - 2 beta u_0 + beta^2 A22(0,0)): this kernel
+// needs nothing from the reflector and runs before it is formed.
+// Workgroup (seg, J) walks blocks I in [J + seg*SEG, J + (seg+1)*SEG) of
+// block-column J: block (I,J) gives the rows of block I the product
+// A_IJ x_J (written to ypart[J][rows]) and, for I > J, gives the columns of
+// block J the product A_IJ^T x_I (accumulated over the segment and written
+// to ycpart[seg][cols]); its share of x^T u goes to qpart.  (Issuing the
+// next block's loads before the current block's reduction measured slower:
+// the extra registers cost more occupancy than the overlap gains.)
+// Workgroups in row y = nbr form the 2jj unscaled dot products
+// sigma = [V W]^T x over rows [c+1, n), one wavefront each.
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
-                                               int jj, int nparts) {
+                                               int jj) {
   __shared__ double vJ[SB];
-  __shared__ double red[4][SB + 1];
+  __shared__ double dg[SB];  // diagonal of block (J, J)
+  __shared__ double red[2][4][SB + 1];
+  __shared__ double qred[4];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
-  if (c >= n - 1) return;
+  if (c >= n - 1) return;  // this tile is reduced (uniform over its workgroups)
   const double* __restrict__ colc = t.A + (int64_t)c * n;
-  double tau, beta, scal;
-  sy_reflector(t, c, nparts, tau, beta, scal);
   const int64_t s = n - c - 1;
   const int nbr = (int)((s + SB - 1) / SB);
   const int J = blockIdx.y;
   const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
-  const int64_t qslot = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const int I0 = J + blockIdx.x * SEG;
+  double qt = 0.0;  // this workgroup's share of x^T u
   if (J >= nbr) {
-    if (tid == 0) t.qpart[qslot] = 0.0;
     const int q = blockIdx.x * 4 + ty;
     if (J == nbr && q < 2 * jj) {
       const double* X = q < jj ? t.A + (int64_t)(k + q) * n : t.W + (int64_t)(q - jj) * n;
       double acc = 0.0;
-      for (int64_t i = c + 2 + tx; i < n; i += 64) acc += X[i] * colc[i];
+      for (int64_t i = c + 1 + tx; i < n; i += 64) acc += X[i] * colc[i];
       acc = wave_sum(acc);
-      if (tx == 0) t.s[q] = X[c + 1] + scal * acc;  // s[0:jj) = V^T v, s[jj:2jj) = W^T v
+      if (tx == 0) t.s[q] = acc;
     }
-    return;
-  }
-  const int I0 = J + blockIdx.x * SEG;
-  if (I0 >= nbr) {
-    if (tid == 0) t.qpart[qslot] = 0.0;
-    return;
-  }
-  const int I1 = min(I0 + SEG, nbr);
-  // v in A22-local indexing: v(0) = 1, v(r) = scal * A(c+1+r, c)
-  if (tid < SB) {
-    const int64_t r = (int64_t)J * SB + tid;
-    vJ[tid] = (r < s) ? (r == 0 ? 1.0 : scal * colc[c + 1 + r]) : 0.0;
-  }
-  __syncthreads();
-  const double* __restrict__ A22 = t.A + (c + 1) + (int64_t)(c + 1) * n;
-  double colacc[16];
+  } else if (I0 < nbr) {
+    const int I1 = min(I0 + SEG, nbr);
+    if (tid < SB) {
+      const int64_t r = (int64_t)J * SB + tid;
+      vJ[tid] = (r < s) ? colc[c + 1 + r] : 0.0;
+    }
+    const double* __restrict__ A22 = t.A + (c + 1) + (int64_t)(c + 1) * n;
+    double colacc[16];
 #pragma unroll
-  for (int q = 0; q < 16; ++q) colacc[q] = 0.0;
-  double qacc = 0.0;  // this workgroup's share of v^T A22 v
-  for (int I = I0; I < I1; ++I) {
-    const int64_t r = (int64_t)I * SB + tx;
-    const bool rv = r < s;
+    for (int q = 0; q < 16; ++q) colacc[q] = 0.0;
+    double qacc = 0.0;
     double a[16];
+    double xI;
+    // diagonal blocks are read on and below the diagonal only (the trailing
+    // update leaves the upper triangle stale): their strictly lower part
+    // serves rows and columns, the diagonal serves rows
+    auto load = [&](int I, double* dst, double& xdst) {
+      const int64_t r = (int64_t)I * SB + tx;
+      const bool rv = r < s;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int64_t col = (int64_t)J * SB + ty + 4 * q;
+        dst[q] = (rv && col < s && (I != J || col <= r)) ? A22[r + col * n] : 0.0;
+      }
+      xdst = rv ? colc[c + 1 + r] : 0.0;
+    };
+    load(I0, a, xI);
+    __syncthreads();  // vJ
+    int buf = 0;
+    for (int I = I0; I < I1; ++I) {
+      double rowp = 0.0;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) rowp += a[q] * vJ[ty + 4 * q];
+      if (I > J) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) colacc[q] += a[q] * xI;
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          if (ty + 4 * q != tx) colacc[q] += a[q] * xI;
+          else dg[tx] = a[q];
+        }
+      }
+      red[buf][ty][tx] = rowp;
+      __syncthreads();
+      const int64_t r = (int64_t)I * SB + tx;
+      if (ty == 0 && r < s) {
+        const double rs = red[buf][0][tx] + red[buf][1][tx] + red[buf][2][tx] + red[buf][3][tx];
+        t.ypart[(int64_t)J * n + r] = rs;
+        // x_I^T A_IJ x_J, counted twice for the mirrored block; a diagonal
+        // block's row sums hold (L + D) x, and x^T (L + D + L^T) x =
+        // 2 x^T (L + D) x - x^T D x
+        qacc += 2.0 * xI * rs - (I == J ? xI * xI * dg[tx] : 0.0);
+      }
+      buf ^= 1;
+      if (I + 1 < I1) load(I + 1, a, xI);
+    }
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
+      const double cs = wave_sum(colacc[q]);
       const int64_t col = (int64_t)J * SB + ty + 4 * q;
-      a[q] = (rv && col < s) ? A22[r + col * n] : 0.0;
+      if (tx == 0 && col < s) t.ycpart[(int64_t)blockIdx.x * n + col] = cs;
     }
-    double rowp = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) rowp += a[q] * vJ[ty + 4 * q];
-    red[ty][tx] = rowp;
-    const double vI = rv ? (r == 0 ? 1.0 : scal * colc[c + 1 + r]) : 0.0;
-    if (I > J) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) colacc[q] += a[q] * vI;
-    }
-    __syncthreads();
-    if (ty == 0 && rv) {
-      const double rs = red[0][tx] + red[1][tx] + red[2][tx] + red[3][tx];
-      t.ypart[(int64_t)J * n + r] = rs;
-      // v_I^T A_IJ v_J, counted twice off the diagonal (the mirrored block)
-      qacc += (I == J ? 1.0 : 2.0) * vI * rs;
-    }
-    __syncthreads();
+    qt = block_sum(qacc, qred);
   }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const double cs = wave_sum(colacc[q]);
-    const int64_t col = (int64_t)J * SB + ty + 4 * q;
-    if (tx == 0 && col < s) t.ycpart[(int64_t)blockIdx.x * n + col] = cs;
-  }
-  {
-    __shared__ double qred[4];
-    const double qt = block_sum(qacc, qred);
-    if (tid == 0) t.qpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = qt;
-  }
-  if (blockIdx.x == 0 && J == 0 && tid == 0) {
-    t.d[c] = colc[c];
-    t.e[c] = beta;
-    t.tau[c] = tau;
-    t.scal[0] = scal;
-  }
+  if (tid == 0) t.qpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = qt;
 }
 
-// w before its v-correction at row i of the trailing matrix:
-//   tau (y_i - sum_p V(i,p) sW_p - W(i,p) sV_p),
-// y_i summed from sy_symv's partials in a fixed order.
-__device__ __forceinline__ double sy_wrow(const SyTile& t, int k, int64_t c, int jj, double tau,
-                                          int64_t i) {
-  const int64_t n = t.n;
-  const int64_t s = n - c - 1;
-  const int nbr = (int)((s + SB - 1) / SB);
-  const int64_t r = i - (c + 1);
-  const int R = (int)(r / SB);
-  double acc = 0.0;
-  for (int J = 0; J <= R; ++J) acc += t.ypart[(int64_t)J * n + r];
-  const int nseg = (nbr - R + SEG - 1) / SEG;
-  for (int sg = 0; sg < nseg; ++sg) acc += t.ycpart[(int64_t)sg * n + r];
-  for (int p = 0; p < jj; ++p) {
-    const double Vip = t.A[i + (int64_t)(k + p) * n];
-    const double Wip = t.W[i + (int64_t)p * n];
-    acc -= Vip * t.s[jj + p] + Wip * t.s[p];
-  }
-  return tau * acc;
-}
-
-// Finish column c and start column c+1 in one pass over rows [c+1, n):
-//   w = tau (y - V sW - W sV) + alpha v,
-//   alpha = -1/2 tau (w_pre^T v) = -1/2 tau^2 (v^T A22 v - 2 sV.sW)
-// (v^T A22 v from sy_symv's per-workgroup partials), stored in W(:, jj); v into
-// A(:, c); then, when column c+1 is in this panel or is the tile's last column,
-// its rank-2(jj+1) update and its norm partials.
-__global__ __launch_bounds__(RB) void sy_wupd(const SyTile* __restrict__ tiles, int k, int c,
-                                              int jj, int nqparts, int in_panel_next) {
-  __shared__ double red[RB / 64];
+// One workgroup per tile, between sy_symv and sy_wupd of column c: the
+// reflector (LAPACK dlarfg conventions: beta = -sign(alpha) |(alpha, x')|,
+// tau = (beta - alpha)/beta, v = [1, x'/(alpha - beta)]) from the norm
+// partials of x' = x(1:), stored as d[c], e[c], tau[c]; then
+//   v^T A22 v = scal^2 (x^T u - 2 beta u_0 + beta^2 A22(0,0)),
+//   V^T v = scal (sigma_V - beta V(c+1,:)), W^T v likewise (into s),
+//   alpha = -1/2 tau^2 (v^T A22 v - 2 (V^T v).(W^T v)),
+//   W(c+1, jj) = tau (y_0 - V(c+1,:) W^T v - W(c+1,:) V^T v) + alpha,
+// y_0 = scal (u_0 - beta A22(0,0)).  scal[0..4] = scal, tau, alpha, beta,
+// W(c+1, jj).  nnorm / nq: partial counts of the norm and of x^T u.
+__global__ __launch_bounds__(256) void sy_mid(const SyTile* __restrict__ tiles, int k, int c,
+                                              int jj, int nnorm, int nq) {
+  __shared__ double red[4];
+  __shared__ double ss[2 * 64];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
   if (c >= n - 1) return;
-  const int upd_next = in_panel_next || (c + 1 == n - 1);
-  const double tau = t.tau[c];
-  const double scal = t.scal[0];
-  double qs = 0.0;
-  for (int q = threadIdx.x; q < nqparts; q += RB) qs += t.qpart[q];
-  const double vAv = block_sum(qs, red);
-  double svw = 0.0;
-  for (int p = 0; p < jj; ++p) svw += t.s[p] * t.s[jj + p];
-  const double alpha = -0.5 * tau * tau * (vAv - 2.0 * svw);
+  const int tid = threadIdx.x;
   const int64_t c1 = c + 1;
-  const int64_t i = c1 + (int64_t)blockIdx.x * RB + threadIdx.x;
-  double* W = t.W;
-  double a = 0.0;
-  if (i < n) {
-    const double* colc = t.A + (int64_t)c * n;
-    const double v = (i == c1) ? 1.0 : colc[i] * scal;
-    const double w = sy_wrow(t, k, c, jj, tau, i) + alpha * v;
-    if (upd_next) {
-      // W(c+1, jj): the same row formula at row c+1, where v = 1
-      const double Wc1 = sy_wrow(t, k, c, jj, tau, c1) + alpha;
-      a = t.A[i + c1 * n];
-      double sacc = 0.0;
-      for (int p = 0; p < jj; ++p) {
-        const double* Vp = t.A + (int64_t)(k + p) * n;
-        const double* Wp = W + (int64_t)p * n;
-        sacc += Vp[i] * Wp[c1] + Wp[i] * Vp[c1];
-      }
-      sacc += v * Wc1 + w * 1.0;  // p = jj: V(c+1, jj) = 1
-      a -= sacc;
-    }
-    // each thread reads and writes only its own row of columns c, c+1 and W(:, jj)
-    W[i + (int64_t)jj * n] = w;
-    t.A[i + (int64_t)c * n] = v;
-    if (upd_next) t.A[i + c1 * n] = a;
+  const double xn2 = sum_parts(t.part, nnorm, red);
+  const double xu = sum_parts(t.qpart, nq, red);
+  const int64_t s = n - c1;
+  const int nbr = (int)((s + SB - 1) / SB);
+  const int nseg0 = (nbr + SEG - 1) / SEG;
+  double u0p = 0.0;
+  for (int sg = tid; sg < nseg0; sg += blockDim.x) u0p += t.ycpart[(int64_t)sg * n];
+  const double u0 = t.ypart[0] + block_sum(u0p, red);
+  const double alpha_r = t.A[c1 + c * n];
+  const double xnorm = sqrt(xn2);
+  double tau, beta, scal;
+  if (xnorm == 0.0) {
+    tau = 0.0;
+    beta = alpha_r;
+    scal = 0.0;
+  } else {
+    beta = -copysign(dlapy2(alpha_r, xnorm), alpha_r);
+    tau = (beta - alpha_r) / beta;
+    scal = 1.0 / (alpha_r - beta);
   }
-  if (upd_next) {
-    const double sq = (i >= c1 + 2 && i < n) ? a * a : 0.0;
-    const double tot = block_sum(sq, red);
-    if (threadIdx.x == 0) t.part[blockIdx.x] = tot;
-    if (c1 == n - 1 && blockIdx.x == 0 && threadIdx.x == 0) t.d[c1] = a;
+  if (tid < 2 * jj) {
+    const double* X = tid < jj ? t.A + (int64_t)(k + tid) * n : t.W + (int64_t)(tid - jj) * n;
+    ss[tid] = scal * (t.s[tid] - beta * X[c1]);
+  }
+  __syncthreads();
+  if (tid < 2 * jj) t.s[tid] = ss[tid];
+  if (tid == 0) {
+    const double g0 = t.A[c1 + c1 * n];
+    const double vAv = scal * scal * (xu - 2.0 * beta * u0 + beta * beta * g0);
+    double svw = 0.0, corr = 0.0;
+    for (int p = 0; p < jj; ++p) {
+      svw += ss[p] * ss[jj + p];
+      corr += t.A[c1 + (int64_t)(k + p) * n] * ss[jj + p] + t.W[c1 + (int64_t)p * n] * ss[p];
+    }
+    const double alpha = -0.5 * tau * tau * (vAv - 2.0 * svw);
+    const double y0 = scal * (u0 - beta * g0);
+    t.scal[0] = scal;
+    t.scal[1] = tau;
+    t.scal[2] = alpha;
+    t.scal[3] = beta;
+    t.scal[4] = tau * (y0 - corr) + alpha;
+    t.d[c] = t.A[c + c * n];
+    t.e[c] = beta;
+    t.tau[c] = tau;
+  }
+}
+
+constexpr int WR = 16;  // rows per sy_wupd workgroup
+constexpr int WL = 16;  // lanes per row (256 threads)
+
+// Finish column c and start column c+1 over rows [c+1, n), 16 rows per
+// workgroup and 16 lanes per row (lane l sums partials l, l+16, ...):
+//   y = scal (u - beta A22(:,0)),  w = tau (y - V sW - W sV) + alpha v,
+// stored in W(:, jj); v into A(:, c); then, when column c+1 is in this panel
+// or is the tile's last column, its rank-2(jj+1) update and its norm
+// partials (for the next sy_mid).
+__global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles, int k, int c,
+                                               int jj, int in_panel_next) {
+  __shared__ double red[3][4][WR];
+  __shared__ double nred[4];
+  const SyTile t = tiles[blockIdx.z];
+  const int64_t n = t.n;
+  if (c >= n - 1) return;  // this tile is reduced (uniform over its workgroups)
+  const bool upd_next = in_panel_next || (c + 1 == n - 1);
+  const int tid = threadIdx.x, rl = tid & (WR - 1), lane = tid >> 4, w = tid >> 6;
+  const int64_t c1 = c + 1;
+  const int64_t s = n - c1;
+  const int nbr = (int)((s + SB - 1) / SB);
+  const int64_t i = c1 + (int64_t)blockIdx.x * WR + rl;
+  double qu = 0.0, qc = 0.0, q1 = 0.0;  // lane partials: u, V sW + W sV, update
+  if (i < n) {
+    const int64_t r = i - c1;
+    const int R = (int)(r / SB);
+    for (int J = lane; J <= R; J += WL) qu += t.ypart[(int64_t)J * n + r];
+    const int nseg = (nbr - R + SEG - 1) / SEG;
+    for (int sg = lane; sg < nseg; sg += WL) qu += t.ycpart[(int64_t)sg * n + r];
+    for (int p = lane; p < jj; p += WL) {
+      const double Vip = t.A[i + (int64_t)(k + p) * n];
+      const double Wip = t.W[i + (int64_t)p * n];
+      qc += Vip * t.s[jj + p] + Wip * t.s[p];
+      if (upd_next) q1 += Vip * t.W[c1 + (int64_t)p * n] + Wip * t.A[c1 + (int64_t)(k + p) * n];
+    }
+  }
+  // lanes of one row: 4 per wavefront (thread bits 4, 5), then across the 4 wavefronts
+  qu += __shfl_xor(qu, 16, kWave);
+  qc += __shfl_xor(qc, 16, kWave);
+  q1 += __shfl_xor(q1, 16, kWave);
+  qu += __shfl_xor(qu, 32, kWave);
+  qc += __shfl_xor(qc, 32, kWave);
+  q1 += __shfl_xor(q1, 32, kWave);
+  if ((lane & 3) == 0) {
+    red[0][w][rl] = qu;
+    red[1][w][rl] = qc;
+    red[2][w][rl] = q1;
+  }
+  __syncthreads();
+  double sq = 0.0;
+  if (tid < WR && i < n) {
+    const double U = red[0][0][rl] + red[0][1][rl] + red[0][2][rl] + red[0][3][rl];
+    const double Cr = red[1][0][rl] + red[1][1][rl] + red[1][2][rl] + red[1][3][rl];
+    const double Q1 = red[2][0][rl] + red[2][1][rl] + red[2][2][rl] + red[2][3][rl];
+    const double scal = t.scal[0], tau = t.scal[1], alpha = t.scal[2], beta = t.scal[3];
+    const double g = t.A[i + c1 * n];  // A22(:, 0), stored
+    const double v = (i == c1) ? 1.0 : t.A[i + (int64_t)c * n] * scal;
+    const double y = scal * (U - beta * g);
+    const double wv = tau * (y - Cr) + alpha * v;
+    if (upd_next) {
+      // p = jj: V(c+1, jj) = 1, W(c+1, jj) = scal[4]
+      const double a = g - (Q1 + v * t.scal[4] + wv);
+      t.A[i + c1 * n] = a;
+      if (i >= c1 + 2) sq = a * a;
+      if (c1 == n - 1) t.d[c1] = a;
+    }
+    // each row's column-c, column-(c+1) and W(:, jj) entries belong to this thread
+    t.W[i + (int64_t)jj * n] = wv;
+    t.A[i + (int64_t)c * n] = v;
+  }
+  if (upd_next && c1 < n - 1) {
+    const double tot = block_sum(sq, nred);
+    if (tid == 0) t.part[blockIdx.x] = tot;
   }
 }
 
@@ -311,58 +363,85 @@ __global__ void bt_gather(const SyTile* __restrict__ tiles, int kb, int nbb) {
   t.Vb[r + (int64_t)p * rows] = v;
 }
 
+constexpr int kMaxNBB = 128;  // largest back-transform block bt_build_T holds in LDS
+
+// T(j,j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) S(0:j, j), column by column.
+// T lives in LDS (column-major, ld kMaxNBB, 128 KB); thread i owns row i, so
+// a column's rows depend only on the same thread's earlier entries and the
+// shared S column, which is loaded one column ahead.
 __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ tiles, int kb,
                                                   int nbb) {
+  __shared__ double Ts[kMaxNBB * kMaxNBB];
+  __shared__ double sS[2][kMaxNBB];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
   if (kb >= n - 1) return;
-  double* T = t.Tb;  // nbb x nbb, ld nbb
-  const double* S = t.Sb;
+  const double* __restrict__ S = t.Sb;
   const int i = threadIdx.x;
+  double sreg = 0.0;  // S(i, j) for the coming column j
   for (int j = 0; j < nbb; ++j) {
+    if (i < j) sS[j & 1][i] = sreg;
+    const double snext = (j + 1 < nbb && i < j + 1) ? S[i + (int64_t)(j + 1) * nbb] : 0.0;
+    __syncthreads();
     const int64_t c = kb + j;
     const double tj = (c < n - 1) ? t.tau[c] : 0.0;
-    double acc = 0.0;
-    if (i < j) {
-      for (int l = i; l < j; ++l) acc += T[i + l * nbb] * S[l + j * nbb];
-    }
-    __syncthreads();
     if (i < nbb) {
-      if (i < j)
-        T[i + j * nbb] = -tj * acc;
-      else if (i == j)
-        T[i + j * nbb] = tj;
-      else
-        T[i + j * nbb] = 0.0;
+      double v;
+      if (i < j) {
+        const double* sj = sS[j & 1];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int l = i;
+        for (; l + 3 < j; l += 4) {
+          a0 += Ts[l * kMaxNBB + i] * sj[l];
+          a1 += Ts[(l + 1) * kMaxNBB + i] * sj[l + 1];
+          a2 += Ts[(l + 2) * kMaxNBB + i] * sj[l + 2];
+          a3 += Ts[(l + 3) * kMaxNBB + i] * sj[l + 3];
+        }
+        for (; l < j; ++l) a0 += Ts[l * kMaxNBB + i] * sj[l];
+        v = -tj * ((a0 + a1) + (a2 + a3));
+      } else {
+        v = (i == j) ? tj : 0.0;
+      }
+      Ts[j * kMaxNBB + i] = v;
     }
-    __syncthreads();
+    sreg = snext;
   }
+  __syncthreads();
+  for (int idx = i; idx < nbb * nbb; idx += blockDim.x)
+    t.Tb[idx] = Ts[(idx / nbb) * kMaxNBB + idx % nbb];
+}
+
+// The pb columns of the panel starting at column k: the first column's norm
+// partials, then three launches per column.
+static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
+                                   int64_t k, int pb, Prof* prof) {
+  size_t tok = 0;
+  int nnorm = std::max(ceil_div(nmax - k - 2, RB), 1);
+  hipLaunchKernelGGL(sy_colnorm, dim3(nnorm, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k);
+  for (int jj = 0; jj < pb; ++jj) {
+    const int64_t c = k + jj;
+    const int nbr = ceil_div(nmax - c - 1, SB);
+    const int gx = std::max(std::max(ceil_div(nbr, SEG), ceil_div(2 * jj, 4)), 1);
+    if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYMV, 2, &tok));
+    hipLaunchKernelGGL(sy_symv, dim3(gx, nbr + 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
+                       (int)c, jj);
+    if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
+    hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k, (int)c, jj,
+                       nnorm, gx * (nbr + 1));
+    const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);
+    hipLaunchKernelGGL(sy_wupd, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k, (int)c,
+                       jj, (jj + 1 < pb) ? 1 : 0);
+    nnorm = nw;  // sy_wupd leaves the next column's norm partials, one per workgroup
+  }
+  return hipGetLastError();
 }
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nb, Prof* prof) {
-  const int rbl = ceil_div(nmax, RB);
   size_t tok = 0;
   for (int64_t k = 0; k < nmax - 1; k += nb) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
-    // first column of the panel: norm partials only (jj = 0)
-    hipLaunchKernelGGL(sy_colupd, dim3(ceil_div(nmax - k, RB), 1, ntiles), dim3(RB), 0, st,
-                       d_tiles, (int)k, (int)k, 0);
-    for (int jj = 0; jj < pb; ++jj) {
-      const int64_t c = k + jj;
-      const int nparts = ceil_div(nmax - c, RB);
-      const int nbr = ceil_div(nmax - c - 1, SB);
-      const int gx = std::max(ceil_div(nbr, SEG), ceil_div(2 * jj, 4));
-      if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYMV, 2, &tok));
-      hipLaunchKernelGGL(sy_symv, dim3(std::max(gx, 1), nbr + 1, ntiles), dim3(256), 0, st,
-                         d_tiles, (int)k, (int)c, jj, nparts);
-      if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
-      const int nb2 = std::max(ceil_div(nmax - c - 1, RB), 1);
-      const int nq = std::max(gx, 1) * (nbr + 1);  // sy_symv's workgroup count
-      hipLaunchKernelGGL(sy_wupd, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
-                         jj, nq, (jj + 1 < pb) ? 1 : 0);
-    }
-    (void)rbl;
+    IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, prof));
     // trailing update A22 -= [V W] [W V]^T over rows/cols [k+nb, n)
     const int64_t r0 = k + nb;
     if (r0 < nmax) {
@@ -373,8 +452,8 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
         const SyTile& h = host[b];
         if (r0 >= h.n) continue;
         const int64_t mm = h.n - r0;
-        IMCOM_HIP_TRY(dgemm(st, false, true, (int)mm, (int)mm, 2 * nb, -1.0, h.VW + r0, h.n,
-                            h.WV + r0, h.n, 1.0, h.A + r0 + r0 * h.n, h.n));
+        IMCOM_HIP_TRY(dgemm_lower(st, false, true, (int)mm, 2 * nb, -1.0, h.VW + r0, h.n,
+                                  h.WV + r0, h.n, 1.0, h.A + r0 + r0 * h.n, h.n));
       }
       if (prof) IMCOM_HIP_TRY(prof->end(PS_SYR2K, tok));
     }
@@ -441,7 +520,7 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
       if (r0 >= h.n) continue;
       const int mm = (int)(h.n - r0);
       g.push_back(GemmDesc{mm, mm, 2 * nb, -1.0, 1.0, h.VW + r0, h.n, h.WV + r0, h.n,
-                           h.A + r0 + r0 * h.n, h.n});
+                           h.A + r0 + r0 * h.n, h.n, 1});
     }
     plan.syr2k.push_back(plan.gp.add(false, true, g));
   }
@@ -476,22 +555,7 @@ hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, 
   int panel = 0;
   for (int64_t k = 0; k < nmax - 1; k += nb, ++panel) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
-    hipLaunchKernelGGL(sy_colupd, dim3(ceil_div(nmax - k, RB), 1, ntiles), dim3(RB), 0, st,
-                       d_tiles, (int)k, (int)k, 0);
-    for (int jj = 0; jj < pb; ++jj) {
-      const int64_t c = k + jj;
-      const int nparts = ceil_div(nmax - c, RB);
-      const int nbr = ceil_div(nmax - c - 1, SB);
-      const int gx = std::max(ceil_div(nbr, SEG), ceil_div(2 * jj, 4));
-      if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYMV, 2, &tok));
-      hipLaunchKernelGGL(sy_symv, dim3(std::max(gx, 1), nbr + 1, ntiles), dim3(256), 0, st,
-                         d_tiles, (int)k, (int)c, jj, nparts);
-      if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
-      const int nb2 = std::max(ceil_div(nmax - c - 1, RB), 1);
-      const int nq = std::max(gx, 1) * (nbr + 1);  // sy_symv's workgroup count
-      hipLaunchKernelGGL(sy_wupd, dim3(nb2, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k, (int)c,
-                         jj, nq, (jj + 1 < pb) ? 1 : 0);
-    }
+    IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, prof));
     const int64_t r0 = k + nb;
     if (r0 < nmax) {
       hipLaunchKernelGGL(sy_pack, dim3(ceil_div(nmax - r0, 256), nb, ntiles), dim3(256), 0, st,

@@ -197,6 +197,41 @@ def test_eigen_random_spd(gpu_ctx, n):
     assert np.max(np.abs(L - Lr)) <= 1e3 * n * EPS * nrm
 
 
+@pytest.mark.parametrize("nb,nbb", [(32, 128), (16, 64), (64, 128)])
+def test_eigen_accuracy_report(gpu_ctx, nb, nbb):
+    """Backward error, orthogonality and eigenvalue error of the GPU
+    eigensolver, in units of n*eps (printed), on a tile whose pixels sit near
+    the leakage floor (n=128, condition ~5e11), for each panel width and
+    back-transform block the tuner may pick.  LAPACK's DSYEVR reaches a few
+    units on this matrix."""
+    _say(f"eigen accuracy nb={nb} nbb={nbb}")
+    t = synth.tiny(seed=100, nexp=2, npix_in=8, nout=8, psf_sigma=1.1, gamma_sigma=1.8)
+    A = O.lookup_A(t)
+    n = t.n
+    gpu_ctx.set_tuning(nb, nbb)
+    try:
+        gpu_ctx.set_tile(t)
+        gpu_ctx.set_A(A)
+        d, e = gpu_ctx.tridiagonalize()
+        gpu_ctx.set_A(A)
+        Q, L, _ = gpu_ctx.eigen_A()
+    finally:
+        gpu_ctx.set_tuning(32, 128)
+    nrm = np.linalg.norm(A, 2)
+    unit = n * EPS
+    res = np.max(np.abs(A @ Q - Q * L)) / (unit * nrm)
+    orth = np.max(np.abs(Q.T @ Q - np.eye(n))) / unit
+    Qr, Lr, _ = O.eigen_A(A)
+    lam = np.max(np.abs(L - Lr)) / (unit * nrm)
+    tri = np.max(np.abs(np.linalg.eigvalsh(_tridiag(d, e)) - np.sort(Lr))) / (unit * nrm)
+    res_r = np.max(np.abs(A @ Qr - Qr * Lr)) / (unit * nrm)
+    orth_r = np.max(np.abs(Qr.T @ Qr - np.eye(n))) / unit
+    print(f"[eig-accuracy] nb={nb} nbb={nbb} n={n} residual={res:.2f} orth={orth:.2f} "
+          f"eigenvalues={lam:.2f} tridiagonal={tri:.2f} (n*eps units; LAPACK residual={res_r:.2f} "
+          f"orth={orth_r:.2f})", flush=True)
+    assert res <= 100 and orth <= 100 and lam <= 100 and tri <= 100
+
+
 def test_eigen_small(gpu_ctx, small_tile, small_ref):
     _say("eigen_A small n=1024")
     gpu_ctx.set_tile(small_tile)
