@@ -59,7 +59,7 @@ struct EigWork {
     IMCOM_HIP_TRY(sWV.reserve(n * 2 * nb * d));
     const int64_t nbr = (n + 63) / 64;
     IMCOM_HIP_TRY(sy.reserve((size_t)nbr * n * d));
-    IMCOM_HIP_TRY(sycp.reserve((size_t)((nbr + 7) / 8) * n * d));
+    IMCOM_HIP_TRY(sycp.reserve((size_t)((nbr + 3) / 4) * n * d));
     IMCOM_HIP_TRY(swpre.reserve(n * d));
     IMCOM_HIP_TRY(svtmp.reserve(n * d));
     IMCOM_HIP_TRY(sd.reserve(n * d));

@@ -126,9 +126,9 @@ struct SyTile {
   double* W;      // n x NB
   double* VW;     // n x 2NB  [V W]
   double* WV;     // n x 2NB  [W V]
-  double* ypart;  // ceil(n/64) x n   row partials of A22 v
-  double* ycpart; // ceil(n/512) x n  column partials of A22 v
-  double* qpart;  // kQParts            per-workgroup partials of v^T A22 v
+  double* ypart;  // ceil(n/64) x n   row partials of A22 x
+  double* ycpart; // ceil(n/256) x n  column partials of A22 x
+  double* qpart;  // kQParts            per-workgroup partials of x^T A22 x
   double* wpre;   // n
   double* vtmp;   // n
   double* d;      // n     tridiagonal diagonal
@@ -148,7 +148,7 @@ struct SyTile {
 
 // Partial-sum slots of the reduction kernels, sized for the largest tile
 // (n <= 32768) so that tiles of a batch can share launch grids.
-constexpr int kQParts = 40000;
+constexpr int kQParts = 70000;
 constexpr int kNormParts = 2048;
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,

@@ -70,29 +70,34 @@ __global__ __launch_bounds__(RB) void sy_colnorm(const SyTile* __restrict__ tile
   if (threadIdx.x == 0) t.part[blockIdx.x] = tot;
 }
 
-constexpr int SB = 64;   // symmetric matrix-vector block
-constexpr int SEG = 8;   // blocks of one block-column per workgroup
+constexpr int SB = 64;  // symmetric matrix-vector block
+constexpr int SEG = 4;  // blocks of one block-column per workgroup
 
 // u = A22 x for the unscaled column x = A(c+1:n, c), reading only the lower
 // triangle (diagonal included) of the stored trailing matrix A22 (order
-// s = n-c-1).  The
-// reflector's v is scal * (x - beta e_0), so A22 v = scal (u - beta A22 e_0)
-// and v^T A22 v = scal^2 (x^T u - 2 beta u_0 + beta^2 A22(0,0)): this kernel
+// s = n-c-1).  The reflector's v is scal * (x - beta e_0), so
+// A22 v = scal (u - beta A22 e_0) and
+// v^T A22 v = scal^2 (x^T u - 2 beta u_0 + beta^2 A22(0,0)): this kernel
 // needs nothing from the reflector and runs before it is formed.
-// Workgroup (seg, J) walks blocks I in [J + seg*SEG, J + (seg+1)*SEG) of
-// block-column J: block (I,J) gives the rows of block I the product
-// A_IJ x_J (written to ypart[J][rows]) and, for I > J, gives the columns of
-// block J the product A_IJ^T x_I (accumulated over the segment and written
-// to ycpart[seg][cols]); its share of x^T u goes to qpart.  (Issuing the
-// next block's loads before the current block's reduction measured slower:
-// the extra registers cost more occupancy than the overlap gains.)
+//
+// Workgroup (seg, J) covers block-column J (64 columns) over the SEG blocks
+// I in [J + seg*SEG, J + (seg+1)*SEG).  Wavefront w owns columns
+// 16w..16w+15 and reads each column's SEG*64 rows top to bottom in one pass
+// (a contiguous 2 KB run per column: column-major A streams in long runs).
+// Lane l holds rows l, l+64, ... and accumulates their row sums over the
+// wavefront's columns (reduced across the 4 wavefronts in LDS and written
+// to ypart[J][rows]); each column's sum against x over the segment's rows
+// goes to ycpart[seg][col]; the workgroup's share of x^T u goes to qpart.
+// Diagonal blocks are read on and below the diagonal only (the trailing
+// update leaves the upper triangle stale): their strictly lower part serves
+// rows and columns, the diagonal serves rows, and
+// x^T (L + D + L^T) x = 2 x^T (L + D) x - x^T D x.
 // Workgroups in row y = nbr form the 2jj unscaled dot products
 // sigma = [V W]^T x over rows [c+1, n), one wavefront each.
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
                                                int jj) {
-  __shared__ double vJ[SB];
-  __shared__ double dg[SB];  // diagonal of block (J, J)
-  __shared__ double red[2][4][SB + 1];
+  __shared__ double xJ[SB];
+  __shared__ double rred[4][SEG * SB];
   __shared__ double qred[4];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
@@ -101,80 +106,68 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
   const int64_t s = n - c - 1;
   const int nbr = (int)((s + SB - 1) / SB);
   const int J = blockIdx.y;
-  const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int I0 = J + blockIdx.x * SEG;
   double qt = 0.0;  // this workgroup's share of x^T u
   if (J >= nbr) {
-    const int q = blockIdx.x * 4 + ty;
+    const int q = blockIdx.x * 4 + w;
     if (J == nbr && q < 2 * jj) {
       const double* X = q < jj ? t.A + (int64_t)(k + q) * n : t.W + (int64_t)(q - jj) * n;
       double acc = 0.0;
-      for (int64_t i = c + 1 + tx; i < n; i += 64) acc += X[i] * colc[i];
+      for (int64_t i = c + 1 + lane; i < n; i += 64) acc += X[i] * colc[i];
       acc = wave_sum(acc);
-      if (tx == 0) t.s[q] = acc;
+      if (lane == 0) t.s[q] = acc;
     }
   } else if (I0 < nbr) {
-    const int I1 = min(I0 + SEG, nbr);
+    const int nb = min(SEG, nbr - I0);  // blocks in this segment
     if (tid < SB) {
       const int64_t r = (int64_t)J * SB + tid;
-      vJ[tid] = (r < s) ? colc[c + 1 + r] : 0.0;
+      xJ[tid] = (r < s) ? colc[c + 1 + r] : 0.0;
     }
     const double* __restrict__ A22 = t.A + (c + 1) + (int64_t)(c + 1) * n;
-    double colacc[16];
+    double xr[SEG], racc[SEG];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) colacc[q] = 0.0;
+    for (int u = 0; u < SEG; ++u) {
+      const int64_t r = (int64_t)(I0 + u) * SB + lane;
+      xr[u] = (u < nb && r < s) ? colc[c + 1 + r] : 0.0;
+      racc[u] = 0.0;
+    }
+    __syncthreads();  // xJ
     double qacc = 0.0;
-    double a[16];
-    double xI;
-    // diagonal blocks are read on and below the diagonal only (the trailing
-    // update leaves the upper triangle stale): their strictly lower part
-    // serves rows and columns, the diagonal serves rows
-    auto load = [&](int I, double* dst, double& xdst) {
-      const int64_t r = (int64_t)I * SB + tx;
-      const bool rv = r < s;
+    const bool diag = (I0 == J);
+    for (int q = 0; q < 16; ++q) {
+      const int cl = 16 * w + q;
+      const int64_t col = (int64_t)J * SB + cl;
+      double a[SEG];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int64_t col = (int64_t)J * SB + ty + 4 * q;
-        dst[q] = (rv && col < s && (I != J || col <= r)) ? A22[r + col * n] : 0.0;
+      for (int u = 0; u < SEG; ++u) {
+        const int64_t r = (int64_t)(I0 + u) * SB + lane;
+        const bool ok = (u < nb) && r < s && col < s && !(diag && u == 0 && lane < cl);
+        a[u] = ok ? A22[r + col * n] : 0.0;
       }
-      xdst = rv ? colc[c + 1 + r] : 0.0;
-    };
-    load(I0, a, xI);
-    __syncthreads();  // vJ
-    int buf = 0;
-    for (int I = I0; I < I1; ++I) {
-      double rowp = 0.0;
+      const double xc = xJ[cl];
+      double cs = 0.0;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) rowp += a[q] * vJ[ty + 4 * q];
-      if (I > J) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) colacc[q] += a[q] * xI;
-      } else {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-          if (ty + 4 * q != tx) colacc[q] += a[q] * xI;
-          else dg[tx] = a[q];
-        }
+      for (int u = 0; u < SEG; ++u) {
+        racc[u] += a[u] * xc;
+        const bool dg = diag && u == 0 && lane == cl;
+        cs += dg ? 0.0 : a[u] * xr[u];
+        if (dg) qacc -= xr[0] * xr[0] * a[0];
       }
-      red[buf][ty][tx] = rowp;
-      __syncthreads();
-      const int64_t r = (int64_t)I * SB + tx;
-      if (ty == 0 && r < s) {
-        const double rs = red[buf][0][tx] + red[buf][1][tx] + red[buf][2][tx] + red[buf][3][tx];
-        t.ypart[(int64_t)J * n + r] = rs;
-        // x_I^T A_IJ x_J, counted twice for the mirrored block; a diagonal
-        // block's row sums hold (L + D) x, and x^T (L + D + L^T) x =
-        // 2 x^T (L + D) x - x^T D x
-        qacc += 2.0 * xI * rs - (I == J ? xI * xI * dg[tx] : 0.0);
-      }
-      buf ^= 1;
-      if (I + 1 < I1) load(I + 1, a, xI);
+      cs = wave_sum(cs);
+      if (lane == 0 && col < s) t.ycpart[(int64_t)blockIdx.x * n + col] = cs;
     }
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const double cs = wave_sum(colacc[q]);
-      const int64_t col = (int64_t)J * SB + ty + 4 * q;
-      if (tx == 0 && col < s) t.ycpart[(int64_t)blockIdx.x * n + col] = cs;
+    for (int u = 0; u < SEG; ++u) rred[w][u * SB + lane] = racc[u];
+    __syncthreads();
+    for (int idx = tid; idx < nb * SB; idx += 256) {
+      const int u = idx / SB, l = idx % SB;
+      const int64_t r = (int64_t)(I0 + u) * SB + l;
+      if (r < s) {
+        const double rs = rred[0][idx] + rred[1][idx] + rred[2][idx] + rred[3][idx];
+        t.ypart[(int64_t)J * n + r] = rs;
+        qacc += 2.0 * colc[c + 1 + r] * rs;
+      }
     }
     qt = block_sum(qacc, qred);
   }
