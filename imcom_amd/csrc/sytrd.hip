@@ -50,11 +50,20 @@ __device__ __forceinline__ double dlapy2(double x, double y) {
   return w * sqrt(1.0 + q * q);
 }
 
-// Fixed-order sum of part[0, np) over the whole workgroup (every thread gets it).
+// Fixed-order sum of part[0, np) over the whole workgroup (every thread gets
+// it); four independent accumulators per thread keep four loads in flight.
 __device__ __forceinline__ double sum_parts(const double* __restrict__ part, int np, double* red) {
-  double acc = 0.0;
-  for (int q = threadIdx.x; q < np; q += blockDim.x) acc += part[q];
-  return block_sum(acc, red);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const int st = blockDim.x;
+  int q = threadIdx.x;
+  for (; q + 3 * st < np; q += 4 * st) {
+    a0 += part[q];
+    a1 += part[q + st];
+    a2 += part[q + 2 * st];
+    a3 += part[q + 3 * st];
+  }
+  for (; q < np; q += st) a0 += part[q];
+  return block_sum((a0 + a1) + (a2 + a3), red);
 }
 
 // First column c of a panel (already up to date): per-workgroup partials of
@@ -72,28 +81,35 @@ __global__ __launch_bounds__(RB) void sy_colnorm(const SyTile* __restrict__ tile
 
 constexpr int SB = 64;  // symmetric matrix-vector block
 constexpr int SEG = 4;  // blocks of one block-column per workgroup
+constexpr int DCH = 2048;  // rows per dot-product chunk (kDotChunks * DCH >= 32768)
 
 // u = A22 x for the unscaled column x = A(c+1:n, c), reading only the lower
-// triangle (diagonal included) of the stored trailing matrix A22 (order
-// s = n-c-1).  The reflector's v is scal * (x - beta e_0), so
+// triangle (diagonal included) of the stored trailing matrix A22 = A(c+1:n,
+// c+1:n).  The reflector's v is scal * (x - beta e_0), so
 // A22 v = scal (u - beta A22 e_0) and
 // v^T A22 v = scal^2 (x^T u - 2 beta u_0 + beta^2 A22(0,0)): this kernel
 // needs nothing from the reflector and runs before it is formed.
 //
-// Workgroup (seg, J) covers block-column J (64 columns) over the SEG blocks
-// I in [J + seg*SEG, J + (seg+1)*SEG).  Wavefront w owns columns
-// 16w..16w+15 and reads each column's SEG*64 rows top to bottom in one pass
-// (a contiguous 2 KB run per column: column-major A streams in long runs).
-// Lane l holds rows l, l+64, ... and accumulates their row sums over the
-// wavefront's columns (reduced across the 4 wavefronts in LDS and written
-// to ypart[J][rows]); each column's sum against x over the segment's rows
-// goes to ycpart[seg][col]; the workgroup's share of x^T u goes to qpart.
-// Diagonal blocks are read on and below the diagonal only (the trailing
-// update leaves the upper triangle stale): their strictly lower part serves
-// rows and columns, the diagonal serves rows, and
+// Blocks are 64x64 tiles of A at absolute positions (rows and columns
+// [64I, 64I+64)), so every column segment starts on a 512-byte boundary;
+// the trailing matrix's first block row and column are partial (entries
+// before c+1 are skipped).  With j0 = (c+1)/64, local block-column
+// Jl = J - j0.  Workgroup (seg, Jl) covers block-column J over the SEG
+// blocks I in [J + seg*SEG, J + (seg+1)*SEG).  Wavefront w owns columns
+// 16w..16w+15 of the block-column and reads each column's SEG*64 rows top to
+// bottom in one pass (a contiguous 2 KB run per column).  Lane l holds rows
+// l, l+64, ... and accumulates their row sums over the wavefront's columns
+// (reduced across the 4 wavefronts in LDS and written to ypart[Jl][row]);
+// each column's sum against x over the segment's rows goes to
+// ycpart[seg][col]; the workgroup's share of x^T u goes to qpart.  Rows and
+// columns are absolute indices.  Diagonal blocks are read on and below the
+// diagonal only (the trailing update leaves the upper triangle stale): their
+// strictly lower part serves rows and columns, the diagonal serves rows, and
 // x^T (L + D + L^T) x = 2 x^T (L + D) x - x^T D x.
-// Workgroups in row y = nbr form the 2jj unscaled dot products
-// sigma = [V W]^T x over rows [c+1, n), one wavefront each.
+// Workgroups in rows y >= nbl form the 2jj unscaled dot products
+// sigma = [V W]^T x over rows [c+1, n) in chunks of DCH rows, one chunk per
+// wavefront (partials in dpart[q][chunk], summed by sy_mid): a whole column
+// per wavefront would leave one long serial wavefront as the launch's tail.
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
                                                int jj) {
   __shared__ double xJ[SB];
@@ -103,33 +119,46 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
   const int64_t n = t.n;
   if (c >= n - 1) return;  // this tile is reduced (uniform over its workgroups)
   const double* __restrict__ colc = t.A + (int64_t)c * n;
-  const int64_t s = n - c - 1;
-  const int nbr = (int)((s + SB - 1) / SB);
-  const int J = blockIdx.y;
+  const int64_t c1 = c + 1;
+  const int j0 = (int)(c1 / SB);
+  const int nbl = (int)((n + SB - 1) / SB) - j0;  // local block count
+  const int Jl = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int I0 = J + blockIdx.x * SEG;
+  const int I0l = Jl + blockIdx.x * SEG;
   double qt = 0.0;  // this workgroup's share of x^T u
-  if (J >= nbr) {
-    const int q = blockIdx.x * 4 + w;
-    if (J == nbr && q < 2 * jj) {
+  if (Jl >= nbl) {
+    const int nch = (int)((n - c1 + DCH - 1) / DCH);
+    const int widx = ((Jl - nbl) * gridDim.x + blockIdx.x) * 4 + w;
+    const int q = widx / nch, ch = widx % nch;
+    if (q < 2 * jj) {
       const double* X = q < jj ? t.A + (int64_t)(k + q) * n : t.W + (int64_t)(q - jj) * n;
-      double acc = 0.0;
-      for (int64_t i = c + 1 + lane; i < n; i += 64) acc += X[i] * colc[i];
-      acc = wave_sum(acc);
-      if (lane == 0) t.s[q] = acc;
+      const int64_t i0 = c1 + (int64_t)ch * DCH;
+      const int64_t i1 = min(n, i0 + DCH);
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int64_t i = i0 + lane;
+      for (; i + 192 < i1; i += 256) {
+        a0 += X[i] * colc[i];
+        a1 += X[i + 64] * colc[i + 64];
+        a2 += X[i + 128] * colc[i + 128];
+        a3 += X[i + 192] * colc[i + 192];
+      }
+      for (; i < i1; i += 64) a0 += X[i] * colc[i];
+      const double acc = wave_sum((a0 + a1) + (a2 + a3));
+      if (lane == 0) t.dpart[q * kDotChunks + ch] = acc;
     }
-  } else if (I0 < nbr) {
-    const int nb = min(SEG, nbr - I0);  // blocks in this segment
+  } else if (I0l < nbl) {
+    const int J = j0 + Jl;
+    const int I0 = j0 + I0l;
+    const int nb = min(SEG, nbl - I0l);  // blocks in this segment
     if (tid < SB) {
-      const int64_t r = (int64_t)J * SB + tid;
-      xJ[tid] = (r < s) ? colc[c + 1 + r] : 0.0;
+      const int64_t j = (int64_t)J * SB + tid;
+      xJ[tid] = (j >= c1 && j < n) ? colc[j] : 0.0;
     }
-    const double* __restrict__ A22 = t.A + (c + 1) + (int64_t)(c + 1) * n;
     double xr[SEG], racc[SEG];
 #pragma unroll
     for (int u = 0; u < SEG; ++u) {
-      const int64_t r = (int64_t)(I0 + u) * SB + lane;
-      xr[u] = (u < nb && r < s) ? colc[c + 1 + r] : 0.0;
+      const int64_t i = (int64_t)(I0 + u) * SB + lane;
+      xr[u] = (u < nb && i >= c1 && i < n) ? colc[i] : 0.0;
       racc[u] = 0.0;
     }
     __syncthreads();  // xJ
@@ -137,13 +166,15 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
     const bool diag = (I0 == J);
     for (int q = 0; q < 16; ++q) {
       const int cl = 16 * w + q;
-      const int64_t col = (int64_t)J * SB + cl;
+      const int64_t j = (int64_t)J * SB + cl;
+      const bool jv = j >= c1 && j < n;
+      const double* __restrict__ Aj = t.A + j * n;
       double a[SEG];
 #pragma unroll
       for (int u = 0; u < SEG; ++u) {
-        const int64_t r = (int64_t)(I0 + u) * SB + lane;
-        const bool ok = (u < nb) && r < s && col < s && !(diag && u == 0 && lane < cl);
-        a[u] = ok ? A22[r + col * n] : 0.0;
+        const int64_t i = (int64_t)(I0 + u) * SB + lane;
+        const bool ok = jv && (u < nb) && i >= c1 && i < n && !(diag && u == 0 && lane < cl);
+        a[u] = ok ? Aj[i] : 0.0;
       }
       const double xc = xJ[cl];
       double cs = 0.0;
@@ -155,18 +186,18 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
         if (dg) qacc -= xr[0] * xr[0] * a[0];
       }
       cs = wave_sum(cs);
-      if (lane == 0 && col < s) t.ycpart[(int64_t)blockIdx.x * n + col] = cs;
+      if (lane == 0 && jv) t.ycpart[(int64_t)blockIdx.x * n + j] = cs;
     }
 #pragma unroll
     for (int u = 0; u < SEG; ++u) rred[w][u * SB + lane] = racc[u];
     __syncthreads();
     for (int idx = tid; idx < nb * SB; idx += 256) {
       const int u = idx / SB, l = idx % SB;
-      const int64_t r = (int64_t)(I0 + u) * SB + l;
-      if (r < s) {
+      const int64_t i = (int64_t)(I0 + u) * SB + l;
+      if (i >= c1 && i < n) {
         const double rs = rred[0][idx] + rred[1][idx] + rred[2][idx] + rred[3][idx];
-        t.ypart[(int64_t)J * n + r] = rs;
-        qacc += 2.0 * colc[c + 1 + r] * rs;
+        t.ypart[(int64_t)Jl * n + i] = rs;
+        qacc += 2.0 * colc[i] * rs;
       }
     }
     qt = block_sum(qacc, qred);
@@ -179,14 +210,17 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
 // tau = (beta - alpha)/beta, v = [1, x'/(alpha - beta)]) from the norm
 // partials of x' = x(1:), stored as d[c], e[c], tau[c]; then
 //   v^T A22 v = scal^2 (x^T u - 2 beta u_0 + beta^2 A22(0,0)),
-//   V^T v = scal (sigma_V - beta V(c+1,:)), W^T v likewise (into s),
+//   V^T v = scal (sigma_V - beta V(c+1,:)), W^T v likewise (into s; sigma
+//   summed from sy_symv's chunk partials),
 //   alpha = -1/2 tau^2 (v^T A22 v - 2 (V^T v).(W^T v)),
 //   W(c+1, jj) = tau (y_0 - V(c+1,:) W^T v - W(c+1,:) V^T v) + alpha,
 // y_0 = scal (u_0 - beta A22(0,0)).  scal[0..4] = scal, tau, alpha, beta,
 // W(c+1, jj).  nnorm / nq: partial counts of the norm and of x^T u.
-__global__ __launch_bounds__(256) void sy_mid(const SyTile* __restrict__ tiles, int k, int c,
-                                              int jj, int nnorm, int nq) {
-  __shared__ double red[4];
+constexpr int MID_THREADS = 1024;
+
+__global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__ tiles, int k,
+                                                      int c, int jj, int nnorm, int nq) {
+  __shared__ double red[MID_THREADS / 64];
   __shared__ double ss[2 * 64];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
@@ -195,12 +229,11 @@ __global__ __launch_bounds__(256) void sy_mid(const SyTile* __restrict__ tiles, 
   const int64_t c1 = c + 1;
   const double xn2 = sum_parts(t.part, nnorm, red);
   const double xu = sum_parts(t.qpart, nq, red);
-  const int64_t s = n - c1;
-  const int nbr = (int)((s + SB - 1) / SB);
-  const int nseg0 = (nbr + SEG - 1) / SEG;
+  const int nbl = (int)((n + SB - 1) / SB) - (int)(c1 / SB);
+  const int nseg0 = (nbl + SEG - 1) / SEG;
   double u0p = 0.0;
-  for (int sg = tid; sg < nseg0; sg += blockDim.x) u0p += t.ycpart[(int64_t)sg * n];
-  const double u0 = t.ypart[0] + block_sum(u0p, red);
+  for (int sg = tid; sg < nseg0; sg += blockDim.x) u0p += t.ycpart[(int64_t)sg * n + c1];
+  const double u0 = t.ypart[c1] + block_sum(u0p, red);
   const double alpha_r = t.A[c1 + c * n];
   const double xnorm = sqrt(xn2);
   double tau, beta, scal;
@@ -214,8 +247,11 @@ __global__ __launch_bounds__(256) void sy_mid(const SyTile* __restrict__ tiles, 
     scal = 1.0 / (alpha_r - beta);
   }
   if (tid < 2 * jj) {
+    const int nch = (int)((n - c1 + DCH - 1) / DCH);
+    double sig = 0.0;
+    for (int ch = 0; ch < nch; ++ch) sig += t.dpart[tid * kDotChunks + ch];
     const double* X = tid < jj ? t.A + (int64_t)(k + tid) * n : t.W + (int64_t)(tid - jj) * n;
-    ss[tid] = scal * (t.s[tid] - beta * X[c1]);
+    ss[tid] = scal * (sig - beta * X[c1]);
   }
   __syncthreads();
   if (tid < 2 * jj) t.s[tid] = ss[tid];
@@ -259,16 +295,15 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
   const bool upd_next = in_panel_next || (c + 1 == n - 1);
   const int tid = threadIdx.x, rl = tid & (WR - 1), lane = tid >> 4, w = tid >> 6;
   const int64_t c1 = c + 1;
-  const int64_t s = n - c1;
-  const int nbr = (int)((s + SB - 1) / SB);
+  const int j0 = (int)(c1 / SB);
+  const int nbl = (int)((n + SB - 1) / SB) - j0;
   const int64_t i = c1 + (int64_t)blockIdx.x * WR + rl;
   double qu = 0.0, qc = 0.0, q1 = 0.0;  // lane partials: u, V sW + W sV, update
   if (i < n) {
-    const int64_t r = i - c1;
-    const int R = (int)(r / SB);
-    for (int J = lane; J <= R; J += WL) qu += t.ypart[(int64_t)J * n + r];
-    const int nseg = (nbr - R + SEG - 1) / SEG;
-    for (int sg = lane; sg < nseg; sg += WL) qu += t.ycpart[(int64_t)sg * n + r];
+    const int R = (int)(i / SB) - j0;  // local block row of row i
+    for (int J = lane; J <= R; J += WL) qu += t.ypart[(int64_t)J * n + i];
+    const int nseg = (nbl - R + SEG - 1) / SEG;
+    for (int sg = lane; sg < nseg; sg += WL) qu += t.ycpart[(int64_t)sg * n + i];
     for (int p = lane; p < jj; p += WL) {
       const double Vip = t.A[i + (int64_t)(k + p) * n];
       const double Wip = t.W[i + (int64_t)p * n];
@@ -413,14 +448,17 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
   hipLaunchKernelGGL(sy_colnorm, dim3(nnorm, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k);
   for (int jj = 0; jj < pb; ++jj) {
     const int64_t c = k + jj;
-    const int nbr = ceil_div(nmax - c - 1, SB);
-    const int gx = std::max(std::max(ceil_div(nbr, SEG), ceil_div(2 * jj, 4)), 1);
+    const int nbl = ceil_div(nmax, SB) - (int)((c + 1) / SB);  // block rows of the trailing matrix
+    const int gx = std::max(ceil_div(nbl, SEG), 1);
+    // rows of dot-product workgroups: 2jj dots x chunks, 4 wavefronts each
+    const int ndot = ceil_div((int64_t)2 * jj * ceil_div(nmax - c - 1, DCH), 4);
+    const int gy = nbl + ceil_div(ndot, gx);
     if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYMV, 2, &tok));
-    hipLaunchKernelGGL(sy_symv, dim3(gx, nbr + 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
-                       (int)c, jj);
+    hipLaunchKernelGGL(sy_symv, dim3(gx, gy, ntiles), dim3(256), 0, st, d_tiles, (int)k, (int)c,
+                       jj);
     if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
-    hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k, (int)c, jj,
-                       nnorm, gx * (nbr + 1));
+    hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
+                       (int)c, jj, nnorm, gx * gy);
     const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);
     hipLaunchKernelGGL(sy_wupd, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k, (int)c,
                        jj, (jj + 1 < pb) ? 1 : 0);

@@ -99,9 +99,13 @@ def grad(p, X, y):
     return g
 
 
+SIGMA_FLOOR = 0.1  # decades; the bracket's sigma is sqrt(sigma^2 + floor^2)
+
+
 def bracket(p, X, zsig=3.0):
     q = unpack(p)
     out, _ = forward(p, X)
     mu = q["ym"] + q["ys"] * out[:, 0]
-    sg = q["ys"] * np.exp(np.clip(out[:, 1], -10.0, 5.0))
+    sn = q["ys"] * np.exp(np.clip(out[:, 1], -10.0, 5.0))
+    sg = np.sqrt(sn * sn + SIGMA_FLOOR ** 2)
     return np.stack([mu - zsig * sg, mu + zsig * sg], axis=1)

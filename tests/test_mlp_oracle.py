@@ -51,3 +51,13 @@ def test_bracket_contains_mean():
     q = M.unpack(p)
     mu = q["ym"] + q["ys"] * out[:, 0]
     assert np.all(br[:, 0] < mu) and np.all(mu < br[:, 1])
+
+
+def test_bracket_width_floor():
+    """The bracket is never narrower than 2 z sigma_floor, however confident
+    the network is (a very negative log sigma output)."""
+    p, X, y = _setup(2)
+    p2 = p.copy()
+    p2[M.OFF_B3 + 1] = -50.0  # log sigma bias: sigma -> 0
+    br = M.bracket(p2, X, 3.0)
+    assert np.all(br[:, 1] - br[:, 0] >= 2 * 3.0 * M.SIGMA_FLOOR * (1 - 1e-9))
