@@ -51,7 +51,7 @@ using imcom_ctx_detail::kNBB;
 // back-transform block nbb.
 struct EigWork {
   DevBuf sW, sVW, sWV, sy, sycp, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb,
-      sTb, sY, sY2, slam, sZ, stile, sdp;
+      sTb, sY, sY2, slam, sZ, stile, sdp, sAt;
   hipError_t reserve(int64_t n, int nb, int nbb, double* A, double* Z, SyTile& t) {
     const size_t d = 8;
     IMCOM_HIP_TRY(sW.reserve(n * nb * d));
@@ -78,8 +78,12 @@ struct EigWork {
     IMCOM_HIP_TRY(slam.reserve(n * d));
     IMCOM_HIP_TRY(sZ.reserve((size_t)n * n * d));
     IMCOM_HIP_TRY(stile.reserve(sizeof(SyTile)));
+    const int nbt = (int)((n + 63) / 64);
+    IMCOM_HIP_TRY(sAt.reserve((size_t)nbt * (nbt + 1) / 2 * 4096 * d));
     t.n = n;
     t.A = A;
+    t.At = sAt.as<double>();
+    t.nbt = nbt;
     t.W = sW.as<double>();
     t.VW = sVW.as<double>();
     t.WV = sWV.as<double>();
@@ -106,7 +110,8 @@ struct EigWork {
   }
   void release() {
     DevBuf* b[] = {&sW, &sVW, &sWV, &sy, &sycp, &swpre, &svtmp, &sd, &se, &stau, &spart,
-                   &spart2, &ss, &sscal, &sVb, &sSb, &sTb, &sY, &sY2, &slam, &sZ, &stile, &sdp};
+                   &spart2, &ss, &sscal, &sVb, &sSb, &sTb, &sY, &sY2, &slam, &sZ, &stile, &sdp,
+                   &sAt};
     for (DevBuf* p : b) p->release();
   }
 };

@@ -171,6 +171,7 @@ __global__ __launch_bounds__(256, (BM >= 128 ? 2 : 1)) void dgemm_kernel(GemmDes
   const double alpha = g.alpha, beta = g.beta;
   double* __restrict__ C = g.C;
   const int64_t ldc = g.ldc;
+  double* __restrict__ Ct = g.Ct;
 #pragma unroll
   for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -180,9 +181,20 @@ __global__ __launch_bounds__(256, (BM >= 128 ? 2 : 1)) void dgemm_kernel(GemmDes
         const int gi = m0 + wm0 + i * 16 + lr;
         const int gj = n0 + wn0 + j * 16 + 4 * r + lk;
         if (gi < M && gj < N) {
-          double* c = C + gi + (int64_t)gj * ldc;
-          const double v = alpha * acc[i][j][r];
-          *c = (beta == 0.0) ? v : v + beta * (*c);
+          const double v0 = alpha * acc[i][j][r];
+          if (Ct) {
+            const int64_t ai = g.ct_off + gi, aj = g.ct_off + gj;
+            const int bi = (int)(ai >> 6), bj = (int)(aj >> 6);
+            if (bi >= bj) {
+              double* ct = Ct + tri_block_offset(bi, bj, g.ct_nbt) + (aj & 63) * 64 + (ai & 63);
+              const double v = (beta == 0.0) ? v0 : v0 + beta * (*ct);
+              *ct = v;
+              if (gj < g.cm_cols) C[gi + (int64_t)gj * ldc] = v;
+            }
+          } else {
+            double* c = C + gi + (int64_t)gj * ldc;
+            *c = (beta == 0.0) ? v0 : v0 + beta * (*c);
+          }
         }
       }
 }
@@ -223,9 +235,10 @@ hipError_t dgemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, double a
 
 hipError_t dgemm_lower(hipStream_t st, bool ta, bool tb, int M, int K, double alpha,
                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                       double* C, int64_t ldc) {
+                       double* C, int64_t ldc, double* Ct, int64_t ct_off, int ct_nbt,
+                       int cm_cols) {
   if (M <= 0) return hipSuccess;
-  GemmDesc g{M, M, K, alpha, beta, A, lda, B, ldb, C, ldc, 1};
+  GemmDesc g{M, M, K, alpha, beta, A, lda, B, ldb, C, ldc, 1, Ct, ct_off, ct_nbt, cm_cols};
   const int64_t T128 = (M + 127) / 128;
   if (T128 * (T128 + 1) / 2 >= 512) {
     dim3 grid((unsigned)(T128 * (T128 + 1) / 2));

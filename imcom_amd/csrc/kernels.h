@@ -53,7 +53,25 @@ struct GemmDesc {
   double* C;
   int64_t ldc;
   int lower = 0;  // 1: only the tiles of C on or below its diagonal (M == N)
+  // Optional block-layout C (see tri_block_offset): element (i, j) of C is
+  // element (ct_off + i, ct_off + j) of a symmetric matrix whose lower
+  // triangle is stored in 64x64 blocks at Ct.  When set, C's old value is
+  // read from and the result written to Ct (blocks on or below the block
+  // diagonal only), and the column-major C receives only columns
+  // j < cm_cols.
+  double* Ct = nullptr;
+  int64_t ct_off = 0;
+  int ct_nbt = 0;
+  int cm_cols = 0;
 };
+
+// Offset (in doubles) of block (I, J), I >= J, in the lower-triangle block
+// layout of an nbt x nbt grid of 64x64 blocks: block-columns one after
+// another, each block column-major (element (i, j) of the block at
+// j * 64 + i).
+__host__ __device__ __forceinline__ int64_t tri_block_offset(int I, int J, int nbt) {
+  return ((int64_t)J * nbt - (int64_t)J * (J + 1) / 2 + I) * 4096;
+}
 
 hipError_t dgemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, double alpha,
                  const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
@@ -62,7 +80,8 @@ hipError_t dgemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, double a
 // update, whose upper triangle is never read.
 hipError_t dgemm_lower(hipStream_t st, bool ta, bool tb, int M, int K, double alpha,
                        const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                       double* C, int64_t ldc);
+                       double* C, int64_t ldc, double* Ct = nullptr, int64_t ct_off = 0,
+                       int ct_nbt = 0, int cm_cols = 0);
 int gemm_tiles64(int M, int N);
 int gemm_tiles64(const GemmDesc& g);
 hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
@@ -123,6 +142,8 @@ hipError_t launch_U_from_K(hipStream_t st, int64_t n, int64_t m, const double* L
 struct SyTile {
   int64_t n;
   double* A;      // n x n, overwritten by the reflectors (lower part)
+  double* At;     // lower-triangle 64x64 blocks of A (tri_block_offset), read by sy_symv
+  int nbt;        // ceil(n / 64)
   double* W;      // n x NB
   double* VW;     // n x 2NB  [V W]
   double* WV;     // n x 2NB  [W V]
@@ -161,7 +182,7 @@ hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
 struct SyPlan {
   GroupedPlan gp;
   std::vector<int> syr2k;            // grouped launch per panel
-  std::vector<std::array<int, 4>> bt;  // grouped launches per back-transform block
+  std::vector<std::array<int, 6>> bt;  // grouped launches per back-transform block
 };
 void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb, SyPlan& plan);
 hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,

@@ -83,6 +83,21 @@ constexpr int SB = 64;  // symmetric matrix-vector block
 constexpr int SEG = 4;  // blocks of one block-column per workgroup
 constexpr int DCH = 2048;  // rows per dot-product chunk (kDotChunks * DCH >= 32768)
 
+// The lower-triangle 64x64 blocks of A into the block layout At
+// (tri_block_offset), before the first panel.  Later panels' blocks are
+// written by the trailing update's GEMM epilogue.
+__global__ __launch_bounds__(256) void sy_tile_init(const SyTile* __restrict__ tiles) {
+  const SyTile t = tiles[blockIdx.z];
+  const int I = blockIdx.x, J = blockIdx.y;
+  if (I >= t.nbt || J > I) return;
+  const int64_t n = t.n;
+  double* __restrict__ dst = t.At + tri_block_offset(I, J, t.nbt);
+  for (int e = threadIdx.x; e < 4096; e += 256) {
+    const int64_t i = (int64_t)I * 64 + (e & 63), j = (int64_t)J * 64 + (e >> 6);
+    dst[e] = (i < n && j < n) ? t.A[i + j * n] : 0.0;
+  }
+}
+
 // u = A22 x for the unscaled column x = A(c+1:n, c), reading only the lower
 // triangle (diagonal included) of the stored trailing matrix A22 = A(c+1:n,
 // c+1:n).  The reflector's v is scal * (x - beta e_0), so
@@ -90,19 +105,27 @@ constexpr int DCH = 2048;  // rows per dot-product chunk (kDotChunks * DCH >= 32
 // v^T A22 v = scal^2 (x^T u - 2 beta u_0 + beta^2 A22(0,0)): this kernel
 // needs nothing from the reflector and runs before it is formed.
 //
-// Blocks are 64x64 tiles of A at absolute positions (rows and columns
-// [64I, 64I+64)), so every column segment starts on a 512-byte boundary;
-// the trailing matrix's first block row and column are partial (entries
-// before c+1 are skipped).  With j0 = (c+1)/64, local block-column
+// Blocks are the 64x64 tiles of A at absolute positions (rows and columns
+// [64I, 64I+64)), read from the block-layout copy At (each block 32 KB
+// contiguous: measured 5.5 TB/s against 4.3 for the same blocks read out of
+// column-major A).  Within a panel the stored trailing matrix does not
+// change.  The trailing matrix lives in At: the trailing update's GEMM reads
+// and writes it there, and writes column-major A only for the next panel's
+// columns, the only part of the trailing matrix the column kernels read.  The trailing
+// matrix's first block row and column are partial (entries before c+1 are
+// skipped).  With j0 = (c+1)/64, local block-column
 // Jl = J - j0.  Workgroup (seg, Jl) covers block-column J over the SEG
 // blocks I in [J + seg*SEG, J + (seg+1)*SEG).  Wavefront w owns columns
 // 16w..16w+15 of the block-column and reads each column's SEG*64 rows top to
 // bottom in one pass (a contiguous 2 KB run per column).  Lane l holds rows
 // l, l+64, ... and accumulates their row sums over the wavefront's columns
 // (reduced across the 4 wavefronts in LDS and written to ypart[Jl][row]);
-// each column's sum against x over the segment's rows goes to
-// ycpart[seg][col]; the workgroup's share of x^T u goes to qpart.  Rows and
-// columns are absolute indices.  Diagonal blocks are read on and below the
+// each column's sum against x over the segment's rows is collected in LDS
+// and the 64 sums go to ycpart[seg][cols] in one coalesced store; the
+// workgroup's share of x^T u goes to qpart.  Rows and
+// columns are absolute indices; the partials are written with non-temporal
+// stores (they are read once, by the next two kernels).  Diagonal blocks are
+// read on and below the
 // diagonal only (the trailing update leaves the upper triangle stale): their
 // strictly lower part serves rows and columns, the diagonal serves rows, and
 // x^T (L + D + L^T) x = 2 x^T (L + D) x - x^T D x.
@@ -113,6 +136,7 @@ constexpr int DCH = 2048;  // rows per dot-product chunk (kDotChunks * DCH >= 32
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
                                                int jj) {
   __shared__ double xJ[SB];
+  __shared__ double csum[SB];
   __shared__ double rred[4][SEG * SB];
   __shared__ double qred[4];
   const SyTile t = tiles[blockIdx.z];
@@ -164,39 +188,51 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
     __syncthreads();  // xJ
     double qacc = 0.0;
     const bool diag = (I0 == J);
-    for (int q = 0; q < 16; ++q) {
-      const int cl = 16 * w + q;
-      const int64_t j = (int64_t)J * SB + cl;
-      const bool jv = j >= c1 && j < n;
-      const double* __restrict__ Aj = t.A + j * n;
-      double a[SEG];
+    // two columns per step: eight loads in flight per lane
+#pragma unroll 1
+    for (int q0 = 0; q0 < 16; q0 += 2) {
+      double a[2][SEG];
 #pragma unroll
-      for (int u = 0; u < SEG; ++u) {
-        const int64_t i = (int64_t)(I0 + u) * SB + lane;
-        const bool ok = jv && (u < nb) && i >= c1 && i < n && !(diag && u == 0 && lane < cl);
-        a[u] = ok ? Aj[i] : 0.0;
-      }
-      const double xc = xJ[cl];
-      double cs = 0.0;
+      for (int p = 0; p < 2; ++p) {
+        const int cl = 16 * w + q0 + p;
+        const int64_t j = (int64_t)J * SB + cl;
+        const bool jv = j >= c1 && j < n;
 #pragma unroll
-      for (int u = 0; u < SEG; ++u) {
-        racc[u] += a[u] * xc;
-        const bool dg = diag && u == 0 && lane == cl;
-        cs += dg ? 0.0 : a[u] * xr[u];
-        if (dg) qacc -= xr[0] * xr[0] * a[0];
+        for (int u = 0; u < SEG; ++u) {
+          const int64_t i = (int64_t)(I0 + u) * SB + lane;
+          const bool ok = jv && (u < nb) && i >= c1 && i < n && !(diag && u == 0 && lane < cl);
+          a[p][u] = ok ? t.At[tri_block_offset(I0 + u, J, t.nbt) + cl * SB + lane] : 0.0;
+        }
       }
-      cs = wave_sum(cs);
-      if (lane == 0 && jv) t.ycpart[(int64_t)blockIdx.x * n + j] = cs;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        const int cl = 16 * w + q0 + p;
+        const double xc = xJ[cl];
+        double cs = 0.0;
+#pragma unroll
+        for (int u = 0; u < SEG; ++u) {
+          racc[u] += a[p][u] * xc;
+          const bool dg = diag && u == 0 && lane == cl;
+          cs += dg ? 0.0 : a[p][u] * xr[u];
+          if (dg) qacc -= xr[0] * xr[0] * a[p][0];
+        }
+        cs = wave_sum(cs);
+        if (lane == 0) csum[cl] = cs;
+      }
     }
 #pragma unroll
     for (int u = 0; u < SEG; ++u) rred[w][u * SB + lane] = racc[u];
     __syncthreads();
+    if (tid < SB) {  // the block-column's 64 column sums, one coalesced store
+      const int64_t j = (int64_t)J * SB + tid;
+      if (j >= c1 && j < n) t.ycpart[(int64_t)blockIdx.x * n + j] = csum[tid];
+    }
     for (int idx = tid; idx < nb * SB; idx += 256) {
       const int u = idx / SB, l = idx % SB;
       const int64_t i = (int64_t)(I0 + u) * SB + l;
       if (i >= c1 && i < n) {
         const double rs = rred[0][idx] + rred[1][idx] + rred[2][idx] + rred[3][idx];
-        t.ypart[(int64_t)Jl * n + i] = rs;
+        __builtin_nontemporal_store(rs, t.ypart + (int64_t)Jl * n + i);
         qacc += 2.0 * colc[i] * rs;
       }
     }
@@ -256,7 +292,8 @@ __global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__
   __syncthreads();
   if (tid < 2 * jj) t.s[tid] = ss[tid];
   if (tid == 0) {
-    const double g0 = t.A[c1 + c1 * n];
+    const double g0 =
+        t.At[tri_block_offset((int)(c1 >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 65];
     const double vAv = scal * scal * (xu - 2.0 * beta * u0 + beta * beta * g0);
     double svw = 0.0, corr = 0.0;
     for (int p = 0; p < jj; ++p) {
@@ -330,7 +367,11 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
     const double Cr = red[1][0][rl] + red[1][1][rl] + red[1][2][rl] + red[1][3][rl];
     const double Q1 = red[2][0][rl] + red[2][1][rl] + red[2][2][rl] + red[2][3][rl];
     const double scal = t.scal[0], tau = t.scal[1], alpha = t.scal[2], beta = t.scal[3];
-    const double g = t.A[i + c1 * n];  // A22(:, 0), stored
+    // A22(:, 0) as stored (the last trailing update's value: column c+1 of
+    // the block copy; column-major A may not hold it when c+1 starts the next
+    // panel)
+    const double g = t.At[tri_block_offset((int)(i >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 64 +
+                          (i & 63)];
     const double v = (i == c1) ? 1.0 : t.A[i + (int64_t)c * n] * scal;
     const double y = scal * (U - beta * g);
     const double wv = tau * (y - Cr) + alpha * v;
@@ -391,12 +432,16 @@ __global__ void bt_gather(const SyTile* __restrict__ tiles, int kb, int nbb) {
   t.Vb[r + (int64_t)p * rows] = v;
 }
 
-constexpr int kMaxNBB = 128;  // largest back-transform block bt_build_T holds in LDS
+constexpr int kMaxNBB = 128;  // largest diagonal block of T that bt_build_T holds in LDS
 
-// T(j,j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) S(0:j, j), column by column.
-// T lives in LDS (column-major, ld kMaxNBB, 128 KB); thread i owns row i, so
-// a column's rows depend only on the same thread's earlier entries and the
-// shared S column, which is loaded one column ahead.
+// T(j,j) = tau_j, T(0:j, j) = -tau_j T(0:j, 0:j) S(0:j, j), column by column,
+// for the diagonal block T(o:o+len, o:o+len), o = blockIdx.x * kMaxNBB,
+// len = min(nbb - o, kMaxNBB) (S and T are nbb x nbb, ld nbb).  The block
+// lives in LDS (column-major, ld kMaxNBB, 128 KB); thread i owns row i, so a
+// column's rows depend only on the same thread's earlier entries and the
+// shared S column, which is loaded one column ahead.  Workgroups past the
+// first also zero the rows of T below the first diagonal block
+// (T(o:o+len, 0:o)); the block above them is formed by sy_t_join.
 __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ tiles, int kb,
                                                   int nbb) {
   __shared__ double Ts[kMaxNBB * kMaxNBB];
@@ -404,16 +449,19 @@ __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ til
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
   if (kb >= n - 1) return;
-  const double* __restrict__ S = t.Sb;
+  const int o = blockIdx.x * kMaxNBB;
+  const int len = min(nbb - o, kMaxNBB);
+  const double* __restrict__ S = t.Sb + o + (int64_t)o * nbb;  // S(o.., o..), ld nbb
+  double* __restrict__ T = t.Tb + o + (int64_t)o * nbb;
   const int i = threadIdx.x;
   double sreg = 0.0;  // S(i, j) for the coming column j
-  for (int j = 0; j < nbb; ++j) {
+  for (int j = 0; j < len; ++j) {
     if (i < j) sS[j & 1][i] = sreg;
-    const double snext = (j + 1 < nbb && i < j + 1) ? S[i + (int64_t)(j + 1) * nbb] : 0.0;
+    const double snext = (j + 1 < len && i < j + 1) ? S[i + (int64_t)(j + 1) * nbb] : 0.0;
     __syncthreads();
-    const int64_t c = kb + j;
+    const int64_t c = kb + o + j;
     const double tj = (c < n - 1) ? t.tau[c] : 0.0;
-    if (i < nbb) {
+    if (i < len) {
       double v;
       if (i < j) {
         const double* sj = sS[j & 1];
@@ -435,8 +483,11 @@ __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ til
     sreg = snext;
   }
   __syncthreads();
-  for (int idx = i; idx < nbb * nbb; idx += blockDim.x)
-    t.Tb[idx] = Ts[(idx / nbb) * kMaxNBB + idx % nbb];
+  for (int idx = i; idx < len * len; idx += blockDim.x)
+    T[idx % len + (int64_t)(idx / len) * nbb] = Ts[(idx / len) * kMaxNBB + idx % len];
+  // rows o.., columns 0..o-1: zero (T is upper triangular)
+  for (int idx = i; idx < len * o; idx += blockDim.x)
+    t.Tb[o + idx % len + (int64_t)(idx / len) * nbb] = 0.0;
 }
 
 // The pb columns of the panel starting at column k: the first column's norm
@@ -470,6 +521,8 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nb, Prof* prof) {
   size_t tok = 0;
+  const int nbt = ceil_div(nmax, SB);
+  hipLaunchKernelGGL(sy_tile_init, dim3(nbt, nbt, ntiles), dim3(256), 0, st, d_tiles);
   for (int64_t k = 0; k < nmax - 1; k += nb) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
     IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, prof));
@@ -484,7 +537,8 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
         if (r0 >= h.n) continue;
         const int64_t mm = h.n - r0;
         IMCOM_HIP_TRY(dgemm_lower(st, false, true, (int)mm, 2 * nb, -1.0, h.VW + r0, h.n,
-                                  h.WV + r0, h.n, 1.0, h.A + r0 + r0 * h.n, h.n));
+                                  h.WV + r0, h.n, 1.0, h.A + r0 + r0 * h.n, h.n, h.At, r0,
+                                  h.nbt, nb));
       }
       if (prof) IMCOM_HIP_TRY(prof->end(PS_SYR2K, tok));
     }
@@ -494,7 +548,9 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
 }
 
 // Q = H(0) ... H(n-2) Z, applied in blocks of NBB reflectors, last block
-// first: Z[kb+1:, :] -= Vb (T (Vb^T Z[kb+1:, :])).  Z is n x n (ld n).
+// first: Z[kb+1:, :] -= Vb (T (Vb^T Z[kb+1:, :])).  Z is n x n (ld n).  For
+// NBB = 256, T's two 128 diagonal blocks come from bt_build_T and the block
+// above them is T12 = -T1 (V1^T V2) T2 (the DLARFT recursion).
 hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nbb) {
   if (nmax <= 1) return hipSuccess;
@@ -512,12 +568,21 @@ hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
       IMCOM_HIP_TRY(dgemm(st, true, false, nbb, nbb, (int)rows, 1.0, h.Vb, rows, h.Vb, rows, 0.0,
                           h.Sb, nbb));
     }
-    hipLaunchKernelGGL(bt_build_T, dim3(1, 1, ntiles), dim3(256), 0, st, d_tiles, kb, nbb);
+    hipLaunchKernelGGL(bt_build_T, dim3(ceil_div(nbb, kMaxNBB), 1, ntiles), dim3(256), 0, st,
+                       d_tiles, kb, nbb);
     for (int b = 0; b < ntiles; ++b) {
       const SyTile& h = host[b];
       if (kb >= h.n - 1) continue;
       const int64_t rows = h.n - kb - 1;
       double* Zr = h.Z + kb + 1;
+      if (nbb > kMaxNBB) {
+        // T(0:128, 128:nbb) = -T1 S12 T2 (the two diagonal blocks joined; X in Y2)
+        const int n2 = nbb - kMaxNBB;
+        IMCOM_HIP_TRY(dgemm(st, false, false, kMaxNBB, n2, n2, 1.0, h.Sb + (int64_t)kMaxNBB * nbb,
+                            nbb, h.Tb + kMaxNBB + (int64_t)kMaxNBB * nbb, nbb, 0.0, h.Y2, kMaxNBB));
+        IMCOM_HIP_TRY(dgemm(st, false, false, kMaxNBB, n2, kMaxNBB, -1.0, h.Tb, nbb, h.Y2, kMaxNBB,
+                            0.0, h.Tb + (int64_t)kMaxNBB * nbb, nbb));
+      }
       // Y = Vb^T Z_r  (nbb x n)
       IMCOM_HIP_TRY(dgemm(st, true, false, nbb, (int)h.n, (int)rows, 1.0, h.Vb, rows, Zr, h.n, 0.0,
                           h.Y, nbb));
@@ -551,7 +616,7 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
       if (r0 >= h.n) continue;
       const int mm = (int)(h.n - r0);
       g.push_back(GemmDesc{mm, mm, 2 * nb, -1.0, 1.0, h.VW + r0, h.n, h.WV + r0, h.n,
-                           h.A + r0 + r0 * h.n, h.n, 1});
+                           h.A + r0 + r0 * h.n, h.n, 1, h.At, r0, h.nbt, nb});
     }
     plan.syr2k.push_back(plan.gp.add(false, true, g));
   }
@@ -559,7 +624,8 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
     const int64_t nblk = (nmax - 1 + nbb - 1) / nbb;
     for (int64_t bi = nblk - 1; bi >= 0; --bi) {
       const int kb = (int)(bi * nbb);
-      std::vector<GemmDesc> gs, gy, gy2, gz;
+      std::vector<GemmDesc> gs, gy, gy2, gz, gj1, gj2;
+      const int n2 = nbb - kMaxNBB;
       for (const SyTile& h : host) {
         if (kb >= h.n - 1) continue;
         const int rows = (int)(h.n - kb - 1);
@@ -569,12 +635,20 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
         gy.push_back(GemmDesc{nbb, n, rows, 1.0, 0.0, h.Vb, rows, Zr, h.n, h.Y, nbb});
         gy2.push_back(GemmDesc{nbb, n, nbb, 1.0, 0.0, h.Tb, nbb, h.Y, nbb, h.Y2, nbb});
         gz.push_back(GemmDesc{rows, n, nbb, -1.0, 1.0, h.Vb, rows, h.Y2, nbb, Zr, h.n});
+        if (n2 > 0) {
+          gj1.push_back(GemmDesc{kMaxNBB, n2, n2, 1.0, 0.0, h.Sb + (int64_t)kMaxNBB * nbb, nbb,
+                                 h.Tb + kMaxNBB + (int64_t)kMaxNBB * nbb, nbb, h.Y2, kMaxNBB});
+          gj2.push_back(GemmDesc{kMaxNBB, n2, kMaxNBB, -1.0, 0.0, h.Tb, nbb, h.Y2, kMaxNBB,
+                                 h.Tb + (int64_t)kMaxNBB * nbb, nbb});
+        }
       }
-      std::array<int, 4> ids;
+      std::array<int, 6> ids;
       ids[0] = plan.gp.add(true, false, gs);
       ids[1] = plan.gp.add(true, false, gy);
       ids[2] = plan.gp.add(false, false, gy2);
       ids[3] = plan.gp.add(false, false, gz);
+      ids[4] = plan.gp.add(false, false, gj1);
+      ids[5] = plan.gp.add(false, false, gj2);
       plan.bt.push_back(ids);
     }
   }
@@ -583,6 +657,8 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
 hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                              int nb, const SyPlan& plan, Prof* prof) {
   size_t tok = 0;
+  const int nbt = ceil_div(nmax, SB);
+  hipLaunchKernelGGL(sy_tile_init, dim3(nbt, nbt, ntiles), dim3(256), 0, st, d_tiles);
   int panel = 0;
   for (int64_t k = 0; k < nmax - 1; k += nb, ++panel) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
@@ -607,11 +683,14 @@ hipError_t ormtr_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, 
   int idx = 0;
   for (int64_t bi = nblk - 1; bi >= 0; --bi, ++idx) {
     const int kb = (int)(bi * nbb);
-    const std::array<int, 4>& ids = plan.bt[idx];
+    const std::array<int, 6>& ids = plan.bt[idx];
     hipLaunchKernelGGL(bt_gather, dim3(ceil_div(nmax - kb - 1, 256), nbb, ntiles), dim3(256), 0,
                        st, d_tiles, kb, nbb);
     IMCOM_HIP_TRY(plan.gp.run(st, ids[0]));
-    hipLaunchKernelGGL(bt_build_T, dim3(1, 1, ntiles), dim3(256), 0, st, d_tiles, kb, nbb);
+    hipLaunchKernelGGL(bt_build_T, dim3(ceil_div(nbb, kMaxNBB), 1, ntiles), dim3(256), 0, st,
+                       d_tiles, kb, nbb);
+    IMCOM_HIP_TRY(plan.gp.run(st, ids[4]));
+    IMCOM_HIP_TRY(plan.gp.run(st, ids[5]));
     IMCOM_HIP_TRY(plan.gp.run(st, ids[1]));
     IMCOM_HIP_TRY(plan.gp.run(st, ids[2]));
     IMCOM_HIP_TRY(plan.gp.run(st, ids[3]));

@@ -168,7 +168,7 @@ typedef struct imcom_gpu_tile {
 
 /* Eigensolver tuning picked by the autotuner (imcom_amd/autotune.py): the
  * reduction panel width nb (16, 32, 64) and the back-transform block nbb
- * (64 or 128); applies to later eigen_A calls and batch_set. */
+ * (64, 128 or 256); applies to later eigen_A calls and batch_set. */
 int imcom_gpu_set_tuning(imcom_gpu_ctx* ctx, int nb, int nbb);
 
 int imcom_gpu_batch_set(imcom_gpu_ctx* ctx, int ntiles, const imcom_gpu_tile* tiles);

@@ -174,9 +174,10 @@ def test_tridiagonalize_small(gpu_ctx, small_tile, small_ref):
     assert np.max(np.abs(lam - ref)) <= 1e3 * n * EPS * nrm
 
 
-@pytest.mark.parametrize("n", [5, 130, 700])
-def test_eigen_random_spd(gpu_ctx, n):
-    _say(f"eigen_A random n={n}")
+@pytest.mark.parametrize("n,nb,nbb", [(5, 32, 128), (130, 32, 128), (700, 32, 128),
+                                       (700, 64, 256), (1100, 16, 256)])
+def test_eigen_random_spd(gpu_ctx, n, nb, nbb):
+    _say(f"eigen_A random n={n} nb={nb} nbb={nbb}")
     rng = np.random.default_rng(3 * n)
     X = rng.standard_normal((n, n))
     A = np.asfortranarray(X @ X.T / n)
@@ -188,7 +189,11 @@ def test_eigen_random_spd(gpu_ctx, n):
     t.Ndiag = np.ones(n)
     gpu_ctx.set_tile(t)
     gpu_ctx.set_A(A)
-    Q, L, cond = gpu_ctx.eigen_A()
+    gpu_ctx.set_tuning(nb, nbb)
+    try:
+        Q, L, cond = gpu_ctx.eigen_A()
+    finally:
+        gpu_ctx.set_tuning(32, 128)
     nrm = np.max(np.abs(A))
     assert np.all(np.diff(L) <= 0)
     assert np.max(np.abs(A @ Q - Q * L)) <= 1e3 * n * EPS * nrm
@@ -197,7 +202,7 @@ def test_eigen_random_spd(gpu_ctx, n):
     assert np.max(np.abs(L - Lr)) <= 1e3 * n * EPS * nrm
 
 
-@pytest.mark.parametrize("nb,nbb", [(32, 128), (16, 64), (64, 128)])
+@pytest.mark.parametrize("nb,nbb", [(32, 128), (16, 64), (64, 128), (32, 256)])
 def test_eigen_accuracy_report(gpu_ctx, nb, nbb):
     """Backward error, orthogonality and eigenvalue error of the GPU
     eigensolver, in units of n*eps (printed), on a tile whose pixels sit near
