@@ -5,6 +5,8 @@ For each size class it times eigen_A (device time of the reduction and the
 back-transform, HIP events) on a seeded SPD matrix of the class's size for
 every candidate (nb, nbb), keeps the fastest, and stores the table in
 imcom_amd/data/tuning.json.  ImcomGPU users apply it with apply(g, n).
+
+  python -m imcom_amd.autotune [--classes 1024,2048,4096,8192,16384] [--out PATH]
 """
 from __future__ import annotations
 
@@ -76,7 +78,9 @@ def tune(g, classes=(1024, 2048, 4096), candidates=CANDIDATES, repeats=2, seed=0
                 g.eigen_A(out=False)
                 st = g.stage_times()
                 g.set_profiling(0)
-                ms = st["sytrd"][0] + st["stedc"][0] + st["ormtr"][0]
+                # nb and nbb act on the reduction and the back-transform only
+                # (the tridiagonal eigensolver between them does not see them)
+                ms = st["sytrd"][0] + st["ormtr"][0]
                 best = ms if best is None else min(best, ms)
             timings[f"{nb},{nbb}"] = round(best, 3)
         nb, nbb = (int(v) for v in min(timings, key=timings.get).split(","))
@@ -89,3 +93,26 @@ def save_table(table: dict, path: str = TABLE):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
+
+
+def main(argv=None):
+    import argparse
+    from .gpu import ImcomGPU
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--classes", default=",".join(str(c) for c in CLASSES))
+    ap.add_argument("--out", default=TABLE)
+    ap.add_argument("--repeats", type=int, default=2)
+    a = ap.parse_args(argv)
+    classes = [int(v) for v in a.classes.split(",")]
+    with ImcomGPU(0) as g:
+        table = {}
+        for n in classes:  # one class at a time, printed as it finishes
+            table.update(tune(g, classes=(n,), repeats=a.repeats))
+            e = table[str(n)]
+            print(f"n={n}: nb={e['nb']} nbb={e['nbb']} {e['ms']}", flush=True)
+    save_table(table, a.out)
+    return table
+
+
+if __name__ == "__main__":
+    main()
