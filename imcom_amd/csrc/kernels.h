@@ -11,7 +11,8 @@ namespace imcom {
 // ---- profiling --------------------------------------------------------------
 // Device-time accounting with HIP events on the context's stream.  Slots
 // name the stages of the solve path; per-launch timing of the tridiagonal
-// reduction's matrix-vector kernel is kept in its own slot (level 2).
+// reduction's matrix-vector kernel is kept in its own slot (level 2), with
+// the events recorded by the kernel dispatch itself.
 enum ProfSlot {
   PS_LOOKUP_A = 0,
   PS_SYTRD,
@@ -37,6 +38,9 @@ struct Prof {
   int64_t count[PS_NSLOTS] = {0};
   hipError_t begin(int slot, int need_level, size_t* token);
   hipError_t end(int slot, size_t token);
+  // a start/stop event pair for one kernel launch (hipExtLaunchKernelGGL records
+  // them at the kernel's own start and end); both null below need_level
+  hipError_t events(int slot, int need_level, hipEvent_t* e0, hipEvent_t* e1);
   hipError_t collect();
   void reset();
   void destroy();

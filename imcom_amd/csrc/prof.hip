@@ -27,6 +27,21 @@ hipError_t Prof::end(int slot, size_t token) {
   return hipSuccess;
 }
 
+hipError_t Prof::events(int slot, int need_level, hipEvent_t* e0, hipEvent_t* e1) {
+  *e0 = *e1 = nullptr;
+  if (level < need_level) return hipSuccess;
+  while (pool.size() < used + 2) {
+    hipEvent_t e;
+    IMCOM_HIP_TRY(hipEventCreate(&e));
+    pool.push_back(e);
+  }
+  *e0 = pool[used];
+  *e1 = pool[used + 1];
+  pending.emplace_back(slot, used);
+  used += 2;
+  return hipSuccess;
+}
+
 hipError_t Prof::collect() {
   if (pending.empty()) return hipSuccess;
   IMCOM_HIP_TRY(hipStreamSynchronize(st));

@@ -32,6 +32,8 @@
 //
 // Tiles of a batch advance column by column in lockstep; blockIdx.z selects
 // the tile and a tile whose columns are exhausted exits.
+#include <hip/hip_ext.h>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -504,10 +506,10 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
     // rows of dot-product workgroups: 2jj dots x chunks, 4 wavefronts each
     const int ndot = ceil_div((int64_t)2 * jj * ceil_div(nmax - c - 1, DCH), 4);
     const int gy = nbl + ceil_div(ndot, gx);
-    if (prof) IMCOM_HIP_TRY(prof->begin(PS_SYMV, 2, &tok));
-    hipLaunchKernelGGL(sy_symv, dim3(gx, gy, ntiles), dim3(256), 0, st, d_tiles, (int)k, (int)c,
-                       jj);
-    if (prof) IMCOM_HIP_TRY(prof->end(PS_SYMV, tok));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (prof) IMCOM_HIP_TRY(prof->events(PS_SYMV, 2, &e0, &e1));
+    hipExtLaunchKernelGGL(sy_symv, dim3(gx, gy, ntiles), dim3(256), 0, st, e0, e1, 0, d_tiles,
+                          (int)k, (int)c, jj);
     hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
                        (int)c, jj, nnorm, gx * gy);
     const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);
