@@ -113,24 +113,21 @@ __global__ __launch_bounds__(256) void sy_tile_init(const SyTile* __restrict__ t
 // column-major A).  Within a panel the stored trailing matrix does not
 // change.  The trailing matrix lives in At: the trailing update's GEMM reads
 // and writes it there, and writes column-major A only for the next panel's
-// columns, the only part of the trailing matrix the column kernels read.  The trailing
-// matrix's first block row and column are partial (entries before c+1 are
-// skipped).  With j0 = (c+1)/64, local block-column
+// columns, the only part of the trailing matrix the column kernels read.
+// The trailing matrix's first block row and column are partial (entries
+// before c+1 are skipped).  With j0 = (c+1)/64, local block-column
 // Jl = J - j0.  Workgroup (seg, Jl) covers block-column J over the SEG
 // blocks I in [J + seg*SEG, J + (seg+1)*SEG).  Wavefront w owns columns
-// 16w..16w+15 of the block-column and reads each column's SEG*64 rows top to
-// bottom in one pass (a contiguous 2 KB run per column).  Lane l holds rows
-// l, l+64, ... and accumulates their row sums over the wavefront's columns
-// (reduced across the 4 wavefronts in LDS and written to ypart[Jl][row]);
-// each column's sum against x over the segment's rows is collected in LDS
-// and the 64 sums go to ycpart[seg][cols] in one coalesced store; the
-// workgroup's share of x^T u goes to qpart.  Rows and
-// columns are absolute indices; the partials are written with non-temporal
-// stores (they are read once, by the next two kernels).  Diagonal blocks are
-// read on and below the
-// diagonal only (the trailing update leaves the upper triangle stale): their
-// strictly lower part serves rows and columns, the diagonal serves rows, and
-// x^T (L + D + L^T) x = 2 x^T (L + D) x - x^T D x.
+// 16w..16w+15 of the block-column and reads them two at a time, all SEG
+// blocks' rows per step.  Lane l holds rows l, l+64, ... and accumulates
+// their row sums over the wavefront's columns (reduced across the 4
+// wavefronts in LDS and written to ypart[Jl][row]); each column's sum against
+// x over the segment's rows is collected in LDS and the 64 sums go to
+// ycpart[seg][cols] in one coalesced store; the workgroup's share of x^T u
+// goes to qpart.  Rows and columns are absolute indices.  Diagonal blocks are
+// read on and below the diagonal only (the trailing update leaves the upper
+// triangle stale): their strictly lower part serves rows and columns, the
+// diagonal serves rows, and x^T (L + D + L^T) x = 2 x^T (L + D) x - x^T D x.
 // Workgroups in rows y >= nbl form the 2jj unscaled dot products
 // sigma = [V W]^T x over rows [c+1, n) in chunks of DCH rows, one chunk per
 // wavefront (partials in dpart[q][chunk], summed by sy_mid): a whole column
@@ -496,7 +493,6 @@ __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ til
 // partials, then three launches per column.
 static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                                    int64_t k, int pb, Prof* prof) {
-  size_t tok = 0;
   int nnorm = std::max(ceil_div(nmax - k - 2, RB), 1);
   hipLaunchKernelGGL(sy_colnorm, dim3(nnorm, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k);
   for (int jj = 0; jj < pb; ++jj) {
