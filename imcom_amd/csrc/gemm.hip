@@ -167,36 +167,92 @@ __global__ __launch_bounds__(256, (BM >= 128 ? 2 : 1)) void dgemm_kernel(GemmDes
 
   // D'[j][i] = sum_k B[k][j] A[i][k] = C[i][j].  The f64 16x16x4 MFMA leaves
   // D'[4*r + l/16][l%16] in lane l, register r: C row (l%16), column
-  // 4*r + l/16 of the sub-tile.
+  // 4*r + l/16 of the sub-tile.  The epilogue runs per 16-row group: all of
+  // the group's old C values are loaded before any store, so the loads are in
+  // flight together (a load-then-store per element serialises on the memory
+  // latency, which dominates short-K products such as the rank-2nb updates).
   const double alpha = g.alpha, beta = g.beta;
   double* __restrict__ C = g.C;
   const int64_t ldc = g.ldc;
   double* __restrict__ Ct = g.Ct;
+  const bool readc = beta != 0.0;
+  if (Ct) {
+    // Ct offset of C(gi, gj) = row part (gi only) + column part (gj only);
+    // both are recomputed for the stores rather than held in registers
+    const int ct_nbt = g.ct_nbt, cm_cols = g.cm_cols;
+    const int64_t ct_off = g.ct_off;
+    auto col_part = [&](int gj) -> int64_t {
+      const int64_t aj = ct_off + gj;
+      const int64_t bj = aj >> 6;
+      return (bj * ct_nbt - bj * (bj + 1) / 2) * 4096 + (aj & 63) * 64;
+    };
 #pragma unroll
-  for (int i = 0; i < TM; ++i)
+    for (int i = 0; i < TM; ++i) {
+      const int gi = m0 + wm0 + i * 16 + lr;
+      const int64_t ai = ct_off + gi;
+      const int64_t bi = ai >> 6;
+      double* __restrict__ ctrow = Ct + bi * 4096 + (ai & 63);
+      double v[TN][4];
+      unsigned mask = 0;
 #pragma unroll
-    for (int j = 0; j < TN; ++j)
+      for (int j = 0; j < TN; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gi = m0 + wm0 + i * 16 + lr;
-        const int gj = n0 + wn0 + j * 16 + 4 * r + lk;
-        if (gi < M && gj < N) {
-          const double v0 = alpha * acc[i][j][r];
-          if (Ct) {
-            const int64_t ai = g.ct_off + gi, aj = g.ct_off + gj;
-            const int bi = (int)(ai >> 6), bj = (int)(aj >> 6);
-            if (bi >= bj) {
-              double* ct = Ct + tri_block_offset(bi, bj, g.ct_nbt) + (aj & 63) * 64 + (ai & 63);
-              const double v = (beta == 0.0) ? v0 : v0 + beta * (*ct);
-              *ct = v;
-              if (gj < g.cm_cols) C[gi + (int64_t)gj * ldc] = v;
-            }
-          } else {
-            double* c = C + gi + (int64_t)gj * ldc;
-            *c = (beta == 0.0) ? v0 : v0 + beta * (*c);
-          }
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + wn0 + j * 16 + 4 * r + lk;
+          const bool in = gi < M && gj < N && bi >= ((ct_off + gj) >> 6);
+          mask |= in ? (1u << (j * 4 + r)) : 0u;
+          v[j][r] = (in && readc) ? ctrow[col_part(gj)] : 0.0;
         }
-      }
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v[j][r] = alpha * acc[i][j][r] + beta * v[j][r];
+          // keep the update here, ahead of the conditional stores: sunk into
+          // them, each would wait on every memory operation before it
+          asm volatile("" ::"v"(v[j][r]));
+        }
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (!(mask & (1u << (j * 4 + r)))) continue;
+          const int gj = n0 + wn0 + j * 16 + 4 * r + lk;
+          ctrow[col_part(gj)] = v[j][r];
+          if (gj < cm_cols) C[gi + (int64_t)gj * ldc] = v[j][r];
+        }
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      const int gi = m0 + wm0 + i * 16 + lr;
+      double* __restrict__ crow = C + gi;
+      double v[TN][4];
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + wn0 + j * 16 + 4 * r + lk;
+          v[j][r] = (readc && gi < M && gj < N) ? crow[(int64_t)gj * ldc] : 0.0;
+        }
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          v[j][r] = alpha * acc[i][j][r] + beta * v[j][r];
+          // keep the update here, ahead of the conditional stores: sunk into
+          // them, each would wait on every memory operation before it
+          asm volatile("" ::"v"(v[j][r]));
+        }
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gj = n0 + wn0 + j * 16 + 4 * r + lk;
+          if (gi < M && gj < N) crow[(int64_t)gj * ldc] = v[j][r];
+        }
+    }
+  }
 }
 
 template <int BM, int BN, bool GROUPED>

@@ -52,22 +52,6 @@ __device__ __forceinline__ double dlapy2(double x, double y) {
   return w * sqrt(1.0 + q * q);
 }
 
-// Fixed-order sum of part[0, np) over the whole workgroup (every thread gets
-// it); four independent accumulators per thread keep four loads in flight.
-__device__ __forceinline__ double sum_parts(const double* __restrict__ part, int np, double* red) {
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  const int st = blockDim.x;
-  int q = threadIdx.x;
-  for (; q + 3 * st < np; q += 4 * st) {
-    a0 += part[q];
-    a1 += part[q + st];
-    a2 += part[q + 2 * st];
-    a3 += part[q + 3 * st];
-  }
-  for (; q < np; q += st) a0 += part[q];
-  return block_sum((a0 + a1) + (a2 + a3), red);
-}
-
 // First column c of a panel (already up to date): per-workgroup partials of
 // its squared norm over rows [c+2, n), read by sy_mid.
 __global__ __launch_bounds__(RB) void sy_colnorm(const SyTile* __restrict__ tiles, int c) {
@@ -253,23 +237,70 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
 // W(c+1, jj).  nnorm / nq: partial counts of the norm and of x^T u.
 constexpr int MID_THREADS = 1024;
 
+// Every load is issued up front: the three grid-wide partial sums (column
+// norm, x^T u, row c+1 of u) in one pass with one block reduction, the chunk
+// sums of the dot products, and V(c+1,:), W(c+1,:); one wavefront then forms
+// the jj-term sums.
 __global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__ tiles, int k,
                                                       int c, int jj, int nnorm, int nq) {
-  __shared__ double red[MID_THREADS / 64];
+  constexpr int NW = MID_THREADS / 64;
+  __shared__ double red[3][NW];
   __shared__ double ss[2 * 64];
   const SyTile t = tiles[blockIdx.z];
   const int64_t n = t.n;
   if (c >= n - 1) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t c1 = c + 1;
-  const double xn2 = sum_parts(t.part, nnorm, red);
-  const double xu = sum_parts(t.qpart, nq, red);
   const int nbl = (int)((n + SB - 1) / SB) - (int)(c1 / SB);
   const int nseg0 = (nbl + SEG - 1) / SEG;
-  double u0p = 0.0;
-  for (int sg = tid; sg < nseg0; sg += blockDim.x) u0p += t.ycpart[(int64_t)sg * n + c1];
-  const double u0 = t.ypart[c1] + block_sum(u0p, red);
+  // phase 1: loads
+  double pn = 0.0, pu = 0.0;
+  for (int q = tid; q < nnorm; q += MID_THREADS) pn += t.part[q];
+  double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+  {
+    int q = tid;
+    for (; q + 3 * MID_THREADS < nq; q += 4 * MID_THREADS) {
+      q0 += t.qpart[q];
+      q1 += t.qpart[q + MID_THREADS];
+      q2 += t.qpart[q + 2 * MID_THREADS];
+      q3 += t.qpart[q + 3 * MID_THREADS];
+    }
+    for (; q < nq; q += MID_THREADS) q0 += t.qpart[q];
+  }
+  for (int sg = tid; sg < nseg0; sg += MID_THREADS) pu += t.ycpart[(int64_t)sg * n + c1];
+  double sig = 0.0, xc1 = 0.0;
+  if (tid < 2 * jj) {
+    const int nch = (int)((n - c1 + DCH - 1) / DCH);
+    for (int ch = 0; ch < nch; ++ch) sig += t.dpart[tid * kDotChunks + ch];
+    const double* X = tid < jj ? t.A + (int64_t)(k + tid) * n : t.W + (int64_t)(tid - jj) * n;
+    xc1 = X[c1];
+  }
+  double vc = 0.0, wc = 0.0;  // V(c+1, p), W(c+1, p) for p = tid < jj
+  if (tid < jj) {
+    vc = t.A[c1 + (int64_t)(k + tid) * n];
+    wc = t.W[c1 + (int64_t)tid * n];
+  }
   const double alpha_r = t.A[c1 + c * n];
+  const double acc = t.A[c + c * n];
+  const double g0 = t.At[tri_block_offset((int)(c1 >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 65];
+  const double y00 = t.ypart[c1];
+  // one block reduction for the three sums
+  {
+    const double a = wave_sum(pn), b = wave_sum((q0 + q1) + (q2 + q3)), d = wave_sum(pu);
+    if (lane == 0) {
+      red[0][w] = a;
+      red[1][w] = b;
+      red[2][w] = d;
+    }
+  }
+  __syncthreads();
+  double xn2 = 0.0, xu = 0.0, u0 = 0.0;
+  for (int v = 0; v < NW; ++v) {
+    xn2 += red[0][v];
+    xu += red[1][v];
+    u0 += red[2][v];
+  }
+  u0 += y00;
   const double xnorm = sqrt(xn2);
   double tau, beta, scal;
   if (xnorm == 0.0) {
@@ -281,34 +312,26 @@ __global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__
     tau = (beta - alpha_r) / beta;
     scal = 1.0 / (alpha_r - beta);
   }
-  if (tid < 2 * jj) {
-    const int nch = (int)((n - c1 + DCH - 1) / DCH);
-    double sig = 0.0;
-    for (int ch = 0; ch < nch; ++ch) sig += t.dpart[tid * kDotChunks + ch];
-    const double* X = tid < jj ? t.A + (int64_t)(k + tid) * n : t.W + (int64_t)(tid - jj) * n;
-    ss[tid] = scal * (sig - beta * X[c1]);
-  }
+  if (tid < 2 * jj) ss[tid] = scal * (sig - beta * xc1);
   __syncthreads();
   if (tid < 2 * jj) t.s[tid] = ss[tid];
-  if (tid == 0) {
-    const double g0 =
-        t.At[tri_block_offset((int)(c1 >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 65];
-    const double vAv = scal * scal * (xu - 2.0 * beta * u0 + beta * beta * g0);
-    double svw = 0.0, corr = 0.0;
-    for (int p = 0; p < jj; ++p) {
-      svw += ss[p] * ss[jj + p];
-      corr += t.A[c1 + (int64_t)(k + p) * n] * ss[jj + p] + t.W[c1 + (int64_t)p * n] * ss[p];
+  if (w == 0) {  // jj <= 63: one lane per panel column
+    const double sv = lane < jj ? ss[lane] * ss[jj + lane] : 0.0;
+    const double cr = lane < jj ? vc * ss[jj + lane] + wc * ss[lane] : 0.0;
+    const double svw = wave_sum(sv), corr = wave_sum(cr);
+    if (lane == 0) {
+      const double vAv = scal * scal * (xu - 2.0 * beta * u0 + beta * beta * g0);
+      const double alpha = -0.5 * tau * tau * (vAv - 2.0 * svw);
+      const double y0 = scal * (u0 - beta * g0);
+      t.scal[0] = scal;
+      t.scal[1] = tau;
+      t.scal[2] = alpha;
+      t.scal[3] = beta;
+      t.scal[4] = tau * (y0 - corr) + alpha;
+      t.d[c] = acc;
+      t.e[c] = beta;
+      t.tau[c] = tau;
     }
-    const double alpha = -0.5 * tau * tau * (vAv - 2.0 * svw);
-    const double y0 = scal * (u0 - beta * g0);
-    t.scal[0] = scal;
-    t.scal[1] = tau;
-    t.scal[2] = alpha;
-    t.scal[3] = beta;
-    t.scal[4] = tau * (y0 - corr) + alpha;
-    t.d[c] = t.A[c + c * n];
-    t.e[c] = beta;
-    t.tau[c] = tau;
   }
 }
 
