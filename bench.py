@@ -228,6 +228,29 @@ def main():
                     "avg_launch_ms": round(avg_ms, 5), "launches": symv_n,
                     "stage_ms": {k: round(v[0], 2) for k, v in st.items()}}
 
+    # the lookup-table set-up on the device (build_Alookup/Blookup from the
+    # tile's spectra) beside the host's numpy set-up; not part of the step
+    tables = None
+    if args.workload != "batch" and tile.Gt_rot is not None:
+        from imcom_amd import tile as TL
+
+        def dev_tables():
+            g.build_Alookup(tile.psfconst, tile.Gt_rot, tile.npad)
+            g.build_Blookup(tile.psfconst, tile.Gammat, tile.Gt_rot, tile.npad)
+            g.synchronize()
+        dev_tables()
+        t1 = time.perf_counter()
+        dev_tables()
+        t_dev = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        TL.build_Alookup(tile.Gt_rot, tile.npad)
+        TL.build_Blookup(tile.Gammat, tile.Gt_rot, tile.npad)
+        t_host = time.perf_counter() - t1
+        N = tile.npad * tile.Gt_rot[0].shape[0]
+        tables = {"grid": f"{N}x{N}", "n_tables": tile.nexppsf * (tile.nexppsf + 3) // 2,
+                  "device_ms": round(t_dev * 1e3, 2), "host_numpy_ms": round(t_host * 1e3, 2)}
+        log(f"table set-up: device {t_dev * 1e3:.1f} ms, host numpy {t_host * 1e3:.1f} ms ({N}^2)")
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -264,6 +287,8 @@ def main():
            "pixels_U_ge_Umax": n_bad, "mean_U_evaluations": round(mean_ev, 2)}
     if pcie is not None:
         cfg["pcie_inclusive_px_per_s"] = round(m_total / pcie, 2)
+    if tables is not None:
+        cfg["table_setup"] = tables
     if plain is not None:
         cfg["plain_fp64_px_per_s"] = round(m_total / plain[0], 2)
         cfg["plain_fp64_ms_per_step"] = round(plain[0] * 1e3, 2)

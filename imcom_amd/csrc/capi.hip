@@ -75,6 +75,7 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
                     &ctx->ms_dY};
   for (DevBuf* b : bufs) b->release();
   ctx->ew.release();
+  ctx->tw.release();
   imcom_batch_free(ctx);
   dc_destroy(ctx->dc);
   ctx->prof.destroy();
@@ -140,13 +141,17 @@ int imcom_gpu_set_tables(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1, 
   ctx->nexppsf = nexppsf;
   ctx->n1 = n1;
   ctx->n2 = n2;
-  ctx->have_tables = true;
+  ctx->have_Atab = Alookup != nullptr;
+  ctx->have_Btab = Blookup != nullptr;
   return IMCOM_OK;
 }
 
-static int check_interp(imcom_gpu_ctx* ctx, int npoly, int npad, double sx, double sy) {
+static int check_interp(imcom_gpu_ctx* ctx, bool table, const char* which, int npoly, int npad,
+                        double sx, double sy) {
   NEED(ctx->have_tile, IMCOM_ESTATE, "tile inputs not set (imcom_gpu_set_tile)");
-  NEED(ctx->have_tables, IMCOM_ESTATE, "lookup tables not set (imcom_gpu_set_tables)");
+  NEED(table, IMCOM_ESTATE,
+       std::string(which) +
+           " lookup table not set (imcom_gpu_set_tables or imcom_gpu_build_" + which + "lookup)");
   NEED(npoly >= 1 && npoly <= 8, IMCOM_EARG, "npoly must be in 1..7 (8 is taken as 7)");
   NEED(npad >= 1, IMCOM_EARG, "npad must be >= 1");
   NEED(sx > 0.0 && sy > 0.0, IMCOM_EARG, "PSF pixel scales must be positive");
@@ -159,10 +164,9 @@ static int check_interp(imcom_gpu_ctx* ctx, int npoly, int npad, double sx, doub
 int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale,
                        double psfyscale, double* A_out) {
   if (!ctx) return IMCOM_EARG;
-  int rc = check_interp(ctx, npoly, npad, psfxscale, psfyscale);
+  int rc = check_interp(ctx, ctx->have_Atab, "A", npoly, npad, psfxscale, psfyscale);
   if (rc) return rc;
   CK(hipSetDevice(ctx->device), "hipSetDevice");
-  NEED(ctx->Atab.p, IMCOM_ESTATE, "A lookup table not set");
   const int64_t n = ctx->n;
   CK(ctx->A.reserve((size_t)n * n * 8), "alloc A");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
@@ -190,10 +194,9 @@ int imcom_gpu_set_A(imcom_gpu_ctx* ctx, const double* A) {
 int imcom_gpu_lookup_B(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale,
                        double psfyscale, double* B_out) {
   if (!ctx) return IMCOM_EARG;
-  int rc = check_interp(ctx, npoly, npad, psfxscale, psfyscale);
+  int rc = check_interp(ctx, ctx->have_Btab, "B", npoly, npad, psfxscale, psfyscale);
   if (rc) return rc;
   CK(hipSetDevice(ctx->device), "hipSetDevice");
-  NEED(ctx->Btab.p, IMCOM_ESTATE, "B lookup table not set");
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->B.reserve((size_t)n * m * 8), "alloc B");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;

@@ -116,6 +116,21 @@ struct EigWork {
   }
 };
 
+// Workspace of the device table set-up (tables.hip): the cached DFT
+// matrices (E_f for the PSF spectra, E_t for the padded table transform,
+// with the (N, n, f0) each was built for), the product spectrum, the
+// half-transformed stack and the uploaded spectra.
+struct TableWork {
+  DevBuf E_f, E_t, Sp, X, img, spec, Y, Sr, Gt, Gam, Cv;
+  int Ef_N = 0, Ef_n = 0, Ef_f0 = -1, Et_N = 0, Et_n = 0, Et_f0 = -1;
+  void release() {
+    DevBuf* b[] = {&E_f, &E_t, &Sp, &X, &img, &spec, &Y, &Sr, &Gt, &Gam, &Cv};
+    for (DevBuf* p : b) p->release();
+    Ef_N = Ef_n = Et_N = Et_n = 0;
+    Ef_f0 = Et_f0 = -1;
+  }
+};
+
 struct imcom_gpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -126,7 +141,8 @@ struct imcom_gpu_ctx {
   std::vector<int32_t> h_exp;
   // tables
   int psfconst = 1, nexppsf = 1, n1 = 0, n2 = 0;
-  bool have_tables = false;
+  bool have_Atab = false, have_Btab = false;  // set_tables or build_Alookup / build_Blookup
+  TableWork tw;
   // stage flags
   bool have_tile = false, have_A = false, have_B = false, have_QL = false, have_P = false,
        have_K = false, have_T = false;

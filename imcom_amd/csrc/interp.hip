@@ -183,4 +183,81 @@ hipError_t launch_lookup_B(hipStream_t st, int dev, int64_t n, int64_t m, const 
   return hipGetLastError();
 }
 
+// imcom_rotate_image (imcom_proc.f90:184-250): output pixel (iout, jout),
+// 1-based, takes the polynomial through the clipped stencil around the
+// rotated position (xs, ys), or 0 when the position's floor/ceiling leave the
+// image.  cost/sint come from the host; the position is formed with the
+// reference's operation order and no fused multiply-adds.
+__global__ __launch_bounds__(256) void rotate_kernel(const double* __restrict__ in, int n1, int n2,
+                                                     double cost, double sint, int npm,
+                                                     double* __restrict__ out) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n1 * n2) return;
+  const int iout = (int)(idx % n1) + 1, jout = (int)(idx / n1) + 1;
+  const double xc = (double)(n1 + 1) / 2.0, yc = (double)(n2 + 1) / 2.0;
+  const double xo = __dsub_rn((double)iout, xc), yo = __dsub_rn((double)jout, yc);
+  const double xs = __dadd_rn(__dadd_rn(__dmul_rn(xo, cost), __dmul_rn(yo, sint)), xc);
+  const double ys = __dadd_rn(__dadd_rn(__dmul_rn(-xo, sint), __dmul_rn(yo, cost)), yc);
+  const double fi = floor(xs), fj = floor(ys), ci = ceil(xs), cj = ceil(ys);
+  double v = 0.0;
+  if (fi >= 1.0 && fj >= 1.0 && ci <= (double)n1 && cj <= (double)n2) {
+    const int imin = max(1, (int)fi - npm), jmin = max(1, (int)fj - npm);
+    const int imax = min(n1, (int)ci + npm), jmax = min(n2, (int)cj + npm);
+    const int cx = imax - imin + 1, cy = jmax - jmin + 1;
+    double wx[8], wy[8];
+    lagrange8(xs, imin, cx, wx);
+    lagrange8(ys, jmin, cy, wy);
+    const double* base = in + (imin - 1) + (int64_t)(jmin - 1) * n1;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const double* col = base + (int64_t)min(l, cy - 1) * n1;
+      double r = 0.0;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) r += wx[k] * col[min(k, cx - 1)];
+      v += wy[l] * r;
+    }
+  }
+  out[idx] = v;
+}
+
+// sums[0] = sum(in), sums[1] = sum(out): one workgroup, fixed order
+__global__ __launch_bounds__(1024) void rotate_sums_kernel(const double* __restrict__ in,
+                                                           const double* __restrict__ out,
+                                                           int64_t cnt, double* __restrict__ sums) {
+  __shared__ double red[16];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = threadIdx.x; i < cnt; i += 1024) {
+    a += in[i];
+    b += out[i];
+  }
+  const double sa = block_sum(a, red);
+  const double sb = block_sum(b, red);
+  if (threadIdx.x == 0) {
+    sums[0] = sa;
+    sums[1] = sb;
+  }
+}
+
+// im_out = sum(im_in) * im_out / sum(im_out)
+__global__ __launch_bounds__(256) void rotate_flux_kernel(double* __restrict__ out, int64_t cnt,
+                                                          const double* __restrict__ sums) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < cnt) out[i] = sums[0] * out[i] / sums[1];
+}
+
+hipError_t launch_rotate(hipStream_t st, int dev, const double* in, int n1, int n2, double cost,
+                         double sint, int npm, int conserve_flux, double* out, double* sums) {
+  IMCOM_HIP_TRY(ensure_invden(dev));
+  const int64_t cnt = (int64_t)n1 * n2;
+  const unsigned nb = (unsigned)((cnt + 255) / 256);
+  hipLaunchKernelGGL(rotate_kernel, dim3(nb), dim3(256), 0, st, in, n1, n2, cost, sint, npm, out);
+  IMCOM_HIP_TRY(hipGetLastError());
+  if (conserve_flux) {
+    hipLaunchKernelGGL(rotate_sums_kernel, dim3(1), dim3(1024), 0, st, in, out, cnt, sums);
+    IMCOM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(rotate_flux_kernel, dim3(nb), dim3(256), 0, st, out, cnt, sums);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace imcom

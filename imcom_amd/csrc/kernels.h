@@ -25,6 +25,7 @@ enum ProfSlot {
   PS_MAPS,
   PS_SYMV,      // sy_symv launches only (level 2)
   PS_SYR2K,     // trailing-update GEMMs only (level 2)
+  PS_TABLES,    // device lookup-table set-up (build_Alookup / build_Blookup)
   PS_NSLOTS
 };
 
@@ -119,6 +120,9 @@ size_t gemm_split_work_bytes(int scheme, int M, int N, int K);
 
 // ---- (a) interpolation ----------------------------------------------------
 hipError_t interp_init(int dev);
+// imcom_rotate_image on the device; sums: 2 doubles of device scratch
+hipError_t launch_rotate(hipStream_t st, int dev, const double* in, int n1, int n2, double cost,
+                         double sint, int npm, int conserve_flux, double* out, double* sums);
 hipError_t launch_lookup_A(hipStream_t st, int dev, int64_t n, const double* x, const double* y,
                            const int32_t* ex, int psfconst, const double* tabs, int64_t tstride,
                            int n1, int n2, int npm, int npad, double sx, double sy, double* A);

@@ -44,6 +44,49 @@ MODULE imcom_gpu_iface
       integer(c_int), value :: psfconst, nexppsf, n1, n2
       real(c_double), intent(in) :: Alookup(*), Blookup(*)
     end function
+    ! device lookup-table set-up: spectra are complex(c_double_complex),
+    ! whose storage is the interleaved (re, im) layout the C side reads
+    integer(c_int) function imcom_gpu_rotate_image(ctx, n1, n2, im_in, theta_deg, npoly, &
+        conserve_flux, im_out) bind(C, name="imcom_gpu_rotate_image")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n1, n2, npoly, conserve_flux
+      real(c_double), intent(in) :: im_in(*)
+      real(c_double), value :: theta_deg
+      real(c_double), intent(out) :: im_out(*)
+    end function
+    integer(c_int) function imcom_gpu_ft_r2c(ctx, n1, n2, count, img, spec_out) &
+        bind(C, name="imcom_gpu_ft_r2c")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n1, n2, count
+      real(c_double), intent(in) :: img(*)
+      complex(c_double_complex), intent(out) :: spec_out(*)
+    end function
+    integer(c_int) function imcom_gpu_build_Alookup(ctx, psfconst, nexppsf, n1psf, n2psf, npad, &
+        Gt_rot, Alookup_out) bind(C, name="imcom_gpu_build_Alookup")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: psfconst, nexppsf, n1psf, n2psf, npad
+      complex(c_double_complex), intent(in) :: Gt_rot(*)
+      real(c_double), intent(out) :: Alookup_out(*)
+    end function
+    integer(c_int) function imcom_gpu_build_Blookup(ctx, psfconst, nexppsf, n1psf, n2psf, npad, &
+        Gammat, Gt_rot, Blookup_out) bind(C, name="imcom_gpu_build_Blookup")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: psfconst, nexppsf, n1psf, n2psf, npad
+      complex(c_double_complex), intent(in) :: Gammat(*), Gt_rot(*)
+      real(c_double), intent(out) :: Blookup_out(*)
+    end function
+    integer(c_int) function imcom_gpu_calc_C(ctx, n1psf, n2psf, Gammat, C_out) &
+        bind(C, name="imcom_gpu_calc_C")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n1psf, n2psf
+      complex(c_double_complex), intent(in) :: Gammat(*)
+      real(c_double), intent(out) :: C_out
+    end function
     integer(c_int) function imcom_gpu_lookup_A(ctx, npoly, npad, psfxscale, psfyscale, A_out) &
         bind(C, name="imcom_gpu_lookup_A")
       import :: c_int, c_ptr, c_double

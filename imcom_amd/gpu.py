@@ -26,7 +26,8 @@ IMCOM_PREC_BF16X6 = 2
 # every symbol declared in include/imcom_gpu.h
 EXPORTS = [
     "imcom_gpu_init", "imcom_gpu_finalize", "imcom_gpu_last_error", "imcom_gpu_synchronize",
-    "imcom_gpu_set_tile", "imcom_gpu_set_tables", "imcom_gpu_lookup_A", "imcom_gpu_set_A",
+    "imcom_gpu_set_tile", "imcom_gpu_set_tables", "imcom_gpu_rotate_image", "imcom_gpu_ft_r2c", "imcom_gpu_build_Alookup",
+    "imcom_gpu_build_Blookup", "imcom_gpu_calc_C", "imcom_gpu_lookup_A", "imcom_gpu_set_A",
     "imcom_gpu_eigen_A", "imcom_gpu_tridiagonalize", "imcom_gpu_tridiag_eig",
     "imcom_gpu_set_QL", "imcom_gpu_lookup_B", "imcom_gpu_set_B", "imcom_gpu_build_P",
     "imcom_gpu_set_P", "imcom_gpu_bisect_kappa", "imcom_gpu_build_T", "imcom_gpu_build_S",
@@ -56,7 +57,7 @@ class TileDesc(ctypes.Structure):
                 ("psfxscale", ctypes.c_double), ("psfyscale", ctypes.c_double)]
 
 STAGE_SLOTS = ["lookup_A", "sytrd", "stedc", "ormtr", "lookup_B", "build_P", "bisect",
-               "build_T", "maps", "symv", "syr2k"]
+               "build_T", "maps", "symv", "syr2k", "tables"]
 
 
 class ImcomError(RuntimeError):
@@ -121,6 +122,11 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_batch_solve": (_int, [_vp, _int, _int, _int, ctypes.c_float, _int]),
         "imcom_gpu_batch_result": (_int, [_vp, _int, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
         "imcom_gpu_set_tuning": (_int, [_vp, _int, _int]),
+        "imcom_gpu_rotate_image": (_int, [_vp, _int, _int, _dp, _dbl, _int, _int, _dp]),
+        "imcom_gpu_ft_r2c": (_int, [_vp, _int, _int, _int, _dp, _dp]),
+        "imcom_gpu_build_Alookup": (_int, [_vp, _int, _int, _int, _int, _int, _dp, _dp]),
+        "imcom_gpu_build_Blookup": (_int, [_vp, _int, _int, _int, _int, _int, _dp, _dp, _dp]),
+        "imcom_gpu_calc_C": (_int, [_vp, _int, _int, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -177,18 +183,82 @@ class ImcomGPU:
             raise ImcomError(f"IMCOM ERROR: {what}: {msg} (code {rc})")
 
     # ---- inputs ---------------------------------------------------------
-    def set_tile(self, tile):
-        """Hand over the imcom_data globals of one tile."""
+    def set_tile(self, tile, tables="host"):
+        """Hand over the imcom_data globals of one tile.  tables="host"
+        uploads the tile's Alookup/Blookup; "device" builds them on the GPU
+        from the tile's spectra Gt_rot and Gammat (build_Alookup/Blookup)."""
         self.n, self.m = tile.n, tile.m
         self._keep = [_vec(tile.x_i), _vec(tile.y_i), _vec(tile.exp_i, np.int32),
                       _vec(tile.X_a), _vec(tile.Y_a), _vec(tile.Ndiag)]
         x, y, e, X, Y, N = self._keep
         self._ck(self.lib.imcom_gpu_set_tile(self.h, tile.n, tile.m, _ptr(x), _ptr(y), _ptr(e, _ip),
                                              _ptr(X), _ptr(Y), _ptr(N), tile.C_a), "set_tile")
-        A, B = _mat(tile.Alookup), _mat(tile.Blookup)
-        self._ck(self.lib.imcom_gpu_set_tables(self.h, tile.psfconst, tile.nexppsf, A.shape[0],
-                                               A.shape[1], _ptr(A), _ptr(B)), "set_tables")
+        if tables == "device":
+            if tile.Gt_rot is None or tile.Gammat is None:
+                raise ImcomError("set_tile(tables='device') needs the tile's spectra")
+            self.build_Alookup(tile.psfconst, tile.Gt_rot, tile.npad)
+            self.build_Blookup(tile.psfconst, tile.Gammat, tile.Gt_rot, tile.npad)
+        elif tables == "host":
+            A, B = _mat(tile.Alookup), _mat(tile.Blookup)
+            self._ck(self.lib.imcom_gpu_set_tables(self.h, tile.psfconst, tile.nexppsf, A.shape[0],
+                                                   A.shape[1], _ptr(A), _ptr(B)), "set_tables")
+        else:
+            raise ValueError("tables must be 'host' or 'device'")
         self.tile = tile
+
+    # ---- lookup-table set-up (imcom_proc.f90:184-250,433-475, imcom_matrix.f90:361-552)
+    def rotate_image(self, im, theta_deg, npoly=7, conserve_flux=1):
+        """imcom_rotate_image on the device."""
+        a = np.asfortranarray(im, dtype=np.float64)
+        out = np.empty(a.shape, order="F")
+        self._ck(self.lib.imcom_gpu_rotate_image(self.h, a.shape[0], a.shape[1], _ptr(a),
+                                                 float(theta_deg), npoly, conserve_flux,
+                                                 _ptr(out)), "rotate_image")
+        return out
+
+    def ft_r2c(self, imgs):
+        """Full complex DFT of real square images (n x n or n x n x k,
+        Fortran order) as imcom_ft_r2c returns it; complex128 of the same shape."""
+        a = np.asfortranarray(imgs, dtype=np.float64)
+        k = 1 if a.ndim == 2 else a.shape[2]
+        out = np.empty(a.shape, dtype=np.complex128, order="F")
+        self._ck(self.lib.imcom_gpu_ft_r2c(self.h, a.shape[0], a.shape[1], k, _ptr(a),
+                                           out.ctypes.data_as(_dp)), "ft_r2c")
+        return out
+
+    @staticmethod
+    def _spectra(Gt_rot):
+        Gs = [np.asarray(g) for g in Gt_rot] if isinstance(Gt_rot, (list, tuple)) else \
+            [Gt_rot[:, :, i] for i in range(Gt_rot.shape[2])] if np.ndim(Gt_rot) == 3 else [Gt_rot]
+        return np.asfortranarray(np.stack(Gs, axis=2).astype(np.complex128)), len(Gs)
+
+    def build_Alookup(self, psfconst, Gt_rot, npad=3, out=False):
+        """imcom_build_Alookup on the device; the tables stay there for lookup_A."""
+        S, k = self._spectra(Gt_rot)
+        n = S.shape[0]
+        A = np.empty((n * npad, n * npad, k * (k + 1) // 2), order="F") if out else None
+        self._ck(self.lib.imcom_gpu_build_Alookup(self.h, psfconst, k, n, S.shape[1], npad,
+                                                  S.ctypes.data_as(_dp), _ptr(A)), "build_Alookup")
+        return A
+
+    def build_Blookup(self, psfconst, Gammat, Gt_rot, npad=3, out=False):
+        """imcom_build_Blookup on the device; the tables stay there for lookup_B."""
+        S, k = self._spectra(Gt_rot)
+        G = np.asfortranarray(Gammat, dtype=np.complex128)
+        n = S.shape[0]
+        B = np.empty((n * npad, n * npad, k), order="F") if out else None
+        self._ck(self.lib.imcom_gpu_build_Blookup(self.h, psfconst, k, n, S.shape[1], npad,
+                                                  G.ctypes.data_as(_dp), S.ctypes.data_as(_dp),
+                                                  _ptr(B)), "build_Blookup")
+        return B
+
+    def calc_C(self, Gammat):
+        """imcom_calc_C on the device."""
+        G = np.asfortranarray(Gammat, dtype=np.complex128)
+        C = np.zeros(1)
+        self._ck(self.lib.imcom_gpu_calc_C(self.h, G.shape[0], G.shape[1], G.ctypes.data_as(_dp),
+                                           _ptr(C)), "calc_C")
+        return float(C[0])
 
     # ---- imcom_matrix ---------------------------------------------------
     def lookup_A(self, npoly=7, npad=3, out=False):

@@ -76,7 +76,7 @@ def _gauss_source(x, y, xc, yc, sigma, flux=1.0):
 def make_tile(*, nexp, npix_in, psf_kind="gauss", psf_sigma=1.2, gamma_sigma=1.5,
               psf_npix=128, psfscale=0.25, nout, outscale=0.5, noise_sigma=1e-3,
               US_max=1e-6, US_tol=1e-9, USB=True, seed=0, lam_over_d=0.5,
-              obscuration=0.3, name="tile") -> Tile:
+              obscuration=0.3, name="tile", host_tables=True) -> Tile:
     rng = np.random.default_rng(seed)
     dither = rng.uniform(0.0, 1.0, size=(nexp, 2))
     rot = 360.0 * rng.uniform(0.0, 1.0, size=nexp)
@@ -106,7 +106,8 @@ def make_tile(*, nexp, npix_in, psf_kind="gauss", psf_sigma=1.2, gamma_sigma=1.5
         noise=np.full(nexp, noise_sigma ** 2), n1out=nout, n2out=nout, outxscale=outscale,
         outyscale=outscale, outpos1=xc - half, outpos2=yc - half, USB=USB, US_max=US_max,
         US_tol=US_tol, meta=dict(name=name, seed=seed, psf_kind=psf_kind, psf_sigma=psf_sigma,
-                                 gamma_sigma=gamma_sigma, noise_sigma=noise_sigma))
+                                 gamma_sigma=gamma_sigma, noise_sigma=noise_sigma),
+        host_tables=host_tables)
 
 
 def small(seed=20261204, **kw) -> Tile:

@@ -270,16 +270,20 @@ class Tile:
     US_tol: float = 1e-9
     maxNbis: int = 53
     meta: dict = dataclasses.field(default_factory=dict)
+    Gt_rot: list = None        # PSF spectra (n1pad^2 complex, nexppsf of them)
+    Gammat: np.ndarray = None  # target spectrum
 
 
 def assemble_tile(*, I_unmasked, gimxscale, gimyscale, dither, psf_img, rotangdeg,
                   gamma_img, psfxscale, psfyscale, noise, n1out, n2out, outxscale,
                   outyscale, outpos1, outpos2, psffile=None, npad=3, npoly=7,
                   saturation=np.finfo(np.float64).max, USB=True, US_max=1e-6,
-                  US_tol=1e-9, meta=None) -> Tile:
+                  US_tol=1e-9, meta=None, host_tables=True) -> Tile:
     """Run the reference's CPU set-up (imcom.f90:55-127) for USERXY=0 inputs:
     masking, coordinates, psfconst, padding, rotation, spectra, lookup
-    tables, Ndiag, C and the kappa range."""
+    tables, Ndiag, C and the kappa range.  host_tables=False leaves the
+    lookup tables to the device (ImcomGPU.set_tile(tile, tables="device"));
+    the spectra are kept on the tile either way."""
     n1gim, n2gim, nexp = I_unmasked.shape
     mask_subs, I_i, exp_i = mask_pixels(I_unmasked, saturation)
     x_i, y_i = build_xy(n1gim, n2gim, gimxscale, gimyscale, dither, mask_subs)
@@ -303,7 +307,9 @@ def assemble_tile(*, I_unmasked, gimxscale, gimyscale, dither, psf_img, rotangde
     return Tile(n=n, m=X_a.shape[0], x_i=x_i, y_i=y_i, exp_i=exp_i.astype(np.int32),
                 X_a=X_a, Y_a=Y_a, I_i=I_i, Ndiag=Ndiag, C_a=C_a,
                 kappa_min=C_a * EPS, kappa_max=C_a / EPS,
-                Alookup=build_Alookup(Gt, npad), Blookup=build_Blookup(Gammat, Gt, npad),
+                Alookup=build_Alookup(Gt, npad) if host_tables else None,
+                Blookup=build_Blookup(Gammat, Gt, npad) if host_tables else None,
                 psfconst=psfconst, nexppsf=nexppsf, npad=npad, npoly=npoly,
                 psfxscale=psfxscale, psfyscale=psfyscale, n1out=n1out, n2out=n2out,
-                USB=USB, US_max=US_max, US_tol=US_tol, meta=dict(meta or {}, n1pad=n1pad))
+                USB=USB, US_max=US_max, US_tol=US_tol, meta=dict(meta or {}, n1pad=n1pad),
+                Gt_rot=Gt, Gammat=Gammat)

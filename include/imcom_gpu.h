@@ -62,6 +62,38 @@ int imcom_gpu_set_tile(imcom_gpu_ctx* ctx, int64_t n, int64_t m, const double* x
 int imcom_gpu_set_tables(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1, int n2,
                          const double* Alookup, const double* Blookup);
 
+/* Device set-up of the lookup tables (the CPU steps of imcom.f90:99-116 and
+ * imcom_build_Alookup / imcom_build_Blookup).  Spectra are Fortran complex(8)
+ * arrays (interleaved re, im), n1psf x n2psf each, stacked over exposures;
+ * the PSF grid is square with an even side, as the reference pads it
+ * (imcom_io.f90:279-286).  The tables stay on the device for
+ * imcom_gpu_lookup_A / _B (in place of imcom_gpu_set_tables); the host copies
+ * are optional. */
+
+/* imcom_rotate_image(im_in, theta_deg, npoly_in, conserve_flux, im_out)
+ * (imcom_proc.f90:184-250): im_in, im_out n1 x n2; npoly <= 7 (8 is taken
+ * as 7, as the reference does with a warning). */
+int imcom_gpu_rotate_image(imcom_gpu_ctx* ctx, int n1, int n2, const double* im_in,
+                           double theta_deg, int npoly, int conserve_flux, double* im_out);
+/* imcom_ft_r2c(n1, n2, plan, F, Ft)  (imcom_proc.f90:433-475): the full
+ * complex DFT of count real n1 x n2 images (count images stacked), with the
+ * reference's Hermitian completion of the upper half of the first axis. */
+int imcom_gpu_ft_r2c(imcom_gpu_ctx* ctx, int n1, int n2, int count, const double* img,
+                     double* spec_out);
+/* imcom_build_Alookup(npad)  (imcom_matrix.f90:361-430): Gt_rot holds
+ * nexppsf spectra; Alookup_out (npad n1psf, npad n2psf,
+ * nexppsf (nexppsf + 1) / 2) may be NULL. */
+int imcom_gpu_build_Alookup(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1psf, int n2psf,
+                            int npad, const double* Gt_rot, double* Alookup_out);
+/* imcom_build_Blookup(npad)  (imcom_matrix.f90:434-493): Blookup_out
+ * (npad n1psf, npad n2psf, nexppsf) may be NULL. */
+int imcom_gpu_build_Blookup(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1psf, int n2psf,
+                            int npad, const double* Gammat, const double* Gt_rot,
+                            double* Blookup_out);
+/* imcom_calc_C  (imcom_matrix.f90:546-552): C = sum |Gammat|^2 / (n1psf n2psf). */
+int imcom_gpu_calc_C(imcom_gpu_ctx* ctx, int n1psf, int n2psf, const double* Gammat,
+                     double* C_out);
+
 /* imcom_lookup_A(npoly, npad, saveA)  (imcom_matrix.f90:206-284).
  * A_out (n*n) may be NULL.  npoly <= 7 (stencil of at most 8 points). */
 int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale,
@@ -185,7 +217,8 @@ int imcom_gpu_batch_result(imcom_gpu_ctx* ctx, int tile, double* K_out, double* 
  * imcom_gpu_stage_times fills ms_out[i] / counts_out[i] for the slots
  *   0 lookup_A, 1 tridiagonal reduction, 2 divide-and-conquer,
  *   3 back-transform, 4 lookup_B, 5 build_P, 6 bisect_kappa, 7 build_T,
- *   8 maps (S, U, H), 9 matrix-vector launches, 10 trailing updates. */
+ *   8 maps (S, U, H), 9 matrix-vector launches, 10 trailing updates,
+ *   11 device lookup-table set-up. */
 int imcom_gpu_set_profiling(imcom_gpu_ctx* ctx, int level);
 int imcom_gpu_stage_times(imcom_gpu_ctx* ctx, double* ms_out, int64_t* counts_out, int nslots);
 

@@ -117,6 +117,54 @@ double or_interp_lookup(int n1, int n2, const double *lookup, double xarg, doubl
   return 0.0;
 }
 
+/* imcom_rotate_image (imcom_proc.f90:184-250): rotation about
+ * ((n1+1)/2, (n2+1)/2) by Neville interpolation on the clipped stencil;
+ * cost, sint as the caller computed them from mod(theta, 360) * pi / 180. */
+void or_rotate_image(int n1, int n2, const double *im_in, double cost, double sint, int npoly,
+                     int conserve_flux, double *im_out) {
+  int npm, iout, jout, is_f, js_f, is_c, js_c, imin, jmin, imax, jmax, ii, jj;
+  double x_cent, y_cent, xout, yout, xs, ys, x1a[16], x2a[16], s_in, s_out;
+  int64_t k;
+  if (npoly % 2 == 0) npoly -= 1;
+  npm = npoly / 2;
+  x_cent = (double)(n1 + 1) / 2.0;
+  y_cent = (double)(n2 + 1) / 2.0;
+  for (iout = 1; iout <= n1; ++iout) {
+    xout = (double)iout - x_cent;
+    for (jout = 1; jout <= n2; ++jout) {
+      double *o = im_out + (iout - 1) + (int64_t)(jout - 1) * n1;
+      yout = (double)jout - y_cent;
+      xs = xout * cost + yout * sint + x_cent;
+      ys = -xout * sint + yout * cost + y_cent;
+      is_f = (int)floor(xs);
+      js_f = (int)floor(ys);
+      is_c = (int)ceil(xs);
+      js_c = (int)ceil(ys);
+      if (is_f >= 1 && js_f >= 1 && is_c <= n1 && js_c <= n2) {
+        imin = is_f - npm > 1 ? is_f - npm : 1;
+        jmin = js_f - npm > 1 ? js_f - npm : 1;
+        imax = is_c + npm < n1 ? is_c + npm : n1;
+        jmax = js_c + npm < n2 ? js_c + npm : n2;
+        for (ii = imin; ii <= imax; ++ii) x1a[ii - imin] = (double)ii;
+        for (jj = jmin; jj <= jmax; ++jj) x2a[jj - jmin] = (double)jj;
+        *o = or_polin2(x1a, imax - imin + 1, x2a, jmax - jmin + 1,
+                       im_in + (imin - 1) + (int64_t)(jmin - 1) * n1, n1, xs, ys);
+      } else {
+        *o = 0.0;
+      }
+    }
+  }
+  if (conserve_flux != 0) {
+    s_in = 0.0;
+    s_out = 0.0;
+    for (k = 0; k < (int64_t)n1 * n2; ++k) {
+      s_in += im_in[k];
+      s_out += im_out[k];
+    }
+    for (k = 0; k < (int64_t)n1 * n2; ++k) im_out[k] = s_in * im_out[k] / s_out;
+  }
+}
+
 /* imcom_matrix.f90:206-284.  Alookup is (n1al, n2al, nA); A is n x n. */
 void or_lookup_A(int64_t n, const double *x_i, const double *y_i, const int32_t *exp_i,
                  int psfconst, int n1al, int n2al, const double *Alookup, int npad, int npoly,
