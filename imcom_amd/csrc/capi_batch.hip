@@ -131,7 +131,7 @@ int imcom_gpu_batch_set(imcom_gpu_ctx* ctx, int ntiles, const imcom_gpu_tile* ti
     CK(t->feat.reserve((size_t)m * 32 * 4), "alloc features");
     CK(t->bracket.reserve((size_t)m * 2 * 4), "alloc bracket");
     CK(t->bins.reserve(n), "alloc bins");
-    CK(t->ew.reserve(n, B->nb, B->nbb, t->A.as<double>(), t->Q.as<double>(), t->sy),
+    CK(t->ew.reserve(n, B->nb, B->nbb, t->A.as<double>(), t->Q.as<double>(), t->sy, false),
        "alloc eig");
     host.push_back(t->sy);
     B->nmax = std::max(B->nmax, n);

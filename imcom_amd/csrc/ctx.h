@@ -4,6 +4,7 @@
 #include "common.h"
 #include "kernels.h"
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -51,8 +52,11 @@ using imcom_ctx_detail::kNBB;
 // back-transform block nbb.
 struct EigWork {
   DevBuf sW, sVW, sWV, sy, sycp, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb,
-      sTb, sY, sY2, slam, sZ, stile, sdp, sAt;
-  hipError_t reserve(int64_t n, int nb, int nbb, double* A, double* Z, SyTile& t) {
+      sTb, sY, sY2, slam, sZ, stile, sdp, sAt, sKs;
+  // splitk: workspace for the single-tile back-transform's split-K products
+  // (the batched back-transform uses grouped GEMMs instead)
+  hipError_t reserve(int64_t n, int nb, int nbb, double* A, double* Z, SyTile& t,
+                     bool splitk = true) {
     const size_t d = 8;
     IMCOM_HIP_TRY(sW.reserve(n * nb * d));
     IMCOM_HIP_TRY(sVW.reserve(n * 2 * nb * d));
@@ -80,6 +84,9 @@ struct EigWork {
     IMCOM_HIP_TRY(stile.reserve(sizeof(SyTile)));
     const int nbt = (int)((n + 63) / 64);
     IMCOM_HIP_TRY(sAt.reserve((size_t)nbt * (nbt + 1) / 2 * 4096 * d));
+    if (splitk)
+      IMCOM_HIP_TRY(sKs.reserve(std::max(dgemm_splitk_work(nbb, nbb, (int)n),
+                                         dgemm_splitk_work(nbb, (int)n, (int)n))));
     t.n = n;
     t.A = A;
     t.At = sAt.as<double>();
@@ -106,12 +113,13 @@ struct EigWork {
     t.Y = sY.as<double>();
     t.Y2 = sY2.as<double>();
     t.Z = Z;
+    t.Ks = sKs.as<double>();
     return hipSuccess;
   }
   void release() {
     DevBuf* b[] = {&sW, &sVW, &sWV, &sy, &sycp, &swpre, &svtmp, &sd, &se, &stau, &spart,
                    &spart2, &ss, &sscal, &sVb, &sSb, &sTb, &sY, &sY2, &slam, &sZ, &stile, &sdp,
-                   &sAt};
+                   &sAt, &sKs};
     for (DevBuf* p : b) p->release();
   }
 };

@@ -22,6 +22,8 @@
 #include "common.h"
 #include "kernels.h"
 
+#include <algorithm>
+
 namespace imcom {
 
 constexpr int BK = 16;
@@ -58,7 +60,18 @@ __global__ __launch_bounds__(256, (BM >= 128 ? 2 : 1)) void dgemm_kernel(GemmDes
   } else {
     g = one;
   }
-  const int M = g.M, N = g.N, K = g.K;
+  const int M = g.M, N = g.N;
+  int K = g.K;
+  const double* Ag = g.A;
+  const double* Bg = g.B;
+  double* Cg = g.C;
+  if (!GROUPED && g.kchunk > 0) {  // split K: this block's k range and partial
+    const int64_t k0 = (int64_t)blockIdx.y * g.kchunk;
+    K = (int)min((int64_t)g.kchunk, (int64_t)g.K - k0);
+    Ag += TA ? k0 : k0 * g.lda;
+    Bg += TB ? k0 * g.ldb : k0;
+    Cg += (int64_t)blockIdx.y * g.c_split_stride;
+  }
   const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
   int tm, tn;
   if (g.lower) {
@@ -84,8 +97,8 @@ __global__ __launch_bounds__(256, (BM >= 128 ? 2 : 1)) void dgemm_kernel(GemmDes
     tn = (b % group) / gm;
   }
   const int m0 = tm * BM, n0 = tn * BN;
-  const double* __restrict__ A = g.A;
-  const double* __restrict__ B = g.B;
+  const double* __restrict__ A = Ag;
+  const double* __restrict__ B = Bg;
   const int64_t lda = g.lda, ldb = g.ldb;
 
   const int tid = threadIdx.x;
@@ -172,7 +185,7 @@ __global__ __launch_bounds__(256, (BM >= 128 ? 2 : 1)) void dgemm_kernel(GemmDes
   // flight together (a load-then-store per element serialises on the memory
   // latency, which dominates short-K products such as the rank-2nb updates).
   const double alpha = g.alpha, beta = g.beta;
-  double* __restrict__ C = g.C;
+  double* __restrict__ C = Cg;
   const int64_t ldc = g.ldc;
   double* __restrict__ Ct = g.Ct;
   const bool readc = beta != 0.0;
@@ -303,6 +316,66 @@ hipError_t dgemm_lower(hipStream_t st, bool ta, bool tb, int M, int K, double al
   const int64_t T64 = (M + 63) / 64;
   dim3 grid((unsigned)(T64 * (T64 + 1) / 2));
   return launch_sz<64, 64, false>(st, ta, tb, grid, g, nullptr, nullptr, 0);
+}
+
+// C = alpha * sum_s W_s in a fixed order (W_s: M x N, leading dimension M)
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int ksplit,
+                                                            const double* __restrict__ W,
+                                                            double alpha, double* __restrict__ C,
+                                                            int64_t ldc) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t mn = (int64_t)M * N;
+  if (idx >= mn) return;
+  double s = 0.0;
+  for (int k = 0; k < ksplit; ++k) s += W[idx + k * mn];
+  const int64_t i = idx % M, j = idx / M;
+  C[i + j * ldc] = alpha * s;
+}
+
+// K chunks of at least 512, about 1024 blocks in all; 128x128 tiles unless
+// that leaves fewer than 256 blocks
+static void splitk_shape(int M, int N, int K, int* tile, int* ksplit) {
+  const int maxs = std::max(1, K / 512);
+  const int64_t t128 = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
+  int s = (int)std::min<int64_t>(maxs, std::max<int64_t>(1, (1024 + t128 - 1) / t128));
+  if (t128 * s >= 256) {
+    *tile = 128;
+    *ksplit = s;
+    return;
+  }
+  const int64_t t64 = (int64_t)((M + 63) / 64) * ((N + 63) / 64);
+  *tile = 64;
+  *ksplit = (int)std::min<int64_t>(maxs, std::max<int64_t>(1, (1024 + t64 - 1) / t64));
+}
+
+size_t dgemm_splitk_work(int M, int N, int K) {
+  int tile, ks;
+  splitk_shape(M, N, K, &tile, &ks);
+  return (size_t)ks * M * N * sizeof(double);
+}
+
+hipError_t dgemm_splitk(hipStream_t st, bool ta, bool tb, int M, int N, int K, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                        int64_t ldc, double* work) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  int tile, ks;
+  splitk_shape(M, N, K, &tile, &ks);
+  const int kchunk = (K + ks - 1) / ks;
+  ks = (K + kchunk - 1) / kchunk;  // no empty chunk
+  GemmDesc g{M, N, K, 1.0, 0.0, A, lda, B, ldb, work, M};
+  g.kchunk = kchunk;
+  g.c_split_stride = (int64_t)M * N;
+  if (tile == 128) {
+    dim3 grid((unsigned)(((M + 127) / 128) * ((N + 127) / 128)), (unsigned)ks);
+    IMCOM_HIP_TRY((launch_sz<128, 128, false>(st, ta, tb, grid, g, nullptr, nullptr, 0)));
+  } else {
+    dim3 grid((unsigned)(((M + 63) / 64) * ((N + 63) / 64)), (unsigned)ks);
+    IMCOM_HIP_TRY((launch_sz<64, 64, false>(st, ta, tb, grid, g, nullptr, nullptr, 0)));
+  }
+  const int64_t mn = (int64_t)M * N;
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, st, M,
+                     N, ks, work, alpha, C, ldc);
+  return hipGetLastError();
 }
 
 int gemm_tiles64(int M, int N) { return ((M + 63) / 64) * ((N + 63) / 64); }

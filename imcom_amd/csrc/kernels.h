@@ -68,6 +68,10 @@ struct GemmDesc {
   int64_t ct_off = 0;
   int ct_nbt = 0;
   int cm_cols = 0;
+  // Split K (non-grouped launches only): blockIdx.y = s takes k in
+  // [s*kchunk, min(K, (s+1)*kchunk)) and writes C + s*c_split_stride.
+  int kchunk = 0;
+  int64_t c_split_stride = 0;
 };
 
 // Offset (in doubles) of block (I, J), I >= J, in the lower-triangle block
@@ -88,6 +92,13 @@ hipError_t dgemm_lower(hipStream_t st, bool ta, bool tb, int M, int K, double al
                        double* C, int64_t ldc, double* Ct = nullptr, int64_t ct_off = 0,
                        int ct_nbt = 0, int cm_cols = 0);
 int gemm_tiles64(int M, int N);
+// C (M x N, ldc) = alpha * op(A) op(B) with K split over workgroups; the
+// partial products go to `work` (ksplit * M * N doubles, from
+// dgemm_splitk_work) and are summed in a fixed order.
+size_t dgemm_splitk_work(int M, int N, int K);
+hipError_t dgemm_splitk(hipStream_t st, bool ta, bool tb, int M, int N, int K, double alpha,
+                        const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
+                        int64_t ldc, double* work);
 int gemm_tiles64(const GemmDesc& g);
 hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
                          const int* d_prefix, int ngroups, int total_blocks);
@@ -174,6 +185,7 @@ struct SyTile {
   double* Y;      // NBB x n
   double* Y2;     // NBB x n
   double* Z;      // n x n  eigenvectors (in: of the tridiagonal, out: of A)
+  double* Ks;     // split-K partials of the back-transform's V^T V and V^T Z
 };
 
 // Partial-sum slots of the reduction kernels, sized for the largest tile

@@ -740,12 +740,19 @@ hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t
     const int tpb = std::min(256, std::max(64, (maxK + 63) / 64 * 64));
     hipLaunchKernelGGL(dc_gather_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, w->d_merges, nm,
                        w->d_counts, w->ndcol, w->dfcol, (const double*)nullptr, w->Zg);
-    // grouped GEMM: Zo_m (K x k) = Zg_m(:, 0:k) (K x k) * U_m (k x k)
+    // Zo_m (K x k) = Zg_m(:, 0:k) (K x k) * U_m (k x k): large merges as
+    // single GEMMs (128x128 tiles when they fill the GPU), the rest as one
+    // grouped launch of 64x64 tiles
     descs.clear();
     bprefix.assign(1, 0);
     for (int i = 0; i < nm; ++i) {
       const int K = lv[i].size, k = w->h_counts[i].k;
       if (k == 0) continue;
+      if ((int64_t)((K + 127) / 128) * ((k + 127) / 128) >= 512) {
+        IMCOM_HIP_TRY(dgemm(st, false, false, K, k, k, 1.0, w->Zg + hm[i].zg_off, K,
+                            w->U + hm[i].u_off, k, 0.0, w->Zo + hm[i].zg_off, K));
+        continue;
+      }
       descs.push_back(GemmDesc{K, k, k, 1.0, 0.0, w->Zg + hm[i].zg_off, K, w->U + hm[i].u_off, k,
                                w->Zo + hm[i].zg_off, K});
       bprefix.push_back(bprefix.back() + gemm_tiles64(K, k));

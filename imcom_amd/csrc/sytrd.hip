@@ -585,9 +585,9 @@ hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
       const SyTile& h = host[b];
       if (kb >= h.n - 1) continue;
       const int64_t rows = h.n - kb - 1;
-      // S = Vb^T Vb
-      IMCOM_HIP_TRY(dgemm(st, true, false, nbb, nbb, (int)rows, 1.0, h.Vb, rows, h.Vb, rows, 0.0,
-                          h.Sb, nbb));
+      // S = Vb^T Vb (K = rows is long and the output small: split K)
+      IMCOM_HIP_TRY(dgemm_splitk(st, true, false, nbb, nbb, (int)rows, 1.0, h.Vb, rows, h.Vb, rows,
+                                 h.Sb, nbb, h.Ks));
     }
     hipLaunchKernelGGL(bt_build_T, dim3(ceil_div(nbb, kMaxNBB), 1, ntiles), dim3(256), 0, st,
                        d_tiles, kb, nbb);
@@ -604,9 +604,9 @@ hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
         IMCOM_HIP_TRY(dgemm(st, false, false, kMaxNBB, n2, kMaxNBB, -1.0, h.Tb, nbb, h.Y2, kMaxNBB,
                             0.0, h.Tb + (int64_t)kMaxNBB * nbb, nbb));
       }
-      // Y = Vb^T Z_r  (nbb x n)
-      IMCOM_HIP_TRY(dgemm(st, true, false, nbb, (int)h.n, (int)rows, 1.0, h.Vb, rows, Zr, h.n, 0.0,
-                          h.Y, nbb));
+      // Y = Vb^T Z_r  (nbb x n; split K)
+      IMCOM_HIP_TRY(dgemm_splitk(st, true, false, nbb, (int)h.n, (int)rows, 1.0, h.Vb, rows, Zr,
+                                 h.n, h.Y, nbb, h.Ks));
       // Y2 = T Y
       IMCOM_HIP_TRY(dgemm(st, false, false, nbb, (int)h.n, nbb, 1.0, h.Tb, nbb, h.Y, nbb, 0.0,
                           h.Y2, nbb));
