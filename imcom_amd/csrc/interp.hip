@@ -195,9 +195,13 @@ __global__ __launch_bounds__(256) void rotate_kernel(const double* __restrict__ 
   if (idx >= (int64_t)n1 * n2) return;
   const int iout = (int)(idx % n1) + 1, jout = (int)(idx / n1) + 1;
   const double xc = (double)(n1 + 1) / 2.0, yc = (double)(n2 + 1) / 2.0;
-  const double xo = __dsub_rn((double)iout, xc), yo = __dsub_rn((double)jout, yc);
-  const double xs = __dadd_rn(__dadd_rn(__dmul_rn(xo, cost), __dmul_rn(yo, sint)), xc);
-  const double ys = __dadd_rn(__dadd_rn(__dmul_rn(-xo, sint), __dmul_rn(yo, cost)), yc);
+  double xs, ys;
+  {
+#pragma clang fp contract(off)  // the rotated position rounded as the reference forms it
+    const double xo = (double)iout - xc, yo = (double)jout - yc;
+    xs = xo * cost + yo * sint + xc;
+    ys = -xo * sint + yo * cost + yc;
+  }
   const double fi = floor(xs), fj = floor(ys), ci = ceil(xs), cj = ceil(ys);
   double v = 0.0;
   if (fi >= 1.0 && fj >= 1.0 && ci <= (double)n1 && cj <= (double)n2) {

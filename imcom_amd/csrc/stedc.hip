@@ -344,39 +344,44 @@ __global__ void dc_rotate_kernel(int64_t n, const DcMerge* __restrict__ mg, int 
 // ---- step 4: secular equation roots ----------------------------------------
 // f(x) = 1 + rho sum_i w_i^2 / (d_i - x); root j lies in (d_j, d_{j+1}) (last
 // root in (d_{k-1}, d_{k-1} + rho |w|^2]).  Stored as lambda_j = d_org + tau.
-__device__ void secular_eval(int k, const double* __restrict__ dl, const double* __restrict__ w,
-                             double rho, int org, double tau, int j, double& f, double& psi,
-                             double& dpsi, double& phi, double& dphi) {
+// f = 1 + psi + phi at tau from d_org, with psi, dpsi over the poles
+// i <= j and phi, dphi over i > j.  The k-term sums are split over the 64
+// lanes of a wavefront (lane l takes terms l, l + 64, ...) and combined by a
+// butterfly, which leaves bitwise the same sums in every lane.
+__device__ void secular_eval_wave(int k, const double* __restrict__ dl,
+                                  const double* __restrict__ w, double rho, int org, double tau,
+                                  int j, double& f, double& psi, double& dpsi, double& phi,
+                                  double& dphi) {
+  const int lane = threadIdx.x & 63;
   const double dorg = dl[org];
-  psi = 0.0;
-  dpsi = 0.0;
-  phi = 0.0;
-  dphi = 0.0;
-  for (int i = 0; i <= j; ++i) {
+  double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+  for (int i = lane; i < k; i += 64) {
     const double del = (dl[i] - dorg) - tau;
     const double t = w[i] / del;
-    psi += w[i] * t;
-    dpsi += t * t;
+    if (i <= j) {
+      a += w[i] * t;
+      b += t * t;
+    } else {
+      c += w[i] * t;
+      d += t * t;
+    }
   }
-  for (int i = j + 1; i < k; ++i) {
-    const double del = (dl[i] - dorg) - tau;
-    const double t = w[i] / del;
-    phi += w[i] * t;
-    dphi += t * t;
-  }
-  psi *= rho;
-  dpsi *= rho;
-  phi *= rho;
-  dphi *= rho;
+  psi = wave_sum(a) * rho;
+  dpsi = wave_sum(b) * rho;
+  phi = wave_sum(c) * rho;
+  dphi = wave_sum(d) * rho;
   f = 1.0 + psi + phi;
 }
 
-__global__ void dc_secular_kernel(const DcMerge* __restrict__ mg, const DcCount* __restrict__ counts,
+__global__ __launch_bounds__(256) void dc_secular_kernel(const DcMerge* __restrict__ mg, const DcCount* __restrict__ counts,
                                   const int* __restrict__ rprefix, int nm, int total,
                                   const double* __restrict__ dlall, const double* __restrict__ wall,
                                   int* __restrict__ orgall, double* __restrict__ tauall,
                                   double* __restrict__ lamall) {
-  const int gr = blockIdx.x * blockDim.x + threadIdx.x;
+  // one wavefront per root (uniform control flow: every lane holds the
+  // same sums)
+  const int gr = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (gr >= total) return;
   int lo = 0, hi = nm - 1;
   while (lo < hi) {
@@ -397,16 +402,18 @@ __global__ void dc_secular_kernel(const DcMerge* __restrict__ mg, const DcCount*
   int org;
   double tlo, thi, tau;
   if (k == 1) {
-    orgall[m.off] = 0;
-    tauall[m.off] = rho * w[0] * w[0];
-    lamall[m.off] = dl[0] + rho * w[0] * w[0];
+    if (lane == 0) {
+      orgall[m.off] = 0;
+      tauall[m.off] = rho * w[0] * w[0];
+      lamall[m.off] = dl[0] + rho * w[0] * w[0];
+    }
     return;
   }
   if (j < k - 1) {
     const double gap = dl[j + 1] - dl[j];
     const double mid = 0.5 * gap;
     double f, psi, dpsi, phi, dphi;
-    secular_eval(k, dl, w, rho, j, mid, j, f, psi, dpsi, phi, dphi);
+    secular_eval_wave(k, dl, w, rho, j, mid, j, f, psi, dpsi, phi, dphi);
     if (f >= 0.0) {
       org = j;  // root in (d_j, mid]
       tlo = 0.0;
@@ -419,7 +426,8 @@ __global__ void dc_secular_kernel(const DcMerge* __restrict__ mg, const DcCount*
     tau = 0.5 * (tlo + thi);
   } else {
     double wn = 0.0;
-    for (int i = 0; i < k; ++i) wn += w[i] * w[i];
+    for (int i = lane; i < k; i += 64) wn += w[i] * w[i];
+    wn = wave_sum(wn);
     org = k - 1;
     tlo = 0.0;
     thi = rho * wn;
@@ -428,7 +436,7 @@ __global__ void dc_secular_kernel(const DcMerge* __restrict__ mg, const DcCount*
   const double dorg = dl[org];
   for (int it = 0; it < 200; ++it) {
     double f, psi, dpsi, phi, dphi;
-    secular_eval(k, dl, w, rho, org, tau, j, f, psi, dpsi, phi, dphi);
+    secular_eval_wave(k, dl, w, rho, org, tau, j, f, psi, dpsi, phi, dphi);
     if (f == 0.0) break;
     const double erretm = 8.0 * (phi - psi) + 1.0 + fabs(tau) * (dpsi + dphi);
     if (fabs(f) <= eps * erretm) break;
@@ -463,9 +471,11 @@ __global__ void dc_secular_kernel(const DcMerge* __restrict__ mg, const DcCount*
     }
     tau = tnew;
   }
-  orgall[m.off + j] = org;
-  tauall[m.off + j] = tau;
-  lamall[m.off + j] = dorg + tau;
+  if (lane == 0) {
+    orgall[m.off + j] = org;
+    tauall[m.off + j] = tau;
+    lamall[m.off + j] = dorg + tau;
+  }
 }
 
 // ---- step 5: Gu-Eisenstat zhat ----------------------------------------------
@@ -730,7 +740,7 @@ hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t
       hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->d_merges,
                          nm, w->d_counts, w->rp, w->rn, w->rc, w->rs, (double*)nullptr);
     if (total > 0) {
-      hipLaunchKernelGGL(dc_secular_kernel, dim3(ceil_div(total, 64)), dim3(64), 0, st, w->d_merges,
+      hipLaunchKernelGGL(dc_secular_kernel, dim3(ceil_div(total, 4)), dim3(256), 0, st, w->d_merges,
                          w->d_counts, w->d_prefix, nm, total, w->dl, w->wz, w->org, w->tau, w->lam);
       hipLaunchKernelGGL(dc_zhat_kernel, dim3(ceil_div(total, 64)), dim3(64), 0, st, w->d_merges,
                          w->d_counts, w->d_prefix, nm, total, w->dl, w->wz, w->org, w->tau, w->zhat);

@@ -68,6 +68,7 @@ __global__ __launch_bounds__(RB) void sy_colnorm(const SyTile* __restrict__ tile
 constexpr int SB = 64;  // symmetric matrix-vector block
 constexpr int SEG = 4;  // blocks of one block-column per workgroup
 constexpr int DCH = 2048;  // rows per dot-product chunk (kDotChunks * DCH >= 32768)
+constexpr int64_t kSymvNtRows = 8704;  // trailing size from which sy_symv streams (4 s^2 B > 300 MB)
 
 // The lower-triangle 64x64 blocks of A into the block layout At
 // (tri_block_offset), before the first panel.  Later panels' blocks are
@@ -116,6 +117,10 @@ __global__ __launch_bounds__(256) void sy_tile_init(const SyTile* __restrict__ t
 // sigma = [V W]^T x over rows [c+1, n) in chunks of DCH rows, one chunk per
 // wavefront (partials in dpart[q][chunk], summed by sy_mid): a whole column
 // per wavefront would leave one long serial wavefront as the launch's tail.
+// NT: the blocks are read with nontemporal loads.  Measured at n = 16384:
+// +7% while the trailing triangle exceeds the 256 MB last-level cache, -8%
+// once it fits there (the next column re-reads it from that cache).
+template <bool NT>
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
                                                int jj) {
   __shared__ double xJ[SB];
@@ -184,7 +189,8 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
         for (int u = 0; u < SEG; ++u) {
           const int64_t i = (int64_t)(I0 + u) * SB + lane;
           const bool ok = jv && (u < nb) && i >= c1 && i < n && !(diag && u == 0 && lane < cl);
-          a[p][u] = ok ? t.At[tri_block_offset(I0 + u, J, t.nbt) + cl * SB + lane] : 0.0;
+          const double* src = t.At + tri_block_offset(I0 + u, J, t.nbt) + cl * SB + lane;
+          a[p][u] = ok ? (NT ? __builtin_nontemporal_load(src) : *src) : 0.0;
         }
       }
 #pragma unroll
@@ -527,8 +533,15 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
     const int gy = nbl + ceil_div(ndot, gx);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) IMCOM_HIP_TRY(prof->events(PS_SYMV, 2, &e0, &e1));
-    hipExtLaunchKernelGGL(sy_symv, dim3(gx, gy, ntiles), dim3(256), 0, st, e0, e1, 0, d_tiles,
-                          (int)k, (int)c, jj);
+    // nontemporal block reads while the lower triangle (4 s^2 bytes) is well
+    // beyond the 256 MB last-level cache
+    const int64_t s = nmax - c - 1;
+    if (s >= kSymvNtRows)
+      hipExtLaunchKernelGGL(sy_symv<true>, dim3(gx, gy, ntiles), dim3(256), 0, st, e0, e1, 0,
+                            d_tiles, (int)k, (int)c, jj);
+    else
+      hipExtLaunchKernelGGL(sy_symv<false>, dim3(gx, gy, ntiles), dim3(256), 0, st, e0, e1, 0,
+                            d_tiles, (int)k, (int)c, jj);
     hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
                        (int)c, jj, nnorm, gx * gy);
     const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);

@@ -1,5 +1,7 @@
-// Lookup-table set-up on the device: the PSF rotation (imcom_rotate_image,
-// imcom_proc.f90:184-250; the kernel is in interp.hip), the PSF spectra
+// Tile set-up on the device: the masked input vectors (imcom_alloc_gims,
+// imcom_io.f90:84-123, and imcom_build_xy, imcom_proc.f90:56-89), the PSF
+// rotation (imcom_rotate_image, imcom_proc.f90:184-250; the kernel is in
+// interp.hip), the PSF spectra
 // (imcom_ft_r2c, imcom_proc.f90:433-475), the A and B interpolation tables
 // (imcom_build_Alookup / imcom_build_Blookup, imcom_matrix.f90:361-493) and
 // C (imcom_calc_C, imcom_matrix.f90:546-552).
@@ -102,6 +104,60 @@ __global__ __launch_bounds__(1024) void calc_C_kernel(const double* __restrict__
   if (threadIdx.x == 0) out[0] = tot * scale;
 }
 
+// Masked input vectors in one workgroup: pixels (i1, i2, e) in column-major
+// order with I < saturation (strict, as the reference's where), compacted in
+// order by a block-wide exclusive scan per 1024-pixel chunk.
+//   x = (i1 - 1) * gimxscale(e) + dither(e, 1), y likewise with i2 and
+//   dither(e, 2); exp = e (1-based); I = Im_unmasked.
+// dither is Fortran (nexp, 2).  count[0] receives n.
+__global__ __launch_bounds__(1024) void assemble_kernel(int n1, int n2, int nexp,
+                                                        const double* __restrict__ Im, double sat,
+                                                        const double* __restrict__ sx,
+                                                        const double* __restrict__ sy,
+                                                        const double* __restrict__ dither,
+                                                        double* __restrict__ x, double* __restrict__ y,
+                                                        int32_t* __restrict__ ex,
+                                                        double* __restrict__ I,
+                                                        int64_t* __restrict__ count) {
+#pragma clang fp contract(off)  // x = i * scale + dither rounded twice, as the reference
+  __shared__ int wsum[16];
+  __shared__ int64_t base;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t npx = (int64_t)n1 * n2, total = npx * nexp;
+  if (tid == 0) base = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < total; c0 += 1024) {
+    const int64_t idx = c0 + tid;
+    const bool in = idx < total;
+    const double v = in ? Im[idx] : 0.0;
+    const int keep = (in && v < sat) ? 1 : 0;
+    // exclusive scan: within the wavefront by ballot, then over wavefronts
+    const unsigned long long bal = __ballot(keep);
+    const int before = __popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[w] = __popcll(bal);
+    __syncthreads();
+    int woff = 0, tot = 0;
+    for (int q = 0; q < 16; ++q) {
+      woff += q < w ? wsum[q] : 0;
+      tot += wsum[q];
+    }
+    if (keep) {
+      const int64_t o = base + woff + before;
+      const int e = (int)(idx / npx);
+      const int64_t r = idx % npx;
+      const int i1 = (int)(r % n1), i2 = (int)(r / n1);
+      x[o] = (double)i1 * sx[e] + dither[e];
+      y[o] = (double)i2 * sy[e] + dither[nexp + e];
+      ex[o] = e + 1;
+      I[o] = v;
+    }
+    __syncthreads();
+    if (tid == 0) base += tot;
+    __syncthreads();
+  }
+  if (tid == 0) count[0] = base;
+}
+
 inline unsigned blocks_for(int64_t cnt, int bs) { return (unsigned)((cnt + bs - 1) / bs); }
 
 // The DFT matrix for (N, n, f0), cached in the context: slot 0 holds the
@@ -164,6 +220,51 @@ int check_psf_grid(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1psf, int
 }  // namespace
 
 extern "C" {
+
+int imcom_gpu_assemble_inputs(imcom_gpu_ctx* ctx, int n1gim, int n2gim, int nexp,
+                              const double* Im_unmasked, double saturation,
+                              const double* gimxscale, const double* gimyscale,
+                              const double* dither, int64_t* n_out, double* x_out, double* y_out,
+                              int32_t* exp_out, double* I_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(n1gim >= 1 && n2gim >= 1 && nexp >= 1, IMCOM_EARG, "assemble_inputs: bad sizes");
+  NEED(Im_unmasked && gimxscale && gimyscale && dither && n_out && x_out && y_out && exp_out &&
+           I_out,
+       IMCOM_EARG, "assemble_inputs: NULL array");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  TableWork& w = ctx->tw;
+  const size_t total = (size_t)n1gim * n2gim * nexp;
+  // inputs: image, then the scales and dithers; outputs x, y, I, exp, count
+  CK(w.img.reserve(total * 8 + (size_t)4 * nexp * 8), "alloc assemble inputs");
+  CK(w.spec.reserve(total * (3 * 8 + 4) + 64), "alloc assemble outputs");
+  double* dIm = w.img.as<double>();
+  double* dsc = dIm + total;
+  CK(hipMemcpyAsync(dIm, Im_unmasked, total * 8, hipMemcpyHostToDevice, ctx->stream), "upload");
+  CK(hipMemcpyAsync(dsc, gimxscale, nexp * 8, hipMemcpyHostToDevice, ctx->stream), "upload");
+  CK(hipMemcpyAsync(dsc + nexp, gimyscale, nexp * 8, hipMemcpyHostToDevice, ctx->stream), "upload");
+  CK(hipMemcpyAsync(dsc + 2 * nexp, dither, 2 * nexp * 8, hipMemcpyHostToDevice, ctx->stream),
+     "upload");
+  double* dx = w.spec.as<double>();
+  double* dy = dx + total;
+  double* dI = dy + total;
+  int64_t* dn = (int64_t*)(dI + total);
+  int32_t* de = (int32_t*)(dn + 8);
+  hipLaunchKernelGGL(assemble_kernel, dim3(1), dim3(1024), 0, ctx->stream, n1gim, n2gim, nexp, dIm,
+                     saturation, dsc, dsc + nexp, dsc + 2 * nexp, dx, dy, de, dI, dn);
+  CK(hipGetLastError(), "assemble_inputs");
+  int64_t n = 0;
+  CK(hipMemcpyAsync(&n, dn, 8, hipMemcpyDeviceToHost, ctx->stream), "download n");
+  CK(hipStreamSynchronize(ctx->stream), "assemble sync");
+  *n_out = n;
+  if (n > 0) {
+    CK(hipMemcpyAsync(x_out, dx, n * 8, hipMemcpyDeviceToHost, ctx->stream), "download x");
+    CK(hipMemcpyAsync(y_out, dy, n * 8, hipMemcpyDeviceToHost, ctx->stream), "download y");
+    CK(hipMemcpyAsync(I_out, dI, n * 8, hipMemcpyDeviceToHost, ctx->stream), "download I");
+    CK(hipMemcpyAsync(exp_out, de, n * 4, hipMemcpyDeviceToHost, ctx->stream), "download exp");
+    CK(hipStreamSynchronize(ctx->stream), "assemble sync");
+  }
+  return IMCOM_OK;
+}
 
 int imcom_gpu_rotate_image(imcom_gpu_ctx* ctx, int n1, int n2, const double* im_in,
                            double theta_deg, int npoly, int conserve_flux, double* im_out) {

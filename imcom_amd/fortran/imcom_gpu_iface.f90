@@ -46,6 +46,18 @@ MODULE imcom_gpu_iface
     end function
     ! device lookup-table set-up: spectra are complex(c_double_complex),
     ! whose storage is the interleaved (re, im) layout the C side reads
+    integer(c_int) function imcom_gpu_assemble_inputs(ctx, n1gim, n2gim, nexp, Im_unmasked, &
+        saturation, gimxscale, gimyscale, dither, n_out, x_out, y_out, exp_out, I_out) &
+        bind(C, name="imcom_gpu_assemble_inputs")
+      import :: c_int, c_ptr, c_double, c_int64_t, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n1gim, n2gim, nexp
+      real(c_double), intent(in) :: Im_unmasked(*), gimxscale(*), gimyscale(*), dither(*)
+      real(c_double), value :: saturation
+      integer(c_int64_t), intent(out) :: n_out
+      real(c_double), intent(out) :: x_out(*), y_out(*), I_out(*)
+      integer(c_int32_t), intent(out) :: exp_out(*)
+    end function
     integer(c_int) function imcom_gpu_rotate_image(ctx, n1, n2, im_in, theta_deg, npoly, &
         conserve_flux, im_out) bind(C, name="imcom_gpu_rotate_image")
       import :: c_int, c_ptr, c_double

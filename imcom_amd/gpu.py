@@ -26,7 +26,8 @@ IMCOM_PREC_BF16X6 = 2
 # every symbol declared in include/imcom_gpu.h
 EXPORTS = [
     "imcom_gpu_init", "imcom_gpu_finalize", "imcom_gpu_last_error", "imcom_gpu_synchronize",
-    "imcom_gpu_set_tile", "imcom_gpu_set_tables", "imcom_gpu_rotate_image", "imcom_gpu_ft_r2c", "imcom_gpu_build_Alookup",
+    "imcom_gpu_set_tile", "imcom_gpu_set_tables", "imcom_gpu_assemble_inputs",
+    "imcom_gpu_rotate_image", "imcom_gpu_ft_r2c", "imcom_gpu_build_Alookup",
     "imcom_gpu_build_Blookup", "imcom_gpu_calc_C", "imcom_gpu_lookup_A", "imcom_gpu_set_A",
     "imcom_gpu_eigen_A", "imcom_gpu_tridiagonalize", "imcom_gpu_tridiag_eig",
     "imcom_gpu_set_QL", "imcom_gpu_lookup_B", "imcom_gpu_set_B", "imcom_gpu_build_P",
@@ -122,6 +123,8 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_batch_solve": (_int, [_vp, _int, _int, _int, ctypes.c_float, _int]),
         "imcom_gpu_batch_result": (_int, [_vp, _int, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
         "imcom_gpu_set_tuning": (_int, [_vp, _int, _int]),
+        "imcom_gpu_assemble_inputs": (_int, [_vp, _int, _int, _int, _dp, _dbl, _dp, _dp, _dp,
+                                              ctypes.POINTER(_i64), _dp, _dp, _ip, _dp]),
         "imcom_gpu_rotate_image": (_int, [_vp, _int, _int, _dp, _dbl, _int, _int, _dp]),
         "imcom_gpu_ft_r2c": (_int, [_vp, _int, _int, _int, _dp, _dp]),
         "imcom_gpu_build_Alookup": (_int, [_vp, _int, _int, _int, _int, _int, _dp, _dp]),
@@ -207,6 +210,24 @@ class ImcomGPU:
         self.tile = tile
 
     # ---- lookup-table set-up (imcom_proc.f90:184-250,433-475, imcom_matrix.f90:361-552)
+    def assemble_inputs(self, I_unmasked, saturation, gimxscale, gimyscale, dither):
+        """Masked input vectors (imcom_io.f90:84-123, imcom_proc.f90:56-89):
+        returns x_i, y_i, exp_i, I_i."""
+        Im = np.asfortranarray(I_unmasked, dtype=np.float64)
+        n1, n2, ne = Im.shape
+        sx, sy = _vec(gimxscale), _vec(gimyscale)
+        d = np.asfortranarray(dither, dtype=np.float64)
+        tot = n1 * n2 * ne
+        x, y, I = np.empty(tot), np.empty(tot), np.empty(tot)
+        e = np.empty(tot, dtype=np.int32)
+        n = _i64(0)
+        self._ck(self.lib.imcom_gpu_assemble_inputs(self.h, n1, n2, ne, _ptr(Im), float(saturation),
+                                                    _ptr(sx), _ptr(sy), _ptr(d), ctypes.byref(n),
+                                                    _ptr(x), _ptr(y), _ptr(e, _ip), _ptr(I)),
+                 "assemble_inputs")
+        k = n.value
+        return x[:k].copy(), y[:k].copy(), e[:k].copy(), I[:k].copy()
+
     def rotate_image(self, im, theta_deg, npoly=7, conserve_flux=1):
         """imcom_rotate_image on the device."""
         a = np.asfortranarray(im, dtype=np.float64)

@@ -62,14 +62,26 @@ int imcom_gpu_set_tile(imcom_gpu_ctx* ctx, int64_t n, int64_t m, const double* x
 int imcom_gpu_set_tables(imcom_gpu_ctx* ctx, int psfconst, int nexppsf, int n1, int n2,
                          const double* Alookup, const double* Blookup);
 
-/* Device set-up of the lookup tables (the CPU steps of imcom.f90:99-116 and
- * imcom_build_Alookup / imcom_build_Blookup).  Spectra are Fortran complex(8)
+/* Device set-up of the tile: input vectors, PSF rotation, spectra and lookup
+ * tables (the CPU steps of imcom.f90:55-116 and imcom_build_Alookup /
+ * imcom_build_Blookup).  Spectra are Fortran complex(8)
  * arrays (interleaved re, im), n1psf x n2psf each, stacked over exposures;
  * the PSF grid is square with an even side, as the reference pads it
  * (imcom_io.f90:279-286).  The tables stay on the device for
  * imcom_gpu_lookup_A / _B (in place of imcom_gpu_set_tables); the host copies
  * are optional. */
 
+/* Masked input vectors for USERXY=0 regular grids: the masking of
+ * imcom_alloc_gims (imcom_io.f90:84-123) and imcom_build_xy
+ * (imcom_proc.f90:56-89).  Im_unmasked is (n1gim, n2gim, nexp); pixels with
+ * I < saturation, in that column-major order, give x_i, y_i, exp_i (1-based)
+ * and I_i; dither is (nexp, 2).  *n_out receives n; each output must hold
+ * n1gim * n2gim * nexp entries. */
+int imcom_gpu_assemble_inputs(imcom_gpu_ctx* ctx, int n1gim, int n2gim, int nexp,
+                              const double* Im_unmasked, double saturation,
+                              const double* gimxscale, const double* gimyscale,
+                              const double* dither, int64_t* n_out, double* x_out, double* y_out,
+                              int32_t* exp_out, double* I_out);
 /* imcom_rotate_image(im_in, theta_deg, npoly_in, conserve_flux, im_out)
  * (imcom_proc.f90:184-250): im_in, im_out n1 x n2; npoly <= 7 (8 is taken
  * as 7, as the reference does with a warning). */

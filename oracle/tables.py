@@ -4,6 +4,9 @@ Imported only by tests/ as the checker of the device table build; the GPU
 library never calls it.
 
 Pieces and what they follow (file:line):
+  assemble_inputs  the masking of imcom_alloc_gims (imcom_io.f90:84-123) and
+                imcom_build_xy (imcom_proc.f90:56-89), as loops over the
+                column-major pixel order
   rotate_image  imcom_rotate_image (imcom_proc.f90:184-250), in C
                 (imcom_oracle.c: or_rotate_image) over the oracle's Neville
                 routine; the angle's cos/sin from the host libm
@@ -21,6 +24,9 @@ Pieces and what they follow (file:line):
   calc_C        imcom_calc_C (imcom_matrix.f90:546-552)
 
 How each piece is pinned (tests/test_tables_oracle.py):
+  assemble_inputs  a hand-worked 3 x 2 x 2 case (saturated pixels, the
+                strict comparison, exposure ids, dithers); agreement with the
+                host set-up (imcom_amd/tile.py)
   rotate_image  identity at 0 degrees; exact reproduction of a cubic image
                 at the rotated positions (rotation direction and centre);
                 a quarter turn as a pixel permutation; flux conservation
@@ -41,6 +47,22 @@ import numpy as np
 from . import oracle as _O
 
 _dp = ctypes.POINTER(ctypes.c_double)
+
+
+def assemble_inputs(Im_unmasked, saturation, gimxscale, gimyscale, dither):
+    """x_i, y_i, exp_i, I_i of the pixels with I < saturation."""
+    n1, n2, nexp = Im_unmasked.shape
+    xs, ys, es, Is = [], [], [], []
+    for e in range(nexp):                     # column-major (i1, i2, e) order
+        for i2 in range(n2):
+            for i1 in range(n1):
+                v = Im_unmasked[i1, i2, e]
+                if v < saturation:
+                    xs.append(float(i1) * gimxscale[e] + dither[e][0])
+                    ys.append(float(i2) * gimyscale[e] + dither[e][1])
+                    es.append(e + 1)
+                    Is.append(v)
+    return np.array(xs), np.array(ys), np.array(es, dtype=np.int32), np.array(Is)
 
 
 def rotate_image(im_in, theta_deg, npoly=7, conserve_flux=1):

@@ -118,3 +118,20 @@ def test_rotate_image_matches_oracle(g, shape, theta):
         # Lagrange form on the device against Neville's tableau on the host
         assert _rel(got, ref) <= 1e-13, cf
         assert np.all(got[ref == 0.0] == 0.0)  # positions outside the image
+
+
+@pytest.mark.parametrize("shape,sat", [((16, 16, 4), 0.9), ((64, 64, 4), 0.97), ((5, 3, 2), 2.0)])
+def test_assemble_inputs_matches_oracle(g, shape, sat):
+    """Integer/ordering work: bit-exact against the oracle."""
+    print(f"[setup] assemble {shape} sat={sat}", flush=True)
+    rng = np.random.default_rng(shape[0])
+    Im = np.asfortranarray(rng.uniform(0, 1, size=shape))
+    ne = shape[2]
+    sx, sy = rng.uniform(0.9, 1.1, ne), rng.uniform(0.9, 1.1, ne)
+    dither = rng.uniform(0, 1, size=(ne, 2))
+    got = g.assemble_inputs(Im, sat, sx, sy, dither)
+    ref = OT.assemble_inputs(Im, sat, sx, sy, dither)
+    for name, a, b in zip(("x", "y", "exp", "I"), got, ref):
+        assert a.shape == b.shape, name
+        bad = np.nonzero(a != b)[0]
+        assert bad.size == 0, (name, bad.size, int(bad[0]), a[bad[0]], b[bad[0]])

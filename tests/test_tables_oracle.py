@@ -155,3 +155,32 @@ def test_rotate_conserves_flux_and_matches_host_setup():
     assert abs(out.sum() - G.sum()) <= 1e-12 * G.sum()
     host = TL.rotate_image(G, 37.5, 7, 1)
     assert np.max(np.abs(out - host)) <= 1e-12 * np.max(np.abs(out))
+
+
+# ---------------------------------------------------------- input vectors
+def test_assemble_inputs_hand_worked():
+    """3 x 2 pixels x 2 exposures, saturation 5: the values 5 (not < 5) and 9
+    are masked; x = (i1-1) * scale + dither in 1-based terms."""
+    Im = np.zeros((3, 2, 2), order="F")
+    Im[:, :, 0] = [[1.0, 5.0], [2.0, 3.0], [9.0, 4.0]]
+    Im[:, :, 1] = [[0.5, 0.25], [6.0, -1.0], [4.999, 7.0]]
+    sx, sy = [1.0, 2.0], [1.0, 0.5]
+    dither = [[0.1, 0.2], [0.3, 0.4]]
+    x, y, e, I = OT.assemble_inputs(Im, 5.0, sx, sy, dither)
+    # column-major order: (1,1),(2,1),(3,1),(1,2),(2,2),(3,2) per exposure
+    assert list(I) == [1.0, 2.0, 3.0, 4.0, 0.5, 4.999, 0.25, -1.0]
+    assert list(e) == [1, 1, 1, 1, 2, 2, 2, 2]
+    assert np.allclose(x, [0.1, 1.1, 1.1, 2.1, 0.3, 4.3, 0.3, 2.3], rtol=0, atol=1e-15)
+    assert np.allclose(y, [0.2, 0.2, 1.2, 1.2, 0.4, 0.4, 0.9, 0.9], rtol=0, atol=1e-15)
+
+
+def test_assemble_inputs_matches_host_setup():
+    rng = np.random.default_rng(5)
+    Im = np.asfortranarray(rng.uniform(0, 1, size=(7, 5, 3)))
+    sx, sy = np.array([1.0, 0.9, 1.1]), np.array([1.0, 1.05, 0.95])
+    dither = rng.uniform(0, 1, size=(3, 2))
+    x, y, e, I = OT.assemble_inputs(Im, 0.8, sx, sy, dither)
+    subs, I_h, e_h = TL.mask_pixels(Im, 0.8)
+    x_h, y_h = TL.build_xy(7, 5, sx, sy, dither, subs)
+    assert np.array_equal(I, I_h) and np.array_equal(e, e_h)
+    assert np.array_equal(x, x_h) and np.array_equal(y, y_h)
