@@ -189,7 +189,11 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
         for (int u = 0; u < SEG; ++u) {
           const int64_t i = (int64_t)(I0 + u) * SB + lane;
           const bool ok = jv && (u < nb) && i >= c1 && i < n && !(diag && u == 0 && lane < cl);
-          const double* src = t.At + tri_block_offset(I0 + u, J, t.nbt) + cl * SB + lane;
+          // global address space: global_load with a scalar base (the
+          // struct-held pointer would otherwise give flat loads)
+          const __attribute__((address_space(1))) double* src =
+              (const __attribute__((address_space(1))) double*)t.At +
+              tri_block_offset(I0 + u, J, t.nbt) + cl * SB + lane;
           a[p][u] = ok ? (NT ? __builtin_nontemporal_load(src) : *src) : 0.0;
         }
       }
@@ -262,17 +266,16 @@ __global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__
   // phase 1: loads
   double pn = 0.0, pu = 0.0;
   for (int q = tid; q < nnorm; q += MID_THREADS) pn += t.part[q];
-  double q0 = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0;
+  double qs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // eight loads in flight
   {
     int q = tid;
-    for (; q + 3 * MID_THREADS < nq; q += 4 * MID_THREADS) {
-      q0 += t.qpart[q];
-      q1 += t.qpart[q + MID_THREADS];
-      q2 += t.qpart[q + 2 * MID_THREADS];
-      q3 += t.qpart[q + 3 * MID_THREADS];
+    for (; q + 7 * MID_THREADS < nq; q += 8 * MID_THREADS) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) qs[r] += t.qpart[q + r * MID_THREADS];
     }
-    for (; q < nq; q += MID_THREADS) q0 += t.qpart[q];
+    for (; q < nq; q += MID_THREADS) qs[0] += t.qpart[q];
   }
+  const double q0 = qs[0] + qs[4], q1 = qs[1] + qs[5], q2 = qs[2] + qs[6], q3 = qs[3] + qs[7];
   for (int sg = tid; sg < nseg0; sg += MID_THREADS) pu += t.ycpart[(int64_t)sg * n + c1];
   double sig = 0.0, xc1 = 0.0;
   if (tid < 2 * jj) {
@@ -363,17 +366,46 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
   const int j0 = (int)(c1 / SB);
   const int nbl = (int)((n + SB - 1) / SB) - j0;
   const int64_t i = c1 + (int64_t)blockIdx.x * WR + rl;
+  // Loads are issued before any of their sums, as unrolled independent
+  // chains: a loop that adds each partial as it arrives serialises on the
+  // memory latency (up to 16 round trips per row at n = 16384).
+  const double scal = t.scal[0], tau = t.scal[1], alpha = t.scal[2], beta = t.scal[3],
+               w1 = t.scal[4];
+  double g = 0.0, xc = 0.0;  // A22(i, 0) as stored, and A(i, c): used after the reduction
   double qu = 0.0, qc = 0.0, q1 = 0.0;  // lane partials: u, V sW + W sV, update
   if (i < n) {
+    if (tid < WR) {
+      // A22(:, 0) as stored (the last trailing update's value: column c+1 of
+      // the block copy; column-major A may not hold it when c+1 starts the
+      // next panel)
+      g = t.At[tri_block_offset((int)(i >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 64 + (i & 63)];
+      xc = t.A[i + (int64_t)c * n];
+    }
     const int R = (int)(i / SB) - j0;  // local block row of row i
-    for (int J = lane; J <= R; J += WL) qu += t.ypart[(int64_t)J * n + i];
+    double u[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int J = lane;
+    for (; J + 7 * WL <= R; J += 8 * WL) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) u[q] += t.ypart[(int64_t)(J + q * WL) * n + i];
+    }
+    for (; J <= R; J += WL) u[0] += t.ypart[(int64_t)J * n + i];
     const int nseg = (nbl - R + SEG - 1) / SEG;
-    for (int sg = lane; sg < nseg; sg += WL) qu += t.ycpart[(int64_t)sg * n + i];
-    for (int p = lane; p < jj; p += WL) {
-      const double Vip = t.A[i + (int64_t)(k + p) * n];
-      const double Wip = t.W[i + (int64_t)p * n];
-      qc += Vip * t.s[jj + p] + Wip * t.s[p];
-      if (upd_next) q1 += Vip * t.W[c1 + (int64_t)p * n] + Wip * t.A[c1 + (int64_t)(k + p) * n];
+    int sg = lane;
+    for (; sg + 3 * WL < nseg; sg += 4 * WL) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) u[4 + q] += t.ycpart[(int64_t)(sg + q * WL) * n + i];
+    }
+    for (; sg < nseg; sg += WL) u[1] += t.ycpart[(int64_t)sg * n + i];
+    qu = ((u[0] + u[1]) + (u[2] + u[3])) + ((u[4] + u[5]) + (u[6] + u[7]));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {  // p < jj <= 63: at most four per lane
+      const int p = lane + q * WL;
+      if (p < jj) {
+        const double Vip = t.A[i + (int64_t)(k + p) * n];
+        const double Wip = t.W[i + (int64_t)p * n];
+        qc += Vip * t.s[jj + p] + Wip * t.s[p];
+        if (upd_next) q1 += Vip * t.W[c1 + (int64_t)p * n] + Wip * t.A[c1 + (int64_t)(k + p) * n];
+      }
     }
   }
   // lanes of one row: 4 per wavefront (thread bits 4, 5), then across the 4 wavefronts
@@ -394,18 +426,12 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
     const double U = red[0][0][rl] + red[0][1][rl] + red[0][2][rl] + red[0][3][rl];
     const double Cr = red[1][0][rl] + red[1][1][rl] + red[1][2][rl] + red[1][3][rl];
     const double Q1 = red[2][0][rl] + red[2][1][rl] + red[2][2][rl] + red[2][3][rl];
-    const double scal = t.scal[0], tau = t.scal[1], alpha = t.scal[2], beta = t.scal[3];
-    // A22(:, 0) as stored (the last trailing update's value: column c+1 of
-    // the block copy; column-major A may not hold it when c+1 starts the next
-    // panel)
-    const double g = t.At[tri_block_offset((int)(i >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 64 +
-                          (i & 63)];
-    const double v = (i == c1) ? 1.0 : t.A[i + (int64_t)c * n] * scal;
+    const double v = (i == c1) ? 1.0 : xc * scal;
     const double y = scal * (U - beta * g);
     const double wv = tau * (y - Cr) + alpha * v;
     if (upd_next) {
       // p = jj: V(c+1, jj) = 1, W(c+1, jj) = scal[4]
-      const double a = g - (Q1 + v * t.scal[4] + wv);
+      const double a = g - (Q1 + v * w1 + wv);
       t.A[i + c1 * n] = a;
       if (i >= c1 + 2) sq = a * a;
       if (c1 == n - 1) t.d[c1] = a;
