@@ -202,6 +202,47 @@ def test_eigen_random_spd(gpu_ctx, n, nb, nbb):
     assert np.max(np.abs(L - Lr)) <= 1e3 * n * EPS * nrm
 
 
+def test_eigen_maximum_size(gpu_ctx):
+    """n = 32768, the largest tile the C ABI accepts: A = H D H with the
+    reflector H = I - 2 u u^T has the eigenvalues diag(D) exactly and the
+    eigenvectors H e_i.  Checks every eigenvalue, the descending order and
+    four eigenpairs' residuals."""
+    n = 32768
+    _say(f"eigen_A reflected diagonal n={n}")
+    rng = np.random.default_rng(32768)
+    d = 1.0 + rng.permutation(n) / n           # distinct, in [1, 2)
+    u = rng.standard_normal(n)
+    u /= np.linalg.norm(u)
+    du = d * u
+    A = np.multiply.outer(u, 4.0 * float(u @ du) * u - 2.0 * du)
+    A -= np.multiply.outer(2.0 * du, u)
+    A[np.diag_indices(n)] += d
+    t = synth.tiny()
+    t.n, t.m = n, 4
+    t.x_i, t.y_i = np.zeros(n), np.zeros(n)
+    t.exp_i = np.ones(n, dtype=np.int32)
+    t.X_a, t.Y_a = np.zeros(4), np.zeros(4)
+    t.Ndiag = np.ones(n)
+    gpu_ctx.set_tile(t)
+    gpu_ctx.set_A(A.T)                          # symmetric: the transpose view is Fortran-ordered
+    gpu_ctx.set_tuning(64, 256)
+    try:
+        Q, L, _ = gpu_ctx.eigen_A()
+    finally:
+        gpu_ctx.set_tuning(32, 128)
+    nrm = 2.0
+    assert np.all(np.diff(L) <= 0)
+    err = np.max(np.abs(L - np.sort(d)[::-1]))
+    print(f"[eigen n={n}] max |lambda - d| = {err:.3e} = {err / (n * EPS * nrm):.3f} n eps |A|")
+    assert err <= 10 * n * EPS * nrm
+    for j in (0, 1, n // 2, n - 1):
+        q = Q[:, j]
+        r = A @ q - L[j] * q
+        assert np.max(np.abs(r)) <= 10 * n * EPS * nrm
+        assert abs(np.linalg.norm(q) - 1.0) <= 1e3 * n * EPS
+    del A, Q
+
+
 @pytest.mark.parametrize("nb,nbb", [(32, 128), (16, 64), (64, 128), (32, 256)])
 def test_eigen_accuracy_report(gpu_ctx, nb, nbb):
     """Backward error, orthogonality and eigenvalue error of the GPU
