@@ -274,20 +274,34 @@ class Tile:
     Gammat: np.ndarray = None  # target spectrum
 
 
-def assemble_tile(*, I_unmasked, gimxscale, gimyscale, dither, psf_img, rotangdeg,
-                  gamma_img, psfxscale, psfyscale, noise, n1out, n2out, outxscale,
-                  outyscale, outpos1, outpos2, psffile=None, npad=3, npoly=7,
+def assemble_tile(*, I_unmasked, gimxscale=None, gimyscale=None, dither=None, psf_img, rotangdeg,
+                  gamma_img, psfxscale, psfyscale, noise, n1out=0, n2out=0, outxscale=None,
+                  outyscale=None, outpos1=None, outpos2=None, psffile=None, npad=3, npoly=7,
                   saturation=np.finfo(np.float64).max, USB=True, US_max=1e-6,
-                  US_tol=1e-9, meta=None, host_tables=True) -> Tile:
-    """Run the reference's CPU set-up (imcom.f90:55-127) for USERXY=0 inputs:
-    masking, coordinates, psfconst, padding, rotation, spectra, lookup
-    tables, Ndiag, C and the kappa range.  host_tables=False leaves the
-    lookup tables to the device (ImcomGPU.set_tile(tile, tables="device"));
-    the spectra are kept on the tile either way."""
+                  US_tol=1e-9, meta=None, host_tables=True, xy_unmasked=None,
+                  XaYa_grid=None) -> Tile:
+    """Run the reference's CPU set-up (imcom.f90:55-127): masking,
+    coordinates, psfconst, padding, rotation, spectra, lookup tables, Ndiag,
+    C and the kappa range.  USERXY=0 inputs give the grid scales, dithers
+    and output grid; USERXY=1 inputs give xy_unmasked = (x, y), each
+    (n1gim, n2gim, nexp) (imcom_alloc_xy, imcom_io.f90:133-197), and
+    XaYa_grid = (Xa, Ya), each (n1out, n2out) (imcom_alloc_XaYa,
+    imcom_io.f90:201-240).  host_tables=False leaves the lookup tables to the
+    device (ImcomGPU.set_tile(tile, tables="device")); the spectra are kept
+    on the tile either way."""
     n1gim, n2gim, nexp = I_unmasked.shape
     mask_subs, I_i, exp_i = mask_pixels(I_unmasked, saturation)
-    x_i, y_i = build_xy(n1gim, n2gim, gimxscale, gimyscale, dither, mask_subs)
-    X_a, Y_a = build_XaYa(n1out, n2out, outxscale, outyscale, outpos1, outpos2)
+    if xy_unmasked is None:
+        x_i, y_i = build_xy(n1gim, n2gim, gimxscale, gimyscale, dither, mask_subs)
+    else:
+        x_i = np.asarray(xy_unmasked[0]).reshape(-1, order="F")[mask_subs].copy()
+        y_i = np.asarray(xy_unmasked[1]).reshape(-1, order="F")[mask_subs].copy()
+    if XaYa_grid is None:
+        X_a, Y_a = build_XaYa(n1out, n2out, outxscale, outyscale, outpos1, outpos2)
+    else:
+        n1out, n2out = XaYa_grid[0].shape
+        X_a = np.asarray(XaYa_grid[0]).reshape(-1, order="F").copy()
+        Y_a = np.asarray(XaYa_grid[1]).reshape(-1, order="F").copy()
     if psffile is None:
         psffile = ["psf.fits"] * nexp
     psfconst = test_psfconst(psffile, rotangdeg)
