@@ -147,6 +147,13 @@ def test_driver_example_matches_oracle(tmp_path, monkeypatch):
     assert np.all(out["U"][flip] < Umax) and np.all(ref["U"][flip] < Umax)
     assert np.all(dT <= t_tol)
     assert H_eq <= 1e-8
+    # a second run reads A, Q, L, B and P back from the stage files it wrote
+    # (imcom_get_A/QL/B/P cache hits) and reproduces the first run exactly
+    logs = []
+    again = D.run("config_example", "U", 1e-6, 1e-9, log=logs.append)
+    assert sum("Reading" in s for s in logs) == 4
+    for k in ("K", "U", "S", "H"):
+        assert np.array_equal(again[k], out[k]), k
 
 
 @pytest.mark.gpu
