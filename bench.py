@@ -55,6 +55,56 @@ def symv_alg_bytes(n: int) -> float:
     return float(np.sum(s * (s + 1) / 2.0) * 8.0)
 
 
+def vendor_eig(g, tile):
+    """rocSOLVER's dsyevd (one-stage reduction, divide and conquer,
+    back-transform; ROCm's FP64 symmetric eigensolver) on the same A as
+    this library's eigen_A, both timed on the device-resident matrix."""
+    import ctypes
+    import torch
+    rb = ctypes.CDLL("/opt/rocm/lib/librocblas.so")
+    rs = ctypes.CDLL("/opt/rocm/lib/librocsolver.so")
+    vp, ci = ctypes.c_void_p, ctypes.c_int
+    rs.rocsolver_dsyevd.argtypes = [vp, ci, ci, ci, vp, ci, vp, vp, vp]
+    rs.rocsolver_dsyevd.restype = ci
+    h = vp()
+    if rb.rocblas_create_handle(ctypes.byref(h)) != 0:
+        raise RuntimeError("rocblas_create_handle failed")
+    EVECT_ORIGINAL, FILL_LOWER = 211, 122
+    n = tile.n
+    A = g.lookup_A(tile.npoly, tile.npad, out=True)
+    g.synchronize()
+    t1 = time.perf_counter()
+    _, L, _ = g.eigen_A(out_Q=False, out_L=True)
+    g.synchronize()
+    ours = time.perf_counter() - t1
+
+    def syevd(Ad, nn):
+        D = torch.empty(nn, dtype=torch.float64, device="cuda:0")
+        E = torch.empty_like(D)
+        info = torch.zeros(1, dtype=torch.int32, device="cuda:0")
+        st = rs.rocsolver_dsyevd(h, EVECT_ORIGINAL, FILL_LOWER, nn, Ad.data_ptr(), nn,
+                                 D.data_ptr(), E.data_ptr(), info.data_ptr())
+        torch.cuda.synchronize()
+        if st != 0 or int(info.item()) != 0:
+            raise RuntimeError(f"rocsolver_dsyevd: status {st}, info {int(info.item())}")
+        return D
+    syevd(torch.eye(512, dtype=torch.float64, device="cuda:0").reshape(-1).clone(), 512)  # code load
+    Ad = torch.from_numpy(np.ascontiguousarray(A.ravel(order="F"))).to("cuda:0")
+    del A
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    D = syevd(Ad, n)
+    vendor = time.perf_counter() - t1
+    Dv = D.cpu().numpy()[::-1]
+    rb.rocblas_destroy_handle(h)
+    diff = float(np.max(np.abs(Dv - L)) / np.max(np.abs(L)))
+    log(f"eigendecomposition n={n}: this library {ours * 1e3:.1f} ms, rocSOLVER dsyevd "
+        f"{vendor * 1e3:.1f} ms; max eigenvalue difference {diff:.2e} of lambda_max")
+    return {"library": "rocSOLVER dsyevd (ROCm 7.2), evect=original, lower", "n": n,
+            "vendor_ms": round(vendor * 1e3, 1), "this_library_ms": round(ours * 1e3, 1),
+            "max_eigenvalue_diff_rel": diff}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +113,8 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "small", "batch"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--vendor-eig", action="store_true",
+                    help="also time rocSOLVER's dsyevd on the same A (single-tile workloads)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
@@ -251,6 +303,10 @@ def main():
                   "device_ms": round(t_dev * 1e3, 2), "host_numpy_ms": round(t_host * 1e3, 2)}
         log(f"table set-up: device {t_dev * 1e3:.1f} ms, host numpy {t_host * 1e3:.1f} ms ({N}^2)")
 
+    vendor = None
+    if args.vendor_eig and args.workload != "batch":
+        vendor = vendor_eig(g, tile)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -289,6 +345,8 @@ def main():
         cfg["pcie_inclusive_px_per_s"] = round(m_total / pcie, 2)
     if tables is not None:
         cfg["table_setup"] = tables
+    if vendor is not None:
+        cfg["vendor_eigensolver"] = vendor
     if plain is not None:
         cfg["plain_fp64_px_per_s"] = round(m_total / plain[0], 2)
         cfg["plain_fp64_ms_per_step"] = round(plain[0] * 1e3, 2)

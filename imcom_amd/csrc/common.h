@@ -27,6 +27,12 @@ struct Status {
     if (_e != hipSuccess) return _e;                          \
   } while (0)
 
+// A load through the global address space (global_load with a scalar or
+// vector base); a pointer read from a struct would otherwise give a flat load.
+__device__ __forceinline__ double ldg(const double* p) {
+  return *(const __attribute__((address_space(1))) double*)p;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
   // butterfly over the 64 lanes of a wavefront
 #pragma unroll

@@ -120,6 +120,10 @@ __global__ __launch_bounds__(256) void sy_tile_init(const SyTile* __restrict__ t
 // NT: the blocks are read with nontemporal loads.  Measured at n = 16384:
 // +7% while the trailing triangle exceeds the 256 MB last-level cache, -8%
 // once it fits there (the next column re-reads it from that cache).
+// Two columns per step (eight block loads in flight per lane) was measured
+// against 4, 8 and 16 at n = 16384 (bitwise the same results): 4 and 8 are
+// no faster, and 16 (the whole strip in one step, 183 VGPRs, two waves per
+// SIMD) is 10-20% slower at every trailing size above 2000.
 template <bool NT>
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
                                                int jj) {
@@ -176,7 +180,6 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
     __syncthreads();  // xJ
     double qacc = 0.0;
     const bool diag = (I0 == J);
-    // two columns per step: eight loads in flight per lane
 #pragma unroll 1
     for (int q0 = 0; q0 < 16; q0 += 2) {
       double a[2][SEG];
@@ -366,9 +369,11 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
   const int j0 = (int)(c1 / SB);
   const int nbl = (int)((n + SB - 1) / SB) - j0;
   const int64_t i = c1 + (int64_t)blockIdx.x * WR + rl;
-  // Loads are issued before any of their sums, as unrolled independent
-  // chains: a loop that adds each partial as it arrives serialises on the
-  // memory latency (up to 16 round trips per row at n = 16384).
+  // Every load is issued before the first sum (straight-line code, one
+  // memory round trip): 16 ypart and 4 ycpart partials of u per lane (all of
+  // them up to n = 16384; longer rows finish in the loops below), and the
+  // panel's V, W entries, up to 4 columns per lane.  A loop that adds each
+  // partial as it arrives serialises on the memory latency.
   const double scal = t.scal[0], tau = t.scal[1], alpha = t.scal[2], beta = t.scal[3],
                w1 = t.scal[4];
   double g = 0.0, xc = 0.0;  // A22(i, 0) as stored, and A(i, c): used after the reduction
@@ -378,34 +383,46 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
       // A22(:, 0) as stored (the last trailing update's value: column c+1 of
       // the block copy; column-major A may not hold it when c+1 starts the
       // next panel)
-      g = t.At[tri_block_offset((int)(i >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 64 + (i & 63)];
-      xc = t.A[i + (int64_t)c * n];
+      g = ldg(t.At + tri_block_offset((int)(i >> 6), (int)(c1 >> 6), t.nbt) + (c1 & 63) * 64 +
+              (i & 63));
+      xc = ldg(t.A + i + (int64_t)c * n);
     }
     const int R = (int)(i / SB) - j0;  // local block row of row i
-    double u[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int J = lane;
-    for (; J + 7 * WL <= R; J += 8 * WL) {
-#pragma unroll
-      for (int q = 0; q < 8; ++q) u[q] += t.ypart[(int64_t)(J + q * WL) * n + i];
-    }
-    for (; J <= R; J += WL) u[0] += t.ypart[(int64_t)J * n + i];
     const int nseg = (nbl - R + SEG - 1) / SEG;
-    int sg = lane;
-    for (; sg + 3 * WL < nseg; sg += 4 * WL) {
+    double u[20];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) u[4 + q] += t.ycpart[(int64_t)(sg + q * WL) * n + i];
+    for (int q = 0; q < 16; ++q) {
+      const int J = lane + q * WL;
+      u[q] = (J <= R) ? ldg(t.ypart + (int64_t)J * n + i) : 0.0;
     }
-    for (; sg < nseg; sg += WL) u[1] += t.ycpart[(int64_t)sg * n + i];
-    qu = ((u[0] + u[1]) + (u[2] + u[3])) + ((u[4] + u[5]) + (u[6] + u[7]));
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {  // p < jj <= 63: at most four per lane
+    for (int q = 0; q < 4; ++q) {
+      const int sg = lane + q * WL;
+      u[16 + q] = (sg < nseg) ? ldg(t.ycpart + (int64_t)sg * n + i) : 0.0;
+    }
+    double pv[4][6];  // V(i,p), W(i,p), sW(p), sV(p), W(c+1,p), V(c+1,p); p < jj <= 63
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
       const int p = lane + q * WL;
-      if (p < jj) {
-        const double Vip = t.A[i + (int64_t)(k + p) * n];
-        const double Wip = t.W[i + (int64_t)p * n];
-        qc += Vip * t.s[jj + p] + Wip * t.s[p];
-        if (upd_next) q1 += Vip * t.W[c1 + (int64_t)p * n] + Wip * t.A[c1 + (int64_t)(k + p) * n];
-      }
+      const bool ok = p < jj, okn = ok && upd_next;
+      pv[q][0] = ok ? ldg(t.A + i + (int64_t)(k + p) * n) : 0.0;
+      pv[q][1] = ok ? ldg(t.W + i + (int64_t)p * n) : 0.0;
+      pv[q][2] = ok ? ldg(t.s + jj + p) : 0.0;
+      pv[q][3] = ok ? ldg(t.s + p) : 0.0;
+      pv[q][4] = okn ? ldg(t.W + c1 + (int64_t)p * n) : 0.0;
+      pv[q][5] = okn ? ldg(t.A + c1 + (int64_t)(k + p) * n) : 0.0;
+    }
+    for (int J = lane + 16 * WL; J <= R; J += WL) u[0] += ldg(t.ypart + (int64_t)J * n + i);
+    for (int sg = lane + 4 * WL; sg < nseg; sg += WL) u[16] += ldg(t.ycpart + (int64_t)sg * n + i);
+#pragma unroll
+    for (int q = 0; q < 10; ++q) u[q] += u[q + 10];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) u[q] += u[q + 5];
+    qu = ((u[0] + u[1]) + (u[2] + u[3])) + u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      qc += pv[q][0] * pv[q][2] + pv[q][1] * pv[q][3];
+      q1 += pv[q][0] * pv[q][4] + pv[q][1] * pv[q][5];
     }
   }
   // lanes of one row: 4 per wavefront (thread bits 4, 5), then across the 4 wavefronts
@@ -562,12 +579,13 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
     // nontemporal block reads while the lower triangle (4 s^2 bytes) is well
     // beyond the 256 MB last-level cache
     const int64_t s = nmax - c - 1;
+    const dim3 grid(gx, gy, ntiles);
     if (s >= kSymvNtRows)
-      hipExtLaunchKernelGGL(sy_symv<true>, dim3(gx, gy, ntiles), dim3(256), 0, st, e0, e1, 0,
-                            d_tiles, (int)k, (int)c, jj);
+      hipExtLaunchKernelGGL(sy_symv<true>, grid, dim3(256), 0, st, e0, e1, 0, d_tiles, (int)k,
+                            (int)c, jj);
     else
-      hipExtLaunchKernelGGL(sy_symv<false>, dim3(gx, gy, ntiles), dim3(256), 0, st, e0, e1, 0,
-                            d_tiles, (int)k, (int)c, jj);
+      hipExtLaunchKernelGGL(sy_symv<false>, grid, dim3(256), 0, st, e0, e1, 0, d_tiles, (int)k,
+                            (int)c, jj);
     hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
                        (int)c, jj, nnorm, gx * gy);
     const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);
