@@ -64,6 +64,8 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  (void)fetch_wait_all(ctx);
+  if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
   DevBuf* bufs[] = {&ctx->x,     &ctx->y,    &ctx->ex,     &ctx->Xa,    &ctx->Ya,   &ctx->Nd,
                     &ctx->Atab,  &ctx->Btab, &ctx->I,      &ctx->A,     &ctx->B,    &ctx->Q,
                     &ctx->L,     &ctx->P,    &ctx->K,      &ctx->US,    &ctx->T,    &ctx->W,
@@ -79,6 +81,10 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
   imcom_batch_free(ctx);
   dc_destroy(ctx->dc);
   ctx->prof.destroy();
+  for (hipEvent_t& e : ctx->fetch_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->fetch_ready) (void)hipEventDestroy(ctx->fetch_ready);
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -108,6 +114,7 @@ int imcom_gpu_set_tile(imcom_gpu_ctx* ctx, int64_t n, int64_t m, const double* x
          "set_tile: pixels must be exposure-major (exp_i non-decreasing)");
   }
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_wait_all(ctx), "set_tile: wait for fetches");
   ctx->n = n;
   ctx->m = m;
   ctx->C_a = C_a;
@@ -167,6 +174,7 @@ int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   int rc = check_interp(ctx, ctx->have_Atab, "A", npoly, npad, psfxscale, psfyscale);
   if (rc) return rc;
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_A), "lookup_A: wait for fetches");
   const int64_t n = ctx->n;
   CK(ctx->A.reserve((size_t)n * n * 8), "alloc A");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
@@ -185,6 +193,7 @@ int imcom_gpu_set_A(imcom_gpu_ctx* ctx, const double* A) {
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_tile && A, IMCOM_EARG, "set_A: tile not set or NULL A");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_A), "set_A: wait for fetches");
   CK(upload(ctx, ctx->A, A, (size_t)ctx->n * ctx->n * 8), "upload A");
   CK(hipStreamSynchronize(ctx->stream), "set_A sync");
   ctx->have_A = true;
@@ -197,6 +206,7 @@ int imcom_gpu_lookup_B(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   int rc = check_interp(ctx, ctx->have_Btab, "B", npoly, npad, psfxscale, psfyscale);
   if (rc) return rc;
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_B), "lookup_B: wait for fetches");
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->B.reserve((size_t)n * m * 8), "alloc B");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
@@ -215,6 +225,7 @@ int imcom_gpu_set_B(imcom_gpu_ctx* ctx, const double* B) {
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_tile && B, IMCOM_EARG, "set_B: tile not set or NULL B");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_B), "set_B: wait for fetches");
   CK(upload(ctx, ctx->B, B, (size_t)ctx->n * ctx->m * 8), "upload B");
   CK(hipStreamSynchronize(ctx->stream), "set_B sync");
   ctx->have_B = true;
@@ -231,6 +242,8 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_A, IMCOM_ESTATE, "eigen_A: A not built (lookup_A or set_A first)");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, (1u << IMCOM_FETCH_A) | (1u << IMCOM_FETCH_Q) | (1u << IMCOM_FETCH_L)),
+     "eigen_A: wait for fetches");
   const int64_t n = ctx->n;
   SyTile t;
   CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
@@ -281,6 +294,7 @@ int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out) {
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_A, IMCOM_ESTATE, "tridiagonalize: A not built (lookup_A or set_A first)");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_A), "tridiagonalize: wait for fetches");
   const int64_t n = ctx->n;
   SyTile t;
   CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
@@ -301,6 +315,8 @@ int imcom_gpu_tridiag_eig(imcom_gpu_ctx* ctx, int64_t n, const double* d, const 
   if (!ctx) return IMCOM_EARG;
   NEED(n >= 1 && n <= 32768 && d && (e || n == 1), IMCOM_EARG, "tridiag_eig: bad arguments");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, (1u << IMCOM_FETCH_Q) | (1u << IMCOM_FETCH_L)),
+     "tridiag_eig: wait for fetches");
   CK(upload(ctx, ctx->ew.sd, d, (size_t)n * 8), "upload d");
   CK(ctx->ew.se.reserve((size_t)n * 8), "alloc e");
   if (n > 1) CK(hipMemcpyAsync(ctx->ew.se.p, e, (size_t)(n - 1) * 8, hipMemcpyHostToDevice, ctx->stream), "upload e");
@@ -319,6 +335,7 @@ int imcom_gpu_set_QL(imcom_gpu_ctx* ctx, const double* Q, const double* L) {
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_tile && Q && L, IMCOM_EARG, "set_QL: tile not set or NULL input");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, (1u << IMCOM_FETCH_Q) | (1u << IMCOM_FETCH_L)), "set_QL: wait for fetches");
   const int64_t n = ctx->n;
   CK(upload(ctx, ctx->Q, Q, (size_t)n * n * 8), "upload Q");
   CK(upload(ctx, ctx->L, L, (size_t)n * 8), "upload L");
@@ -350,6 +367,7 @@ int imcom_gpu_build_P(imcom_gpu_ctx* ctx, int precision, double* P_out) {
   NEED(ctx->have_QL, IMCOM_ESTATE, "build_P: Q, L not available (eigen_A or set_QL first)");
   NEED(ctx->have_B, IMCOM_ESTATE, "build_P: B not built (lookup_B or set_B first)");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_P), "build_P: wait for fetches");
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->P.reserve((size_t)n * m * 8), "alloc P");
   StageTimer tm(ctx, PS_BUILD_P);
@@ -366,6 +384,7 @@ int imcom_gpu_set_P(imcom_gpu_ctx* ctx, const double* P) {
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_tile && P, IMCOM_EARG, "set_P: tile not set or NULL P");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_P), "set_P: wait for fetches");
   CK(upload(ctx, ctx->P, P, (size_t)ctx->n * ctx->m * 8), "upload P");
   CK(hipStreamSynchronize(ctx->stream), "set_P sync");
   ctx->have_P = true;
@@ -409,6 +428,7 @@ int imcom_gpu_build_T(imcom_gpu_ctx* ctx, int precision, double* T_out) {
   NEED(ctx->have_K && ctx->have_P && ctx->have_QL, IMCOM_ESTATE,
        "build_T: needs Q, L, P and K (bisect_kappa first)");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_T), "build_T: wait for fetches");
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->W.reserve((size_t)n * m * 8), "alloc W");
   CK(ctx->T.reserve((size_t)n * m * 8), "alloc T");

@@ -171,7 +171,38 @@ struct imcom_gpu_ctx {
   int mlp_step = 0;
   // batched tiles
   struct BatchState* batch = nullptr;
+  // asynchronous fetches of stage outputs (capi_fetch.hip): the copy stream,
+  // the event the copies start after, and per output the event after which
+  // its device buffer may be overwritten
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t fetch_ready = nullptr;
+  hipEvent_t fetch_ev[IMCOM_FETCH_N] = {};
+  bool fetch_pending[IMCOM_FETCH_N] = {};
 };
+
+// A stage about to write outputs with a fetch in flight (bits of
+// 1 << IMCOM_FETCH_*) makes the compute stream wait for those copies; the
+// host does not block.
+inline hipError_t fetch_guard(imcom_gpu_ctx* ctx, unsigned mask) {
+  for (int w = 0; w < IMCOM_FETCH_N; ++w) {
+    if (!((mask >> w) & 1u) || !ctx->fetch_pending[w]) continue;
+    const hipError_t e = hipStreamWaitEvent(ctx->stream, ctx->fetch_ev[w], 0);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// Host-side wait for every fetch in flight (before buffers are reallocated
+// or released).
+inline hipError_t fetch_wait_all(imcom_gpu_ctx* ctx) {
+  for (int w = 0; w < IMCOM_FETCH_N; ++w) {
+    if (!ctx->fetch_pending[w]) continue;
+    const hipError_t e = hipEventSynchronize(ctx->fetch_ev[w]);
+    if (e != hipSuccess) return e;
+    ctx->fetch_pending[w] = false;
+  }
+  return hipSuccess;
+}
 
 void imcom_batch_free(imcom_gpu_ctx* ctx);
 

@@ -223,7 +223,9 @@ def _cached(name: str, forceSys: int) -> bool:
 def run(config: str, mode: str, US_max: float, US_tol: float, forceT: int = 0, forceSys: int = 0,
         saturation: float = np.finfo(np.float64).max, device: int = 0, log=print):
     """One tile, as the reference's main program; returns the outputs.  The
-    FITS files are written on a background thread while the device works."""
+    stage outputs are copied to page-locked host memory on a second stream
+    while the next stages run (fetch_async), and the FITS files are written
+    on a background thread."""
     from .gpu import ImcomGPU
     if mode not in ("U", "u", "S", "s"):
         raise ConfigError("IMCOM ERROR: mode must be U or S")
@@ -233,50 +235,68 @@ def run(config: str, mode: str, US_max: float, US_tol: float, forceT: int = 0, f
     sf = cfg.stage_files
     n, m = t.n, t.m
     w = _Writer()
+    inflight = []  # (output, file name, host array, file shape): copies not yet handed over
+
+    def fetch(g, what, name, shape):
+        inflight.append((what, name, g.fetch_async(what), shape))
+
+    def hand_over(g, *whats):
+        """Wait for the named copies and queue their files."""
+        for item in [f for f in inflight if f[0] in whats]:
+            g.fetch_wait(item[0])
+            w.put(item[1], item[2].reshape(item[3], order="F"))
+            inflight.remove(item)
+
     try:
         with ImcomGPU(device) as g:
             g.set_tile(t, tables="device")
-            # imcom_get_A / imcom_get_QL / imcom_get_B / imcom_get_P
+            # imcom_get_A / imcom_get_QL / imcom_get_B / imcom_get_P; each
+            # stage file is copied out on a second stream while the next
+            # stages run
             if _cached(sf["A"], forceSys):
                 log(f"IMCOM: Reading A matrix from {sf['A']}")
                 g.set_A(_sized(fits.readfits(sf["A"]), (n, n), sf["A"]))
             else:
-                A = g.lookup_A(t.npoly, t.npad, out=sf["A"] != "None")
-                if A is not None:
-                    w.put(sf["A"], A)
+                g.lookup_A(t.npoly, t.npad)
+                if sf["A"] != "None":
+                    fetch(g, "A", sf["A"], (n, n))
             if _cached(sf["Q"], forceSys) and _cached(sf["L"], forceSys):
                 log(f"IMCOM: Reading Q, L from {sf['Q']}, {sf['L']}")
                 g.set_QL(_sized(fits.readfits(sf["Q"]), (n, n), sf["Q"]),
                          _sized(fits.readfits(sf["L"]), (n, 1), sf["L"]).ravel(order="F"))
             else:
-                save = sf["Q"] != "None" and sf["L"] != "None"
-                Q, L, cond = g.eigen_A(out_Q=save, out_L=save)
+                _, _, cond = g.eigen_A(out_Q=False, out_L=False)
                 log(f"IMCOM: A matrix condition number = {cond:24.16E}")
-                if save:
-                    w.put(sf["Q"], Q)
-                    w.put(sf["L"], L.reshape(n, 1))
+                if sf["Q"] != "None" and sf["L"] != "None":
+                    fetch(g, "Q", sf["Q"], (n, n))
+                    fetch(g, "L", sf["L"], (n, 1))
+            hand_over(g, "A")  # the reduction waited for this copy before overwriting A
             if _cached(sf["B"], forceSys):
                 log(f"IMCOM: Reading B matrix from {sf['B']}")
                 g.set_B(_sized(fits.readfits(sf["B"]), (n, m), sf["B"]))
             else:
-                B = g.lookup_B(t.npoly, t.npad, out=sf["B"] != "None")
-                if B is not None:
-                    w.put(sf["B"], B)
+                g.lookup_B(t.npoly, t.npad)
+                if sf["B"] != "None":
+                    fetch(g, "B", sf["B"], (n, m))
             if _cached(sf["P"], forceSys):
                 log(f"IMCOM: Reading P matrix from {sf['P']}")
                 g.set_P(_sized(fits.readfits(sf["P"]), (n, m), sf["P"]))
             else:
-                P = g.build_P(out=sf["P"] != "None")
-                if P is not None:
-                    w.put(sf["P"], P)
+                g.build_P()
+                if sf["P"] != "None":
+                    fetch(g, "P", sf["P"], (n, m))
             # imcom_get_T (always solved) and imcom_build_image
             K, US, ev = g.bisect_kappa()
-            T = g.build_T(out=True)
+            hand_over(g, "Q", "L", "B")
+            g.build_T(out=False)
+            T = g.fetch_async("T")
             if USB:
                 U, S = US, g.build_S()
             else:
                 S, U = US, g.build_U()
             H = g.build_image()
+            hand_over(g, "P")
+            g.fetch_wait("T")
         grid = (t.n1out, t.n2out)
         w.put(cfg.Kfile, K.reshape(grid, order="F"))
         w.put(cfg.Ufile, U.reshape(grid, order="F"))

@@ -13,6 +13,8 @@ MODULE imcom_gpu_iface
 
   integer(c_int), parameter :: IMCOM_PREC_FP64 = 0, IMCOM_PREC_FP32X3 = 1, &
                                IMCOM_PREC_BF16X6 = 2
+  integer(c_int), parameter :: IMCOM_FETCH_A = 0, IMCOM_FETCH_Q = 1, IMCOM_FETCH_L = 2, &
+                               IMCOM_FETCH_B = 3, IMCOM_FETCH_P = 4, IMCOM_FETCH_T = 5
 
   interface
     integer(c_int) function imcom_gpu_init(ctx, device) bind(C, name="imcom_gpu_init")
@@ -179,6 +181,19 @@ MODULE imcom_gpu_iface
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: I_i(*)
       real(c_double), intent(out) :: H_out(*)
+    end function
+    ! stage files copied out while the next stages run: the host array must
+    ! stay allocated and untouched until imcom_gpu_fetch_wait returns
+    integer(c_int) function imcom_gpu_fetch_async(ctx, what, host) bind(C, name="imcom_gpu_fetch_async")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: what
+      real(c_double) :: host(*)
+    end function
+    integer(c_int) function imcom_gpu_fetch_wait(ctx, what) bind(C, name="imcom_gpu_fetch_wait")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: what
     end function
   end interface
 

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -38,8 +39,12 @@ EXPORTS = [
     "imcom_gpu_mlp_get_params", "imcom_gpu_kappa_features", "imcom_gpu_mlp_infer",
     "imcom_gpu_mlp_grad", "imcom_gpu_mlp_train", "imcom_gpu_bisect_kappa_learned",
     "imcom_gpu_batch_set", "imcom_gpu_batch_solve", "imcom_gpu_batch_result",
-    "imcom_gpu_set_tuning",
+    "imcom_gpu_set_tuning", "imcom_gpu_fetch_async", "imcom_gpu_fetch_wait",
+    "imcom_gpu_host_alloc", "imcom_gpu_host_free",
 ]
+
+# stage outputs of fetch_async (IMCOM_FETCH_*)
+FETCH = {"A": 0, "Q": 1, "L": 2, "B": 3, "P": 4, "T": 5}
 
 
 class TileDesc(ctypes.Structure):
@@ -132,6 +137,10 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_build_Alookup": (_int, [_vp, _int, _int, _int, _int, _int, _dp, _dp]),
         "imcom_gpu_build_Blookup": (_int, [_vp, _int, _int, _int, _int, _int, _dp, _dp, _dp]),
         "imcom_gpu_calc_C": (_int, [_vp, _int, _int, _dp, _dp]),
+        "imcom_gpu_fetch_async": (_int, [_vp, _int, _dp]),
+        "imcom_gpu_fetch_wait": (_int, [_vp, _int]),
+        "imcom_gpu_host_alloc": (_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+        "imcom_gpu_host_free": (_int, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -147,6 +156,21 @@ def set_gemm_library(lib: int):
     rc = load_library().imcom_gpu_set_gemm_library(int(lib))
     if rc != 0:
         raise ImcomError(f"set_gemm_library({lib}) failed (code {rc})")
+
+
+def host_empty(shape, order="F") -> np.ndarray:
+    """An uninitialised float64 array in page-locked host memory
+    (imcom_gpu_host_alloc), released when the last view of it is gone.
+    Device-to-host copies into it run asynchronously (fetch_async)."""
+    L = load_library()
+    count = int(np.prod(shape))
+    p = _vp()
+    rc = L.imcom_gpu_host_alloc(max(count, 1) * 8, ctypes.byref(p))
+    if rc != 0:
+        raise ImcomError(f"host_alloc of {count * 8} bytes failed (code {rc})")
+    flat = np.ctypeslib.as_array(ctypes.cast(p, _dp), shape=(max(count, 1),))
+    weakref.finalize(flat, L.imcom_gpu_host_free, p.value)
+    return flat[:count].reshape(shape, order=order)
 
 
 def _ptr(a, kind=_dp):
@@ -354,6 +378,27 @@ class ImcomGPU:
     def set_P(self, P):
         P = _mat(P)
         self._ck(self.lib.imcom_gpu_set_P(self.h, _ptr(P)), "set_P")
+
+    # ---- stage files ----------------------------------------------------
+    def fetch_async(self, what, out=None):
+        """Start copying the stage output `what` ("A", "Q", "L", "B", "P"
+        or "T"), as it stands after the work issued so far, into `out` (a
+        new page-locked array by default) and return that array; it holds
+        the output once fetch_wait(what) has returned.  Later stages that
+        overwrite the device buffer wait for the copy on the device."""
+        n, m = self.n, self.m
+        shape = {"A": (n, n), "Q": (n, n), "L": (n,), "B": (n, m), "P": (n, m), "T": (n, m)}[what]
+        if out is None:
+            out = host_empty(shape)
+        elif out.shape != shape or out.dtype != np.float64 or not out.flags.f_contiguous:
+            raise ValueError(f"fetch_async: out must be a Fortran-ordered float64 {shape} array")
+        self._ck(self.lib.imcom_gpu_fetch_async(self.h, FETCH[what], _ptr(out)), f"fetch_async({what})")
+        return out
+
+    def fetch_wait(self, what=None):
+        """Wait for the copy of `what` (every output when None)."""
+        self._ck(self.lib.imcom_gpu_fetch_wait(self.h, -1 if what is None else FETCH[what]),
+                 f"fetch_wait({what})")
 
     # ---- imcom_bisect ---------------------------------------------------
     def bisect_kappa(self, UnotS=None, US=None, k_min=None, k_max=None, tol=None, maxnits=None,

@@ -18,6 +18,7 @@
 #ifndef IMCOM_GPU_H
 #define IMCOM_GPU_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -39,6 +40,17 @@ enum {
   IMCOM_PREC_FP64 = 0,   /* FP64 matrix cores */
   IMCOM_PREC_FP32X3 = 1, /* split FP32 x3 on FP32 MFMA, FP64 recombination */
   IMCOM_PREC_BF16X6 = 2  /* split BF16 x6 on BF16 MFMA, FP64 recombination */
+};
+
+/* stage outputs for imcom_gpu_fetch_async */
+enum {
+  IMCOM_FETCH_A = 0, /* A_aij (n x n) */
+  IMCOM_FETCH_Q = 1, /* Q_ij (n x n) */
+  IMCOM_FETCH_L = 2, /* L_i (n) */
+  IMCOM_FETCH_B = 3, /* B_ia (n x m) */
+  IMCOM_FETCH_P = 4, /* P_ia (n x m) */
+  IMCOM_FETCH_T = 5, /* T_ia (n x m) */
+  IMCOM_FETCH_N = 6
 };
 
 /* Context lifetime.  No reference counterpart (the reference keeps module
@@ -240,6 +252,25 @@ int imcom_gpu_set_profiling(imcom_gpu_ctx* ctx, int level);
  * same results on the path's shapes. */
 int imcom_gpu_set_gemm_library(int lib);
 int imcom_gpu_stage_times(imcom_gpu_ctx* ctx, double* ms_out, int64_t* counts_out, int nslots);
+
+/* Stage files written while the device works on.  The reference writes A, Q,
+ * L, B, P and T synchronously between stages when their file names are not
+ * None (imcom_matrix.f90:275-278, 348-351, 672-677, 700-703, 748-749); the
+ * synchronous form here is the stage's own output pointer.  These three
+ * calls have no reference counterpart:
+ * imcom_gpu_fetch_async copies the stage output `what` (IMCOM_FETCH_*), as
+ * it stands after the work issued so far, to `host` on a second stream and
+ * returns at once.  The output must exist (the stage has run, or its set_*
+ * call).  Stages that overwrite that device buffer (lookup_A and eigen_A for
+ * A, eigen_A for Q and L, lookup_B for B, build_P for P, build_T for T, and
+ * the set_* calls) wait for the copy on the device.  `host` must stay valid
+ * until imcom_gpu_fetch_wait(ctx, what) (what = -1: every output) returns;
+ * set_tile and finalize wait for every copy.  Page-locked memory from
+ * imcom_gpu_host_alloc lets the copies run beside the later stages. */
+int imcom_gpu_fetch_async(imcom_gpu_ctx* ctx, int what, double* host);
+int imcom_gpu_fetch_wait(imcom_gpu_ctx* ctx, int what);
+int imcom_gpu_host_alloc(size_t bytes, void** ptr);
+int imcom_gpu_host_free(void* ptr);
 
 #ifdef __cplusplus
 }
