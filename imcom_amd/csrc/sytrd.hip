@@ -356,6 +356,7 @@ constexpr int WL = 16;  // lanes per row (256 threads)
 // stored in W(:, jj); v into A(:, c); then, when column c+1 is in this panel
 // or is the tile's last column, its rank-2(jj+1) update and its norm
 // partials (for the next sy_mid).
+template <bool WIDE>
 __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles, int k, int c,
                                                int jj, int in_panel_next) {
   __shared__ double red[3][4][WR];
@@ -369,11 +370,14 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
   const int j0 = (int)(c1 / SB);
   const int nbl = (int)((n + SB - 1) / SB) - j0;
   const int64_t i = c1 + (int64_t)blockIdx.x * WR + rl;
-  // Every load is issued before the first sum (straight-line code, one
-  // memory round trip): 16 ypart and 4 ycpart partials of u per lane (all of
-  // them up to n = 16384; longer rows finish in the loops below), and the
-  // panel's V, W entries, up to 4 columns per lane.  A loop that adds each
-  // partial as it arrives serialises on the memory latency.
+  // WIDE (one tile): every load is issued before the first sum (straight-line
+  // code, one memory round trip): 16 ypart and 4 ycpart partials of u per
+  // lane (all of them up to n = 16384; longer rows finish in the loops
+  // below), and the panel's V, W entries, up to 4 columns per lane.  This
+  // takes 118 VGPRs; the batched launch, where most workgroups exit at once,
+  // runs the 36-VGPR form with eight partials in flight per step instead
+  // (measured: 8.4 -> 7.0 us per launch for one tile at n = 16384, but
+  // 18 -> 22 us for the 64-tile batch).
   const double scal = t.scal[0], tau = t.scal[1], alpha = t.scal[2], beta = t.scal[3],
                w1 = t.scal[4];
   double g = 0.0, xc = 0.0;  // A22(i, 0) as stored, and A(i, c): used after the reduction
@@ -389,40 +393,68 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
     }
     const int R = (int)(i / SB) - j0;  // local block row of row i
     const int nseg = (nbl - R + SEG - 1) / SEG;
-    double u[20];
+    if (WIDE) {
+      double u[20];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int J = lane + q * WL;
-      u[q] = (J <= R) ? ldg(t.ypart + (int64_t)J * n + i) : 0.0;
-    }
+      for (int q = 0; q < 16; ++q) {
+        const int J = lane + q * WL;
+        u[q] = (J <= R) ? ldg(t.ypart + (int64_t)J * n + i) : 0.0;
+      }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int sg = lane + q * WL;
-      u[16 + q] = (sg < nseg) ? ldg(t.ycpart + (int64_t)sg * n + i) : 0.0;
-    }
-    double pv[4][6];  // V(i,p), W(i,p), sW(p), sV(p), W(c+1,p), V(c+1,p); p < jj <= 63
+      for (int q = 0; q < 4; ++q) {
+        const int sg = lane + q * WL;
+        u[16 + q] = (sg < nseg) ? ldg(t.ycpart + (int64_t)sg * n + i) : 0.0;
+      }
+      double pv[4][6];  // V(i,p), W(i,p), sW(p), sV(p), W(c+1,p), V(c+1,p); p < jj <= 63
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int p = lane + q * WL;
-      const bool ok = p < jj, okn = ok && upd_next;
-      pv[q][0] = ok ? ldg(t.A + i + (int64_t)(k + p) * n) : 0.0;
-      pv[q][1] = ok ? ldg(t.W + i + (int64_t)p * n) : 0.0;
-      pv[q][2] = ok ? ldg(t.s + jj + p) : 0.0;
-      pv[q][3] = ok ? ldg(t.s + p) : 0.0;
-      pv[q][4] = okn ? ldg(t.W + c1 + (int64_t)p * n) : 0.0;
-      pv[q][5] = okn ? ldg(t.A + c1 + (int64_t)(k + p) * n) : 0.0;
-    }
-    for (int J = lane + 16 * WL; J <= R; J += WL) u[0] += ldg(t.ypart + (int64_t)J * n + i);
-    for (int sg = lane + 4 * WL; sg < nseg; sg += WL) u[16] += ldg(t.ycpart + (int64_t)sg * n + i);
+      for (int q = 0; q < 4; ++q) {
+        const int p = lane + q * WL;
+        const bool ok = p < jj, okn = ok && upd_next;
+        pv[q][0] = ok ? ldg(t.A + i + (int64_t)(k + p) * n) : 0.0;
+        pv[q][1] = ok ? ldg(t.W + i + (int64_t)p * n) : 0.0;
+        pv[q][2] = ok ? ldg(t.s + jj + p) : 0.0;
+        pv[q][3] = ok ? ldg(t.s + p) : 0.0;
+        pv[q][4] = okn ? ldg(t.W + c1 + (int64_t)p * n) : 0.0;
+        pv[q][5] = okn ? ldg(t.A + c1 + (int64_t)(k + p) * n) : 0.0;
+      }
+      for (int J = lane + 16 * WL; J <= R; J += WL) u[0] += ldg(t.ypart + (int64_t)J * n + i);
+      for (int sg = lane + 4 * WL; sg < nseg; sg += WL) u[16] += ldg(t.ycpart + (int64_t)sg * n + i);
 #pragma unroll
-    for (int q = 0; q < 10; ++q) u[q] += u[q + 10];
+      for (int q = 0; q < 10; ++q) u[q] += u[q + 10];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) u[q] += u[q + 5];
-    qu = ((u[0] + u[1]) + (u[2] + u[3])) + u[4];
+      for (int q = 0; q < 5; ++q) u[q] += u[q + 5];
+      qu = ((u[0] + u[1]) + (u[2] + u[3])) + u[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      qc += pv[q][0] * pv[q][2] + pv[q][1] * pv[q][3];
-      q1 += pv[q][0] * pv[q][4] + pv[q][1] * pv[q][5];
+      for (int q = 0; q < 4; ++q) {
+        qc += pv[q][0] * pv[q][2] + pv[q][1] * pv[q][3];
+        q1 += pv[q][0] * pv[q][4] + pv[q][1] * pv[q][5];
+      }
+    } else {
+      double u[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+      int J = lane;
+      for (; J + 7 * WL <= R; J += 8 * WL) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) u[q] += ldg(t.ypart + (int64_t)(J + q * WL) * n + i);
+      }
+      for (; J <= R; J += WL) u[0] += ldg(t.ypart + (int64_t)J * n + i);
+      int sg = lane;
+      for (; sg + 3 * WL < nseg; sg += 4 * WL) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) u[4 + q] += ldg(t.ycpart + (int64_t)(sg + q * WL) * n + i);
+      }
+      for (; sg < nseg; sg += WL) u[1] += ldg(t.ycpart + (int64_t)sg * n + i);
+      qu = ((u[0] + u[1]) + (u[2] + u[3])) + ((u[4] + u[5]) + (u[6] + u[7]));
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {  // p < jj <= 63: at most four per lane
+        const int p = lane + q * WL;
+        if (p < jj) {
+          const double Vip = ldg(t.A + i + (int64_t)(k + p) * n);
+          const double Wip = ldg(t.W + i + (int64_t)p * n);
+          qc += Vip * ldg(t.s + jj + p) + Wip * ldg(t.s + p);
+          if (upd_next)
+            q1 += Vip * ldg(t.W + c1 + (int64_t)p * n) + Wip * ldg(t.A + c1 + (int64_t)(k + p) * n);
+        }
+      }
     }
   }
   // lanes of one row: 4 per wavefront (thread bits 4, 5), then across the 4 wavefronts
@@ -589,8 +621,12 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
     hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
                        (int)c, jj, nnorm, gx * gy);
     const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);
-    hipLaunchKernelGGL(sy_wupd, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k, (int)c,
-                       jj, (jj + 1 < pb) ? 1 : 0);
+    if (ntiles == 1)
+      hipLaunchKernelGGL(sy_wupd<true>, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
+                         (int)c, jj, (jj + 1 < pb) ? 1 : 0);
+    else
+      hipLaunchKernelGGL(sy_wupd<false>, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
+                         (int)c, jj, (jj + 1 < pb) ? 1 : 0);
     nnorm = nw;  // sy_wupd leaves the next column's norm partials, one per workgroup
   }
   return hipGetLastError();
