@@ -196,11 +196,13 @@ class ImcomGPU:
             raise ImcomError(f"imcom_gpu_init failed (code {rc}): no usable GPU {device}")
         self.h = h
         self.n = self.m = 0
+        self._inflight = {}  # output name -> host arrays with a copy in flight
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.imcom_gpu_finalize(self.h)
+            self.lib.imcom_gpu_finalize(self.h)  # waits for every copy in flight
             self.h = None
+            self._inflight = {}
 
     def __enter__(self):
         return self
@@ -230,6 +232,7 @@ class ImcomGPU:
         x, y, e, X, Y, N = self._keep
         self._ck(self.lib.imcom_gpu_set_tile(self.h, tile.n, tile.m, _ptr(x), _ptr(y), _ptr(e, _ip),
                                              _ptr(X), _ptr(Y), _ptr(N), tile.C_a), "set_tile")
+        self._inflight = {}  # set_tile has waited for every copy
         if tables == "device":
             if tile.Gt_rot is None or tile.Gammat is None:
                 raise ImcomError("set_tile(tables='device') needs the tile's spectra")
@@ -393,12 +396,18 @@ class ImcomGPU:
         elif out.shape != shape or out.dtype != np.float64 or not out.flags.f_contiguous:
             raise ValueError(f"fetch_async: out must be a Fortran-ordered float64 {shape} array")
         self._ck(self.lib.imcom_gpu_fetch_async(self.h, FETCH[what], _ptr(out)), f"fetch_async({what})")
+        # the copy writes into `out` until fetch_wait: keep it alive until then
+        self._inflight.setdefault(what, []).append(out)
         return out
 
     def fetch_wait(self, what=None):
         """Wait for the copy of `what` (every output when None)."""
         self._ck(self.lib.imcom_gpu_fetch_wait(self.h, -1 if what is None else FETCH[what]),
                  f"fetch_wait({what})")
+        if what is None:
+            self._inflight.clear()
+        else:
+            self._inflight.pop(what, None)
 
     # ---- imcom_bisect ---------------------------------------------------
     def bisect_kappa(self, UnotS=None, US=None, k_min=None, k_max=None, tol=None, maxnits=None,

@@ -17,7 +17,7 @@ Pieces and what follows the reference (file:line):
                         (imcom_matrix.f90:904-970)
   build_T/U/S/image     C restatements (imcom_matrix.f90:716-877)
 
-How each piece is pinned (tests/test_oracle_*.py):
+How each piece is pinned (tests/test_oracle.py, tests/test_tables_oracle.py):
   interp        exact reproduction of degree<=7 polynomials sampled on a table
                 (8-point and edge-clipped stencils), exact node values, and
                 the Gaussian closed form A(D) = s^2/(4 pi sig^2) exp(-D^2/4sig^2)
