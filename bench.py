@@ -131,8 +131,9 @@ def main():
                           "launches": 5}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); tiles shard "
-              "across GPUs with no communication, see DESIGN section 7", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); a mosaic's "
+              "tiles would shard across GPUs with no communication, one tile's solve stays "
+              "on one GPU", file=sys.stderr)
         sys.exit(2)
 
     import torch  # device plumbing only: barrier/synchronize around the timed region
