@@ -3,7 +3,8 @@
  *
  * A plain-C restatement of the reference Fortran routines on the solve path,
  * used by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg as
- * the checker.  It is never linked into, or called by, the GPU library.
+ * the reference the GPU results are compared with.  It is never linked into,
+ * or called by, the GPU library.
  *
  * Every array is Fortran column-major with leading dimension n (the
  * imcom_data globals, imcom_data.f90:95-132); exposure ids are 1-based.

@@ -1,7 +1,8 @@
 """CPU oracle of the IMCOM solve path -- TEST INFRASTRUCTURE ONLY.
 
 Imported only by tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
-leg, as the checker.  The GPU library never calls it.
+leg, as the reference the GPU results are compared with.  The GPU library
+never calls it.
 
 Pieces and what follows the reference (file:line):
   lookup_A / lookup_B   C restatement (imcom_oracle.c) of imcom_lookup_A/B and

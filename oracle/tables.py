@@ -1,7 +1,7 @@
 """CPU oracle of the lookup-table set-up -- TEST INFRASTRUCTURE ONLY.
 
-Imported only by tests/ as the checker of the device table build; the GPU
-library never calls it.
+Imported only by tests/, as the reference the device table build is compared
+with; the GPU library never calls it.
 
 Pieces and what they follow (file:line):
   assemble_inputs  the masking of imcom_alloc_gims (imcom_io.f90:84-123) and
