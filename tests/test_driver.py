@@ -157,6 +157,39 @@ def test_driver_example_matches_oracle(tmp_path, monkeypatch):
 
 
 @pytest.mark.gpu
+def test_driver_example_noise_mode_matches_oracle(tmp_path, monkeypatch):
+    """Mode S (imcom_bisect_by_S, imcom_bisect.f90:226-267; U from
+    imcom_build_U, imcom_matrix.f90:795-835) on the example, noise target
+    0.05 with tolerance 1e-6: every pixel's bisection ends inside the
+    target, away from both ends of the kappa range."""
+    from oracle import oracle as O
+    from imcom_amd import tile as TL
+    print("[driver] example userxy0, mode S", flush=True)
+    monkeypatch.chdir(_workdir(tmp_path, 0))
+    Smax, tol = 0.05, 1e-6
+    out = D.run("config_example", "S", Smax, tol, log=lambda s: None)
+    assert np.array_equal(fits.readfits("S.example.fits").ravel(order="F"), out["S"])
+    t = out["tile"]
+    assert not t.USB and t.US_max == Smax
+    t.Alookup = TL.build_Alookup(t.Gt_rot, t.npad)
+    t.Blookup = TL.build_Blookup(t.Gammat, t.Gt_rot, t.npad)
+    ref = O.solve(t)
+    eq = out["K"] == ref["K"]
+    dS = np.abs(out["S"] - ref["S"])
+    print(f"[driver] mode S: K equal {eq.mean():.3f} max|dS|={dS.max():.3e} "
+          f"S in [{out['S'].min():.6f}, {out['S'].max():.6f}]", flush=True)
+    assert np.all(out["S"] < Smax) and np.all(Smax - out["S"] <= tol)
+    assert np.all(ref["S"] < Smax) and np.all(Smax - ref["S"] <= tol)
+    assert dS.max() <= tol
+    assert eq.mean() >= 0.85
+    assert np.all((out["K"] > t.kappa_min) & (out["K"] < t.kappa_max))
+    # U recomputed at the pixel's kappa, and H, where kappa agrees
+    assert np.max(np.abs(out["U"][eq] - ref["U"][eq])) <= 1e-9
+    H_eq = np.abs(out["H"][eq] - ref["H"][eq]).max() / np.abs(ref["H"]).max()
+    assert H_eq <= 1e-8
+
+
+@pytest.mark.gpu
 def test_driver_batch_matches_single_tile(tmp_path, monkeypatch):
     """The batched front-end on the example's two forms (USERXY=0 and 1, the
     same tile) gives the single-tile driver's kappa, U and H."""
