@@ -421,21 +421,29 @@ hipError_t dgemm_splitk(hipStream_t st, bool ta, bool tb, int M, int N, int K, d
   return hipGetLastError();
 }
 
-int gemm_tiles64(int M, int N) { return ((M + 63) / 64) * ((N + 63) / 64); }
-
-int gemm_tiles64(const GemmDesc& g) {
-  if (g.lower) {
-    const int T = (g.M + 63) / 64;
-    return T * (T + 1) / 2;
-  }
-  return gemm_tiles64(g.M, g.N);
+static int gemm_tiles(int M, int N, int tile) {
+  return ((M + tile - 1) / tile) * ((N + tile - 1) / tile);
 }
 
+static int gemm_tiles(const GemmDesc& g, int tile) {
+  if (g.lower) {
+    const int T = (g.M + tile - 1) / tile;
+    return T * (T + 1) / 2;
+  }
+  return gemm_tiles(g.M, g.N, tile);
+}
+
+int gemm_tiles64(int M, int N) { return gemm_tiles(M, N, 64); }
+
+int gemm_tiles64(const GemmDesc& g) { return gemm_tiles(g, 64); }
+
 hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
-                         const int* d_prefix, int ngroups, int total_blocks) {
+                         const int* d_prefix, int ngroups, int total_blocks, int tile) {
   if (ngroups <= 0 || total_blocks <= 0) return hipSuccess;
   GemmDesc dummy{};
   dim3 grid((unsigned)total_blocks);
+  if (tile == 128)
+    return launch_sz<128, 128, true>(st, ta, tb, grid, dummy, d_descs, d_prefix, ngroups);
   return launch_sz<64, 64, true>(st, ta, tb, grid, dummy, d_descs, d_prefix, ngroups);
 }
 
@@ -443,13 +451,21 @@ hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_des
 
 namespace imcom {
 
+// 128x128 tiles once they give at least two workgroups per CU over the
+// launch; smaller launches keep 64x64 tiles to fill the GPU.
+constexpr int kGrouped128Min = 512;
+
 int GroupedPlan::add(bool ta, bool tb, const std::vector<GemmDesc>& groups) {
-  Launch L{(int)descs.size(), (int)prefix.size(), 0, 0, ta, tb};
+  Launch L{(int)descs.size(), (int)prefix.size(), 0, 0, ta, tb, 64};
+  int blocks128 = 0;
+  for (const GemmDesc& g : groups)
+    if (g.M > 0 && g.N > 0) blocks128 += gemm_tiles(g, 128);
+  if (blocks128 >= kGrouped128Min) L.tile = 128;
   prefix.push_back(0);
   for (const GemmDesc& g : groups) {
     if (g.M <= 0 || g.N <= 0) continue;
     descs.push_back(g);
-    L.blocks += gemm_tiles64(g);
+    L.blocks += gemm_tiles(g, L.tile);
     prefix.push_back(L.blocks);
     L.ngroups += 1;
   }
@@ -476,7 +492,7 @@ hipError_t GroupedPlan::run(hipStream_t st, int i) const {
   const Launch& L = launches[i];
   if (L.ngroups == 0) return hipSuccess;
   return dgemm_grouped(st, L.ta, L.tb, d_descs + L.desc_off, d_prefix + L.prefix_off, L.ngroups,
-                       L.blocks);
+                       L.blocks, L.tile);
 }
 
 void GroupedPlan::release() {

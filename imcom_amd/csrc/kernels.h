@@ -104,8 +104,9 @@ hipError_t dgemm_splitk(hipStream_t st, bool ta, bool tb, int M, int N, int K, d
                         const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
                         int64_t ldc, double* work);
 int gemm_tiles64(const GemmDesc& g);
+// tile: 64 or 128; total_blocks and d_prefix count tiles of that size
 hipError_t dgemm_grouped(hipStream_t st, bool ta, bool tb, const GemmDesc* d_descs,
-                         const int* d_prefix, int ngroups, int total_blocks);
+                         const int* d_prefix, int ngroups, int total_blocks, int tile = 64);
 
 // A sequence of grouped GEMM launches whose descriptors are uploaded once
 // (before any graph capture) and replayed by index.
@@ -113,6 +114,7 @@ struct GroupedPlan {
   struct Launch {
     int desc_off, prefix_off, ngroups, blocks;
     bool ta, tb;
+    int tile;  // 64 or 128 (square output tiles)
   };
   std::vector<GemmDesc> descs;
   std::vector<int> prefix;
