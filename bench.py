@@ -10,8 +10,8 @@ Workloads:
          Airy (obscuration 0.3, lambda/D = 0.5 px) x pixel PSF, 96x96 output
          pixels at 0.5 px (m = 9216), U_max = 1e-6
   small  one tile: 4 x 16x16 (n = 1024), Gaussian PSF, 24x24 output (m = 576)
-  batch  64 tiles with n in [1024, 6144] and m in [256, 2304], one batched
-         solve per step (HIP-graph replay)
+  batch  64 tiles with n in [1024, 6144] and m in [256, 2304]; the fast path
+         is three concurrent batched solves per step (HIP-graph replay)
 
 Two paths are timed: "plain" (full-range bisection, FP64 GEMMs) and "fast"
 (learned kappa bracket when trained weights exist, BF16x6 split GEMM for T;
@@ -149,18 +149,24 @@ def main():
     m_total = sum(t.m for t in tiles)
     log(f"workload {args.workload}: {len(tiles)} tile(s), n in [{min(t.n for t in tiles)}, "
         f"{max(t.n for t in tiles)}], sum m = {m_total} ({time.time() - t0:.1f}s host set-up)")
-    g = ImcomGPU(0)
-    autotune.apply(g, max(t.n for t in tiles))
     weights = kappa_net.load_weights()
-    if weights is not None:
-        g.mlp_set_params(weights)
     learned = weights is not None
+    g = None  # the single-tile context (the batch workload makes its own)
+    if args.workload != "batch":
+        g = ImcomGPU(0)
+        autotune.apply(g, max(t.n for t in tiles))
+        if weights is not None:
+            g.mlp_set_params(weights)
 
     if args.workload == "batch":
-        # fast: every tile in one batched solve (lockstep launches, one HIP graph);
-        # plain: the single-tile FP64 path tile after tile, each tile's inputs
-        # resident in its own context before the timed region
-        g.batch_set(tiles)
+        # fast: the tiles dealt over three batched solves (lockstep launches
+        # replayed from HIP graphs), one stream each, solved concurrently
+        # (imcom_amd.batch); plain: the single-tile FP64 path tile after tile,
+        # each tile's inputs resident in its own context before the timed region
+        # (the plain path's contexts are released before the streamed solves
+        # are set up: streams beyond the process's hardware queues share them)
+        from imcom_amd.batch import DEFAULT_STREAMS, StreamedBatch
+        sbs = []
         singles = []
         if not args.skip_plain:
             for t in tiles:
@@ -181,18 +187,26 @@ def main():
             gs.build_image()
             return U, ev
 
+        def prepare_fast():
+            for gs in singles:
+                gs.close()
+            singles.clear()
+            sbs.append(StreamedBatch(tiles, DEFAULT_STREAMS, weights=weights))
+
         def step(fast):
             if not fast:
                 res = [single_step(gs, t) for gs, t in zip(singles, tiles)]
                 return (np.concatenate([r[0] for r in res]), np.concatenate([r[1] for r in res]),
                         Umax_all)
-            g.batch_solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
-            res = [g.batch_result(i) for i in range(len(tiles))]
+            sb = sbs[0]
+            sb.solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
+            res = [sb.result(i) for i in range(len(tiles))]
             U = np.concatenate([r["U"] for r in res])
             ev = np.concatenate([r["evals"] for r in res])
             return U, ev, Umax_all
     else:
         g.set_tile(tile)
+        prepare_fast = lambda: None  # noqa: E731
 
         def step(fast):
             g.lookup_A(tile.npoly, tile.npad)
@@ -212,11 +226,13 @@ def main():
         for _ in range(args.warmup):
             step(fast)
         torch.cuda.synchronize()
-        g.synchronize()
+        if g is not None:
+            g.synchronize()
         t1 = time.perf_counter()
         for _ in range(args.steps):
             U, ev, Umax = step(fast)
-        g.synchronize()
+        if g is not None:
+            g.synchronize()
         torch.cuda.synchronize()
         sec = (time.perf_counter() - t1) / args.steps
         bad = int(np.sum(~(U < Umax)))
@@ -225,6 +241,7 @@ def main():
         return sec, bad, float(ev.mean())
 
     plain = None if args.skip_plain else timed(False)
+    prepare_fast()
     fast = timed(True)
     sec, n_bad, mean_ev = fast
 
@@ -335,7 +352,8 @@ def main():
 
     cfg = {"workload": (f"{args.workload}: {len(tiles)} tile(s), n={tile.n if len(tiles) == 1 else 'mixed'}"
                         f" input px, sum m={m_total} output px, U_max={tile.US_max}"),
-           "path": (("batched tiles in one HIP graph + " if args.workload == "batch" else "")
+           "path": (("batched tiles (three concurrent batched solves, one stream each, each "
+                     "replayed from HIP graphs) + " if args.workload == "batch" else "")
                     + ("learned kappa bracket + BF16x6 split GEMM for T" if learned
                        else "BF16x6 split GEMM for T (no trained bracket weights found)")
                     + "; P and the bisection FP64"),
@@ -361,9 +379,12 @@ def main():
     }
     print(json.dumps(out), flush=True)
     if args.workload == "batch":
+        for sb in sbs:
+            sb.close()
         for gs in singles:
             gs.close()
-    g.close()
+    if g is not None:
+        g.close()
 
 
 if __name__ == "__main__":

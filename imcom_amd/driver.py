@@ -325,9 +325,10 @@ class _InDir:
 
 
 def run_batch(configs, mode: str, US_max: float, US_tol: float, device: int = 0, log=print):
-    """Several tiles (one configuration file each) in one batched solve.
-    Each configuration's file names are taken relative to its own directory."""
-    from .gpu import ImcomGPU
+    """Several tiles (one configuration file each) in concurrent batched
+    solves (imcom_amd.batch).  Each configuration's file names are taken
+    relative to its own directory."""
+    from .batch import StreamedBatch
     USB = mode in ("U", "u")
     cfgs, tiles = [], []
     for path in configs:
@@ -335,11 +336,10 @@ def run_batch(configs, mode: str, US_max: float, US_tol: float, device: int = 0,
             cfgs.append(read_config(os.path.basename(path)))
             tiles.append(build_tile(cfgs[-1], USB, US_max, US_tol, host_tables=True))
     outs = []
-    with ImcomGPU(device) as g:
-        g.batch_set(tiles)
-        g.batch_solve()
+    with StreamedBatch(tiles, device=device) as sb:
+        sb.solve()
         for i, (path, c, t) in enumerate(zip(configs, cfgs, tiles)):
-            r = g.batch_result(i, want_T=True)
+            r = sb.result(i, want_T=True)
             grid = (t.n1out, t.n2out)
             U, S = r["U"], r["S"]
             with _InDir(path):

@@ -88,6 +88,31 @@ def test_batch_matches_oracle(gpu_ctx, batch_tiles, use_graph):
             assert np.all(np.isfinite(out["S"][fl])) and np.all(np.isfinite(out["H"][fl]))
 
 
+def test_streamed_batch_matches_oracle(batch_tiles):
+    """The tiles dealt over three concurrent batched solves (imcom_amd.batch,
+    one context and stream each, solved from three host threads): the same
+    bounds against the oracle, and a second solve reproduces the first."""
+    from imcom_amd.batch import StreamedBatch
+    print("[case] streamed batch, 3 streams", flush=True)
+    ts, refs = batch_tiles
+    with StreamedBatch(ts, 3) as sb:
+        assert sorted(i for gr in sb.groups for i in gr) == list(range(len(ts)))
+        sb.solve(use_graph=True)
+        first = [sb.result(i) for i in range(len(ts))]
+        sb.solve(use_graph=True)
+        for i, (t, r) in enumerate(zip(ts, refs)):
+            out = sb.result(i)
+            assert np.array_equal(out["K"], first[i]["K"]) and np.array_equal(out["H"], first[i]["H"])
+            ok = r["evals"] > 1
+            close = np.abs(out["K"] / r["K"] - 1.0) <= 1e-12
+            assert close.mean() >= 0.98, (i, close.mean())
+            assert np.all((np.abs(out["U"] - r["U"]) <= u_tol(r, t))[ok])
+            tT = t_tol(r, t)
+            Htol = 1e-8 * np.max(np.abs(r["H"])) + tT * np.sum(np.abs(t.I_i))
+            assert np.all((np.abs(out["H"] - r["H"]) <= Htol)[ok])
+            assert np.all(out["U"][~ok] >= t.US_max)
+
+
 def test_batch_equals_single_tile_path(gpu_ctx, batch_tiles):
     """The batched solve and the single-tile path run the same kernels with
     different launch groupings, so their results agree to the conditioning
