@@ -4,11 +4,14 @@ One batched solve (imcom_gpu_batch_solve) advances all its tiles column by
 column in lockstep; its last columns, where only the largest tiles are still
 being reduced, are short latency-bound launches.  Several batched solves on
 separate contexts (each with its own HIP stream), issued from separate host
-threads, fill those gaps with each other's work.  Tiles are dealt to the
-solves round-robin in decreasing size, so each solve gets a similar mix.
+threads, fill those gaps with each other's work.  The tiles, sorted by
+size, are cut into runs of about equal eigendecomposition work (sum of n^3):
+the first solve takes the few largest tiles, the last the many small ones.
 
-Measured on the 64-tile workload (n from 1024 to 5766): one solve 962 ms,
-two 905 ms, three 877 ms, four 986 ms.  The streams share the process's
+Measured on the 64-tile workload (n from 1024 to 5766): one solve 962 ms;
+tiles dealt round-robin by size over two solves 905 ms, three 872 ms, four
+986 ms; three runs of equal n^3 work (4, 9 and 51 tiles) 829 ms, against
+893 ms for three runs of equal tile count.  The streams share the process's
 hardware queues (four by default): with four solves, or with another
 context still open that was created before these (its stream took a queue
 first), two solves end up on one queue and the gain is lost (1055 ms
@@ -28,14 +31,16 @@ DEFAULT_STREAMS = 3
 
 
 class StreamedBatch:
-    """Tiles dealt over `nstreams` batched solves, solved concurrently."""
+    """Tiles split over `nstreams` batched solves, solved concurrently."""
 
     def __init__(self, tiles, nstreams: int = DEFAULT_STREAMS, device: int = 0, weights=None,
                  tune: bool = True):
         if nstreams < 1:
             raise ValueError("nstreams must be >= 1")
         order = np.argsort([t.n for t in tiles], kind="stable")[::-1]
-        self.groups = [[int(i) for i in order[j::nstreams]] for j in range(nstreams)]
+        work = np.cumsum(np.array([tiles[i].n for i in order], dtype=np.float64) ** 3)
+        cuts = [int(np.searchsorted(work, work[-1] * j / nstreams)) + 1 for j in range(1, nstreams)]
+        self.groups = [[int(i) for i in part] for part in np.split(order, cuts)]
         self.groups = [g for g in self.groups if g]
         self.where = {}  # tile index -> (solve, index within it)
         self.ctxs = []
