@@ -159,9 +159,9 @@ def main():
             g.mlp_set_params(weights)
 
     if args.workload == "batch":
-        # fast: the tiles dealt over three batched solves (lockstep launches
-        # replayed from HIP graphs), one stream each, solved concurrently
-        # (imcom_amd.batch); plain: the single-tile FP64 path tile after tile,
+        # fast: the size-sorted tiles cut into three runs of about equal n^3
+        # work, one batched solve each (lockstep launches replayed from HIP
+        # graphs) on its own stream, solved concurrently (imcom_amd.batch); plain: the single-tile FP64 path tile after tile,
         # each tile's inputs resident in its own context before the timed region
         # (the plain path's contexts are released before the streamed solves
         # are set up: streams beyond the process's hardware queues share them)
