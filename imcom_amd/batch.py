@@ -30,18 +30,26 @@ from .gpu import ImcomGPU
 DEFAULT_STREAMS = 3
 
 
+def split_tiles(ns, nstreams: int = DEFAULT_STREAMS) -> list[list[int]]:
+    """Tile indices, largest first, cut into at most `nstreams` runs of about
+    equal sum of n^3 (empty runs dropped)."""
+    if nstreams < 1:
+        raise ValueError("nstreams must be >= 1")
+    if len(ns) == 0:
+        raise ValueError("no tiles")
+    order = np.argsort(np.asarray(ns), kind="stable")[::-1]
+    work = np.cumsum(np.asarray(ns, dtype=np.float64)[order] ** 3)
+    cuts = [int(np.searchsorted(work, work[-1] * j / nstreams)) + 1 for j in range(1, nstreams)]
+    groups = [[int(i) for i in part] for part in np.split(order, cuts)]
+    return [g for g in groups if g]
+
+
 class StreamedBatch:
     """Tiles split over `nstreams` batched solves, solved concurrently."""
 
     def __init__(self, tiles, nstreams: int = DEFAULT_STREAMS, device: int = 0, weights=None,
                  tune: bool = True):
-        if nstreams < 1:
-            raise ValueError("nstreams must be >= 1")
-        order = np.argsort([t.n for t in tiles], kind="stable")[::-1]
-        work = np.cumsum(np.array([tiles[i].n for i in order], dtype=np.float64) ** 3)
-        cuts = [int(np.searchsorted(work, work[-1] * j / nstreams)) + 1 for j in range(1, nstreams)]
-        self.groups = [[int(i) for i in part] for part in np.split(order, cuts)]
-        self.groups = [g for g in self.groups if g]
+        self.groups = split_tiles([t.n for t in tiles], nstreams)
         self.where = {}  # tile index -> (solve, index within it)
         self.ctxs = []
         try:
