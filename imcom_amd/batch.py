@@ -11,12 +11,13 @@ the first solve takes the few largest tiles, the last the many small ones.
 Measured on the 64-tile workload (n from 1024 to 5766): one solve 962 ms;
 tiles dealt round-robin by size over two solves 905 ms, three 872 ms, four
 986 ms; three runs of equal n^3 work (4, 9 and 51 tiles) 829 ms, against
-893 ms for three runs of equal tile count.  The streams share the process's
-hardware queues (four by default): with four solves, or with another
-context still open that was created before these (its stream took a queue
-first), two solves end up on one queue and the gain is lost (1055 ms
-measured with one idle context opened first; contexts already closed do no
-harm).  Open a StreamedBatch before any context that stays open beside it.
+893 ms for three runs of equal tile count; four and six runs of equal work
+1004 and 1015 ms, also with eight hardware queues per process
+(GPU_MAX_HW_QUEUES=8), so three is the default.  A context opened before
+these and still open beside them (it was idle) made the three solves take
+1055 ms, probably by sharing a hardware queue with one of them; contexts
+already closed did no harm.  Open a StreamedBatch before any context that
+stays open beside it.
 """
 from __future__ import annotations
 
