@@ -485,12 +485,6 @@ int imcom_gpu_build_image(imcom_gpu_ctx* ctx, const double* I_i, double* H_out) 
   return IMCOM_OK;
 }
 
-int imcom_gpu_set_gemm_library(int lib) {
-  if (lib != 0 && lib != 1) return IMCOM_EARG;
-  set_gemm_library(lib);
-  return IMCOM_OK;
-}
-
 int imcom_gpu_set_profiling(imcom_gpu_ctx* ctx, int level) {
   if (!ctx) return IMCOM_EARG;
   NEED(level >= 0 && level <= 2, IMCOM_EARG, "profiling level must be 0, 1 or 2");

@@ -33,12 +33,16 @@ struct BatchState {
   int nb = 32, nbb = 128;
   hipGraphExec_t g1 = nullptr, g2 = nullptr;
   int g2_key = -1;
+  // split-precision workspace address baked into g2's products at capture;
+  // the single-tile entry points may reallocate that buffer in between
+  const void* g2_split = nullptr;
   ~BatchState() { clear(); }
   void drop_graphs() {
     if (g1) (void)hipGraphExecDestroy(g1);
     if (g2) (void)hipGraphExecDestroy(g2);
     g1 = g2 = nullptr;
     g2_key = -1;
+    g2_split = nullptr;
   }
   void clear() {
     drop_graphs();
@@ -278,18 +282,17 @@ int imcom_gpu_batch_solve(imcom_gpu_ctx* ctx, int precision_P, int precision_T, 
       if (precision_P) wb = std::max(wb, gemm_split_work_bytes(precision_P, n, m, n));
       if (precision_T) wb = std::max(wb, gemm_split_work_bytes(precision_T, n, m, n));
     }
-    if (ctx->split_work.cap < wb) {
-      if (B->g2) (void)hipGraphExecDestroy(B->g2);
-      B->g2 = nullptr;  // the graph holds the old workspace address
-      CK(ctx->split_work.reserve(wb), "alloc split workspace");
-    }
+    if (ctx->split_work.cap < wb) CK(ctx->split_work.reserve(wb), "alloc split workspace");
   }
   const int key = (learned ? 1 + (int)(zsig * 1000.0f) : 0) * 16 + precision_P * 4 + precision_T;
-  if (B->g2_key != key && B->g2) {
+  // re-capture when the controls change or when the split workspace moved
+  // (grown here, or reallocated by imcom_gpu_build_P / _build_T since)
+  if (B->g2 && (B->g2_key != key || B->g2_split != ctx->split_work.p)) {
     (void)hipGraphExecDestroy(B->g2);
     B->g2 = nullptr;
   }
   B->g2_key = key;
+  B->g2_split = ctx->split_work.p;
   {
     StageTimer tm(ctx, PS_SYTRD);
     if (use_graph)

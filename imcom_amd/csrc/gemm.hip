@@ -23,9 +23,6 @@
 #include "kernels.h"
 
 #include <algorithm>
-#include <atomic>
-
-#include <rocblas/rocblas.h>
 
 namespace imcom {
 
@@ -289,50 +286,10 @@ static hipError_t launch_sz(hipStream_t st, bool ta, bool tb, dim3 grid, const G
   return hipGetLastError();
 }
 
-// Large plain products go to rocBLAS (process-wide choice, see
-// set_gemm_library): on the path's shapes its FP64 kernels run at 74-77
-// TF/s against 60 for the kernel above, with bitwise the same results
-// (measured at n = 16384).  Products inside a stream capture, grouped, split-K
-// and block-layout products always use the kernels in this file.
-static std::atomic<int> g_gemm_library{1};
-
-void set_gemm_library(int lib) { g_gemm_library.store(lib); }
-int gemm_library() { return g_gemm_library.load(); }
-
-static rocblas_handle blas_handle(int dev) {
-  thread_local rocblas_handle handles[64] = {};
-  if (dev < 0 || dev >= 64) return nullptr;
-  if (!handles[dev] && rocblas_create_handle(&handles[dev]) != rocblas_status_success)
-    handles[dev] = nullptr;
-  return handles[dev];
-}
-
-// true: done by rocBLAS (status in *err); false: use the built-in kernel
-static bool dgemm_library(hipStream_t st, bool ta, bool tb, int M, int N, int K, double alpha,
-                          const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
-                          double* C, int64_t ldc, hipError_t* err) {
-  if (g_gemm_library.load() != 1 || 2.0 * M * (double)N * K < 1e9) return false;
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  if (hipStreamIsCapturing(st, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return false;
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return false;
-  rocblas_handle h = blas_handle(dev);
-  if (!h || rocblas_set_stream(h, st) != rocblas_status_success) return false;
-  const rocblas_status rs =
-      rocblas_dgemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
-                    tb ? rocblas_operation_transpose : rocblas_operation_none, M, N, K, &alpha, A,
-                    (rocblas_int)lda, B, (rocblas_int)ldb, &beta, C, (rocblas_int)ldc);
-  *err = rs == rocblas_status_success ? hipSuccess : hipErrorLaunchFailure;
-  return true;
-}
-
 hipError_t dgemm(hipStream_t st, bool ta, bool tb, int M, int N, int K, double alpha,
                  const double* A, int64_t lda, const double* B, int64_t ldb, double beta,
                  double* C, int64_t ldc) {
   if (M <= 0 || N <= 0) return hipSuccess;
-  hipError_t lib_err = hipSuccess;
-  if (dgemm_library(st, ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, &lib_err))
-    return lib_err;
   GemmDesc g{M, N, K, alpha, beta, A, lda, B, ldb, C, ldc};
   // 128x128 tiles when that still gives >= 2 tiles per CU, else 64x64
   const int64_t t128 = (int64_t)((M + 127) / 128) * ((N + 127) / 128);

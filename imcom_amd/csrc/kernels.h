@@ -92,10 +92,6 @@ hipError_t dgemm_lower(hipStream_t st, bool ta, bool tb, int M, int K, double al
                        double* C, int64_t ldc, double* Ct = nullptr, int64_t ct_off = 0,
                        int ct_nbt = 0, int cm_cols = 0);
 int gemm_tiles64(int M, int N);
-// Which implementation dgemm uses for large plain products outside stream
-// captures: 1 = rocBLAS (default), 0 = the built-in kernel.  Process-wide.
-void set_gemm_library(int lib);
-int gemm_library();
 // C (M x N, ldc) = alpha * op(A) op(B) with K split over workgroups; the
 // partial products go to `work` (ksplit * M * N doubles, from
 // dgemm_splitk_work) and are summed in a fixed order.

@@ -27,7 +27,6 @@ IMCOM_PREC_BF16X6 = 2
 # every symbol declared in include/imcom_gpu.h
 EXPORTS = [
     "imcom_gpu_init", "imcom_gpu_finalize", "imcom_gpu_last_error", "imcom_gpu_synchronize",
-    "imcom_gpu_set_gemm_library",
     "imcom_gpu_set_tile", "imcom_gpu_set_tables", "imcom_gpu_assemble_inputs",
     "imcom_gpu_rotate_image", "imcom_gpu_ft_r2c", "imcom_gpu_build_Alookup",
     "imcom_gpu_build_Blookup", "imcom_gpu_calc_C", "imcom_gpu_lookup_A", "imcom_gpu_set_A",
@@ -129,7 +128,6 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_batch_solve": (_int, [_vp, _int, _int, _int, ctypes.c_float, _int]),
         "imcom_gpu_batch_result": (_int, [_vp, _int, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
         "imcom_gpu_set_tuning": (_int, [_vp, _int, _int]),
-        "imcom_gpu_set_gemm_library": (_int, [_int]),
         "imcom_gpu_assemble_inputs": (_int, [_vp, _int, _int, _int, _dp, _dbl, _dp, _dp, _dp,
                                               ctypes.POINTER(_i64), _dp, _dp, _ip, _dp]),
         "imcom_gpu_rotate_image": (_int, [_vp, _int, _int, _dp, _dbl, _int, _int, _dp]),
@@ -148,14 +146,6 @@ def load_library(path: str = LIB_PATH):
         f.argtypes = args
     _LIB = L
     return L
-
-
-def set_gemm_library(lib: int):
-    """Large plain FP64 products on rocBLAS (1, the default) or on the
-    library's own MFMA kernel (0); process-wide."""
-    rc = load_library().imcom_gpu_set_gemm_library(int(lib))
-    if rc != 0:
-        raise ImcomError(f"set_gemm_library({lib}) failed (code {rc})")
 
 
 def host_empty(shape, order="F") -> np.ndarray:

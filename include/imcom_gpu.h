@@ -245,12 +245,6 @@ int imcom_gpu_batch_result(imcom_gpu_ctx* ctx, int tile, double* K_out, double* 
  *   11 device lookup-table set-up. */
 int imcom_gpu_set_profiling(imcom_gpu_ctx* ctx, int level);
 
-/* Implementation of the large plain FP64 products (P, T, the back-transform's
- * updates, the divide-and-conquer merges): 1 = rocBLAS (default), 0 = the
- * library's own MFMA kernel.  Process-wide; no reference counterpart (the
- * reference links whichever BLAS it is built with).  The two give bitwise the
- * same results on the path's shapes. */
-int imcom_gpu_set_gemm_library(int lib);
 int imcom_gpu_stage_times(imcom_gpu_ctx* ctx, double* ms_out, int64_t* counts_out, int nslots);
 
 /* Stage files written while the device works on.  The reference writes A, Q,
