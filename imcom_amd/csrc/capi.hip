@@ -47,6 +47,8 @@ int imcom_gpu_init(imcom_gpu_ctx** out, int device) {
   if (device < 0 || device >= ndev) return IMCOM_EARG;
   e = hipSetDevice(device);
   if (e != hipSuccess) return IMCOM_EHIP;
+  e = noalloc_scratch_setup(device);
+  if (e != hipSuccess) return IMCOM_EHIP;
   imcom_gpu_ctx* ctx = new imcom_gpu_ctx();
   ctx->device = device;
   e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);

@@ -206,6 +206,10 @@ inline hipError_t fetch_wait_all(imcom_gpu_ctx* ctx) {
 
 void imcom_batch_free(imcom_gpu_ctx* ctx);
 
+// capi_noalloc.hip: records this device's address of the no-allocation entry
+// point's scratch array (called by imcom_gpu_init, device current)
+hipError_t noalloc_scratch_setup(int device);
+
 // stage timing helper: events recorded only when profiling is on
 struct StageTimer {
   imcom_gpu_ctx* c;

@@ -39,7 +39,7 @@ EXPORTS = [
     "imcom_gpu_mlp_grad", "imcom_gpu_mlp_train", "imcom_gpu_bisect_kappa_learned",
     "imcom_gpu_batch_set", "imcom_gpu_batch_solve", "imcom_gpu_batch_result",
     "imcom_gpu_set_tuning", "imcom_gpu_fetch_async", "imcom_gpu_fetch_wait",
-    "imcom_gpu_host_alloc", "imcom_gpu_host_free",
+    "imcom_gpu_host_alloc", "imcom_gpu_host_free", "imcom_gpu_bisect_kappa_learned_noalloc",
 ]
 
 # stage outputs of fetch_async (IMCOM_FETCH_*)
@@ -139,6 +139,10 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_fetch_wait": (_int, [_vp, _int]),
         "imcom_gpu_host_alloc": (_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
         "imcom_gpu_host_free": (_int, [_vp]),
+        "imcom_gpu_bisect_kappa_learned_noalloc": (_int, [_vp, _vp, _i64, _i64, _vp, _vp, _i64, _vp,
+                                                          _dbl, _vp, _int, _dbl, _dbl, _dbl, _dbl,
+                                                          _int, ctypes.c_float, _vp, _vp, _vp,
+                                                          _vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -555,6 +559,19 @@ class ImcomGPU:
                                                          _ptr(ev, _ip), _ptr(hits, _ip)),
                  "bisect_kappa_learned")
         return K, USa, ev, hits
+
+    def bisect_kappa_learned_noalloc(self, n, m, L, P, ldp, Ndiag, C_a, params, K, US,
+                                     evals=0, hits=0, stream=0, zsig=3.0, UnotS=True,
+                                     US_max=1e-6, k_min=None, k_max=None, tol=1e-9, maxnits=53):
+        """imcom_gpu_bisect_kappa_learned_noalloc: every array argument is a
+        device address (int), as from torch's data_ptr(); the call only
+        enqueues the work (on `stream`, 0 for the context's stream)."""
+        k_min = C_a * 2.0 ** -53 if k_min is None else k_min
+        k_max = C_a * 2.0 ** 53 if k_max is None else k_max
+        self._ck(self.lib.imcom_gpu_bisect_kappa_learned_noalloc(
+            self.h, stream or None, n, m, L, P, ldp, Ndiag or None, C_a, params,
+            1 if UnotS else 0, US_max, k_min, k_max, tol, maxnits, zsig, K, US, evals or None,
+            hits or None), "bisect_kappa_learned_noalloc")
 
     def synchronize(self):
         self._ck(self.lib.imcom_gpu_synchronize(self.h), "synchronize")

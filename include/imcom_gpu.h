@@ -198,6 +198,27 @@ int imcom_gpu_bisect_kappa_learned(imcom_gpu_ctx* ctx, int UnotS, double US_max,
                                    double* K_out, double* US_out, int32_t* evals_out,
                                    int32_t* hits_out);
 
+/* imcom_gpu_bisect_kappa_learned for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Every pointer is a
+ * device pointer: L (n, descending), P (n x m, leading dimension ldp >= n),
+ * Ndiag (n; only read in S mode, may be NULL in U mode), params
+ * (imcom_gpu_mlp_nparams() floats, as imcom_gpu_mlp_set_params takes them,
+ * not validated here), outputs K and US (m), evals and hits (m, may be NULL).
+ * stream: a hipStream_t, or NULL for the context's stream; the call only
+ * enqueues work and returns.  The kernels' scratch (per-pixel features and
+ * brackets, eigenvalue bins) is a fixed device array of 2.16 MiB per GPU,
+ * held until the process ends, which limits n <= 32768 and m <= 16384
+ * (larger sizes return IMCOM_EARG).  Calls that overlap on one GPU share that
+ * array and race.  Nothing is allocated: no hipMalloc, hipFree or allocating
+ * library routine. */
+int imcom_gpu_bisect_kappa_learned_noalloc(imcom_gpu_ctx* ctx, void* stream, int64_t n, int64_t m,
+                                           const double* L, const double* P, int64_t ldp,
+                                           const double* Ndiag, double C_a, const float* params,
+                                           int UnotS, double US_max, double k_min, double k_max,
+                                           double tol, int maxnits, float zsig, double* K,
+                                           double* US, int32_t* evals, int32_t* hits);
+
 /* Batched tiles.  No reference counterpart (the reference solves one tile per
  * process); each tile's result is what the single-tile entry points give for
  * it.  A tile carries its own inputs (the fields of imcom_gpu_set_tile and
