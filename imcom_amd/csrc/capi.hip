@@ -446,6 +446,17 @@ int imcom_gpu_build_T(imcom_gpu_ctx* ctx, int precision, double* T_out) {
   return IMCOM_OK;
 }
 
+int imcom_gpu_set_T(imcom_gpu_ctx* ctx, const double* T) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_tile && T, IMCOM_EARG, "set_T: tile not set or NULL T");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_T), "set_T: wait for fetches");
+  CK(upload(ctx, ctx->T, T, (size_t)ctx->n * ctx->m * 8), "upload T");
+  CK(hipStreamSynchronize(ctx->stream), "set_T sync");
+  ctx->have_T = true;
+  return IMCOM_OK;
+}
+
 int imcom_gpu_build_S(imcom_gpu_ctx* ctx, double* S_out) {
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_T, IMCOM_ESTATE, "build_S: T not built");

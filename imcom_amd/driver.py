@@ -12,8 +12,10 @@ with the lookup tables built on the device) and the solve path through
 ImcomGPU, and writes the reference's outputs: H (OUTFILE), K, T, U and S.
 The A, B, Q/L and P stage files are read back when they exist and forceSys
 is 0 and written when their name is not None (imcom_get_A/QL/B/P,
-imcom_matrix.f90:41-202).  T is always solved (an existing TFILE is
-overwritten), i.e. forceT is taken as 1.
+imcom_matrix.f90:41-202).  An existing TFILE with forceT = 0 is read back
+instead of solved (imcom_get_T, imcom_bisect.f90:47-64): then only H is
+formed and written, and the system (A, Q/L, B, P) is not built unless
+forceSys is set (imcom.f90:146-160), as in the reference.
 
 The second form solves several tiles (one configuration file each) as one
 batched launch sequence (imcom_gpu_batch_*) and writes every tile's outputs;
@@ -247,66 +249,86 @@ def run(config: str, mode: str, US_max: float, US_tol: float, forceT: int = 0, f
             w.put(item[1], item[2].reshape(item[3], order="F"))
             inflight.remove(item)
 
+    Texists = os.path.exists(cfg.Tfile)
+    read_T = Texists and forceT == 0                      # imcom_get_T's cache hit
+    build_system = forceSys != 0 or forceT != 0 or not Texists   # imcom.f90:151
+    K = U = S = ev = None
     try:
         with ImcomGPU(device) as g:
-            g.set_tile(t, tables="device")
-            # imcom_get_A / imcom_get_QL / imcom_get_B / imcom_get_P; each
-            # stage file is copied out on a second stream while the next
-            # stages run
-            if _cached(sf["A"], forceSys):
-                log(f"IMCOM: Reading A matrix from {sf['A']}")
-                g.set_A(_sized(fits.readfits(sf["A"]), (n, n), sf["A"]))
+            g.set_tile(t, tables="device" if build_system else "none")
+            if build_system:
+                _system(g, t, sf, forceSys, fetch, hand_over, log)
+            if read_T:
+                log(f"IMCOM: Reading T matrix from {cfg.Tfile}")
+                T = _sized(fits.readfits(cfg.Tfile), (n, m), cfg.Tfile)
+                g.set_T(T)
+                H = g.build_image()
+                hand_over(g, "A", "Q", "L", "B", "P")
             else:
-                g.lookup_A(t.npoly, t.npad)
-                if sf["A"] != "None":
-                    fetch(g, "A", sf["A"], (n, n))
-            if _cached(sf["Q"], forceSys) and _cached(sf["L"], forceSys):
-                log(f"IMCOM: Reading Q, L from {sf['Q']}, {sf['L']}")
-                g.set_QL(_sized(fits.readfits(sf["Q"]), (n, n), sf["Q"]),
-                         _sized(fits.readfits(sf["L"]), (n, 1), sf["L"]).ravel(order="F"))
-            else:
-                _, _, cond = g.eigen_A(out_Q=False, out_L=False)
-                log(f"IMCOM: A matrix condition number = {cond:24.16E}")
-                if sf["Q"] != "None" and sf["L"] != "None":
-                    fetch(g, "Q", sf["Q"], (n, n))
-                    fetch(g, "L", sf["L"], (n, 1))
-            hand_over(g, "A")  # the reduction waited for this copy before overwriting A
-            if _cached(sf["B"], forceSys):
-                log(f"IMCOM: Reading B matrix from {sf['B']}")
-                g.set_B(_sized(fits.readfits(sf["B"]), (n, m), sf["B"]))
-            else:
-                g.lookup_B(t.npoly, t.npad)
-                if sf["B"] != "None":
-                    fetch(g, "B", sf["B"], (n, m))
-            if _cached(sf["P"], forceSys):
-                log(f"IMCOM: Reading P matrix from {sf['P']}")
-                g.set_P(_sized(fits.readfits(sf["P"]), (n, m), sf["P"]))
-            else:
-                g.build_P()
-                if sf["P"] != "None":
-                    fetch(g, "P", sf["P"], (n, m))
-            # imcom_get_T (always solved) and imcom_build_image
-            K, US, ev = g.bisect_kappa()
-            hand_over(g, "Q", "L", "B")
-            g.build_T(out=False)
-            T = g.fetch_async("T")
-            if USB:
-                U, S = US, g.build_S()
-            else:
-                S, U = US, g.build_U()
-            H = g.build_image()
-            hand_over(g, "P")
-            g.fetch_wait("T")
+                K, US, ev = g.bisect_kappa()
+                hand_over(g, "Q", "L", "B")
+                g.build_T(out=False)
+                T = g.fetch_async("T")
+                if USB:
+                    U, S = US, g.build_S()
+                else:
+                    S, U = US, g.build_U()
+                H = g.build_image()
+                hand_over(g, "A", "P")
+                g.fetch_wait("T")
         grid = (t.n1out, t.n2out)
-        w.put(cfg.Kfile, K.reshape(grid, order="F"))
-        w.put(cfg.Ufile, U.reshape(grid, order="F"))
-        w.put(cfg.Sfile, S.reshape(grid, order="F"))
-        w.put(cfg.Tfile, T)
+        if not read_T:
+            w.put(cfg.Kfile, K.reshape(grid, order="F"))
+            w.put(cfg.Ufile, U.reshape(grid, order="F"))
+            w.put(cfg.Sfile, S.reshape(grid, order="F"))
+            w.put(cfg.Tfile, T)
         w.put(cfg.Hfile, H.reshape(grid, order="F"))
     finally:
         w.close()
-    log(f"IMCOM: wrote {cfg.Hfile}, {cfg.Kfile}, {cfg.Ufile}, {cfg.Sfile}, {cfg.Tfile}")
-    return dict(K=K, U=U, S=S, T=T, H=H, evals=ev, tile=t)
+    if read_T:
+        log(f"IMCOM: wrote {cfg.Hfile} (T read from {cfg.Tfile})")
+    else:
+        log(f"IMCOM: wrote {cfg.Hfile}, {cfg.Kfile}, {cfg.Ufile}, {cfg.Sfile}, {cfg.Tfile}")
+    return dict(K=K, U=U, S=S, T=T, H=H, evals=ev, tile=t, T_read=read_T)
+
+
+def _system(g, t, sf, forceSys, fetch, hand_over, log):
+    """imcom_get_A / imcom_get_QL / imcom_get_B / imcom_get_P
+    (imcom_matrix.f90:41-202); each stage file is copied out on a second
+    stream while the next stages run."""
+    n, m = t.n, t.m
+    if _cached(sf["A"], forceSys):
+        log(f"IMCOM: Reading A matrix from {sf['A']}")
+        g.set_A(_sized(fits.readfits(sf["A"]), (n, n), sf["A"]))
+    else:
+        g.lookup_A(t.npoly, t.npad)
+        if sf["A"] != "None":
+            fetch(g, "A", sf["A"], (n, n))
+    if _cached(sf["Q"], forceSys) and _cached(sf["L"], forceSys):
+        log(f"IMCOM: Reading Q, L from {sf['Q']}, {sf['L']}")
+        g.set_QL(_sized(fits.readfits(sf["Q"]), (n, n), sf["Q"]),
+                 _sized(fits.readfits(sf["L"]), (n, 1), sf["L"]).ravel(order="F"))
+    else:
+        _, _, cond = g.eigen_A(out_Q=False, out_L=False)
+        log(f"IMCOM: A matrix condition number = {cond:24.16E}")
+        if sf["Q"] != "None" and sf["L"] != "None":
+            fetch(g, "Q", sf["Q"], (n, n))
+            fetch(g, "L", sf["L"], (n, 1))
+    hand_over(g, "A")  # the reduction waited for this copy before overwriting A
+    if _cached(sf["B"], forceSys):
+        log(f"IMCOM: Reading B matrix from {sf['B']}")
+        g.set_B(_sized(fits.readfits(sf["B"]), (n, m), sf["B"]))
+    else:
+        g.lookup_B(t.npoly, t.npad)
+        if sf["B"] != "None":
+            fetch(g, "B", sf["B"], (n, m))
+    if _cached(sf["P"], forceSys):
+        log(f"IMCOM: Reading P matrix from {sf['P']}")
+        g.set_P(_sized(fits.readfits(sf["P"]), (n, m), sf["P"]))
+    else:
+        g.build_P()
+        if sf["P"] != "None":
+            fetch(g, "P", sf["P"], (n, m))
 
 
 class _InDir:

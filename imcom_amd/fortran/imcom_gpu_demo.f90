@@ -21,8 +21,8 @@ PROGRAM imcom_gpu_demo
   integer(c_int32_t) :: psfconst, nexppsf, n1, n2, npoly, npad, maxnits, unots
   real(c_double) :: C_a, kmin, kmax, usmax, ustol, sx, sy, cond
   real(c_double), allocatable :: x(:), y(:), Xa(:), Ya(:), Nd(:), Ii(:), Atab(:), Btab(:)
-  real(c_double), allocatable :: K(:), US(:), SU(:), H(:), Q(:), L(:), A(:), B(:), P(:), T(:)
-  integer(c_int32_t), allocatable :: ex(:), ev(:)
+  real(c_double), allocatable :: K(:), US(:), SU(:), H(:), L(:)
+  integer(c_int32_t), allocatable :: ex(:)
   integer(c_int) :: st
   integer :: nA, nB
   character(len=1024) :: fin, fout
@@ -48,23 +48,23 @@ PROGRAM imcom_gpu_demo
   call imcom_gpu_check(ctx, imcom_gpu_set_tile(ctx, n, m, x, y, ex, Xa, Ya, Nd, C_a), "set_tile")
   call imcom_gpu_check(ctx, imcom_gpu_set_tables(ctx, int(psfconst, c_int), int(nexppsf, c_int), &
                        int(n1, c_int), int(n2, c_int), Atab, Btab), "set_tables")
-  allocate(A(n * n), Q(n * n), L(n), B(n * m), P(n * m), T(n * m))
-  allocate(K(m), US(m), SU(m), H(m), ev(m))
-  ! imcom_get_A -> imcom_lookup_A (imcom_matrix.f90:206)
-  call imcom_gpu_check(ctx, imcom_gpu_lookup_A(ctx, int(npoly, c_int), int(npad, c_int), sx, sy, A), &
-                       "lookup_A")
-  ! imcom_get_QL -> imcom_eigen_A (imcom_matrix.f90:586)
-  call imcom_gpu_check(ctx, imcom_gpu_eigen_A(ctx, Q, L, cond), "eigen_A")
+  allocate(L(n))
+  allocate(K(m), US(m), SU(m), H(m))
+  ! imcom_get_A -> imcom_lookup_A (imcom_matrix.f90:206); A stays on the device
+  call imcom_gpu_check(ctx, imcom_gpu_lookup_A(ctx, int(npoly, c_int), int(npad, c_int), sx, sy, &
+                       c_null_ptr), "lookup_A")
+  ! imcom_get_QL -> imcom_eigen_A (imcom_matrix.f90:586); Q stays on the device
+  call imcom_gpu_check(ctx, imcom_gpu_eigen_A(ctx, c_null_ptr, imcom_gpu_addr(L), cond), "eigen_A")
   write(*, FMT='(A, E24.16)') "IMCOM: A matrix condition number = ", cond
   ! imcom_get_B -> imcom_lookup_B (imcom_matrix.f90:288)
-  call imcom_gpu_check(ctx, imcom_gpu_lookup_B(ctx, int(npoly, c_int), int(npad, c_int), sx, sy, B), &
-                       "lookup_B")
+  call imcom_gpu_check(ctx, imcom_gpu_lookup_B(ctx, int(npoly, c_int), int(npad, c_int), sx, sy, &
+                       c_null_ptr), "lookup_B")
   ! imcom_get_P -> imcom_build_P (imcom_matrix.f90:682)
-  call imcom_gpu_check(ctx, imcom_gpu_build_P(ctx, IMCOM_PREC_FP64, P), "build_P")
+  call imcom_gpu_check(ctx, imcom_gpu_build_P(ctx, IMCOM_PREC_FP64, c_null_ptr), "build_P")
   ! imcom_get_T -> imcom_bisect_kappa + imcom_build_T (imcom_bisect.f90:40-74)
   call imcom_gpu_check(ctx, imcom_gpu_bisect_kappa(ctx, int(unots, c_int), usmax, kmin, kmax, ustol, &
-                       int(maxnits, c_int), c_null_ptr, K, US, ev), "bisect_kappa")
-  call imcom_gpu_check(ctx, imcom_gpu_build_T(ctx, IMCOM_PREC_FP64, T), "build_T")
+                       int(maxnits, c_int), c_null_ptr, K, US, c_null_ptr), "bisect_kappa")
+  call imcom_gpu_check(ctx, imcom_gpu_build_T(ctx, IMCOM_PREC_FP64, c_null_ptr), "build_T")
   if (unots.ne.0) then
     call imcom_gpu_check(ctx, imcom_gpu_build_S(ctx, SU), "build_S")
   else

@@ -1,11 +1,14 @@
 ! ISO_C_BINDING interface to libimcom_hip.so (include/imcom_gpu.h) for the
-! reference's Fortran driver.  A replacement imcom_matrix / imcom_bisect pair
-! calls these from the bodies of imcom_lookup_A, imcom_eigen_A,
-! imcom_lookup_B, imcom_build_P, imcom_bisect_kappa, imcom_build_T/S/U and
-! imcom_build_image, keeping every public procedure name and argument, so
-! imcom.f90 compiles unchanged against them.  Sizes are int64, arrays are
-! passed as assumed-size real(c_double) (Fortran column-major, as the C side
-! expects); a C status other than 0 is reported as "IMCOM ERROR: <message>"
+! reference's Fortran driver.  The replacement imcom_matrix / imcom_bisect
+! modules (imcom_matrix_gpu.f90, imcom_bisect_gpu.f90) call these from the
+! bodies of imcom_get_A/B/QL/P, imcom_lookup_A/B, imcom_eigen_A, imcom_build_P,
+! imcom_get_T, imcom_bisect_kappa, imcom_build_T/S/U and imcom_build_image,
+! keeping every public procedure name and argument, so imcom.f90 compiles
+! unchanged against them.  Sizes are int64, inputs are assumed-size
+! real(c_double) (Fortran column-major, as the C side expects).  A stage
+! output that may stay on the device is a type(c_ptr), value argument: pass
+! imcom_gpu_addr(array) to receive it or c_null_ptr to leave it on the
+! device.  A C status other than 0 is reported as "IMCOM ERROR: <message>"
 ! followed by STOP, as the reference does.
 MODULE imcom_gpu_iface
   USE, INTRINSIC :: iso_c_binding
@@ -83,7 +86,7 @@ MODULE imcom_gpu_iface
       type(c_ptr), value :: ctx
       integer(c_int), value :: psfconst, nexppsf, n1psf, n2psf, npad
       complex(c_double_complex), intent(in) :: Gt_rot(*)
-      real(c_double), intent(out) :: Alookup_out(*)
+      type(c_ptr), value :: Alookup_out
     end function
     integer(c_int) function imcom_gpu_build_Blookup(ctx, psfconst, nexppsf, n1psf, n2psf, npad, &
         Gammat, Gt_rot, Blookup_out) bind(C, name="imcom_gpu_build_Blookup")
@@ -91,7 +94,7 @@ MODULE imcom_gpu_iface
       type(c_ptr), value :: ctx
       integer(c_int), value :: psfconst, nexppsf, n1psf, n2psf, npad
       complex(c_double_complex), intent(in) :: Gammat(*), Gt_rot(*)
-      real(c_double), intent(out) :: Blookup_out(*)
+      type(c_ptr), value :: Blookup_out
     end function
     integer(c_int) function imcom_gpu_calc_C(ctx, n1psf, n2psf, Gammat, C_out) &
         bind(C, name="imcom_gpu_calc_C")
@@ -107,7 +110,7 @@ MODULE imcom_gpu_iface
       type(c_ptr), value :: ctx
       integer(c_int), value :: npoly, npad
       real(c_double), value :: psfxscale, psfyscale
-      real(c_double), intent(out) :: A_out(*)
+      type(c_ptr), value :: A_out
     end function
     integer(c_int) function imcom_gpu_set_A(ctx, A) bind(C, name="imcom_gpu_set_A")
       import :: c_int, c_ptr, c_double
@@ -118,7 +121,7 @@ MODULE imcom_gpu_iface
         bind(C, name="imcom_gpu_eigen_A")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
-      real(c_double), intent(out) :: Q_out(*), L_out(*)
+      type(c_ptr), value :: Q_out, L_out
       real(c_double), intent(out) :: condition
     end function
     integer(c_int) function imcom_gpu_set_QL(ctx, Q, L) bind(C, name="imcom_gpu_set_QL")
@@ -132,7 +135,7 @@ MODULE imcom_gpu_iface
       type(c_ptr), value :: ctx
       integer(c_int), value :: npoly, npad
       real(c_double), value :: psfxscale, psfyscale
-      real(c_double), intent(out) :: B_out(*)
+      type(c_ptr), value :: B_out
     end function
     integer(c_int) function imcom_gpu_set_B(ctx, B) bind(C, name="imcom_gpu_set_B")
       import :: c_int, c_ptr, c_double
@@ -143,7 +146,7 @@ MODULE imcom_gpu_iface
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: precision
-      real(c_double), intent(out) :: P_out(*)
+      type(c_ptr), value :: P_out
     end function
     integer(c_int) function imcom_gpu_set_P(ctx, P) bind(C, name="imcom_gpu_set_P")
       import :: c_int, c_ptr, c_double
@@ -158,13 +161,35 @@ MODULE imcom_gpu_iface
       real(c_double), value :: US_max, k_min, k_max, tol
       type(c_ptr), value :: bracket
       real(c_double), intent(out) :: K_out(*), US_out(*)
-      integer(c_int32_t), intent(out) :: evals_out(*)
+      type(c_ptr), value :: evals_out
+    end function
+    integer(c_int) function imcom_gpu_bisect_kappa_learned(ctx, UnotS, US_max, k_min, k_max, tol, &
+        maxnits, zsig, K_out, US_out, evals_out, hits_out) &
+        bind(C, name="imcom_gpu_bisect_kappa_learned")
+      import :: c_int, c_ptr, c_double, c_float
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: UnotS, maxnits
+      real(c_double), value :: US_max, k_min, k_max, tol
+      real(c_float), value :: zsig
+      real(c_double), intent(out) :: K_out(*), US_out(*)
+      type(c_ptr), value :: evals_out, hits_out
+    end function
+    integer(c_int) function imcom_gpu_mlp_set_params(ctx, params) &
+        bind(C, name="imcom_gpu_mlp_set_params")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: params(*)
     end function
     integer(c_int) function imcom_gpu_build_T(ctx, precision, T_out) bind(C, name="imcom_gpu_build_T")
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       integer(c_int), value :: precision
-      real(c_double), intent(out) :: T_out(*)
+      type(c_ptr), value :: T_out
+    end function
+    integer(c_int) function imcom_gpu_set_T(ctx, T) bind(C, name="imcom_gpu_set_T")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: T(*)
     end function
     integer(c_int) function imcom_gpu_build_S(ctx, S_out) bind(C, name="imcom_gpu_build_S")
       import :: c_int, c_ptr, c_double
@@ -198,6 +223,14 @@ MODULE imcom_gpu_iface
   end interface
 
 CONTAINS
+
+  ! Address of a contiguous real(c_double) array, for the type(c_ptr) stage
+  ! outputs above (the array itself need not carry TARGET).
+  FUNCTION imcom_gpu_addr(a) RESULT(p)
+    real(c_double), target, intent(inout) :: a(*)
+    type(c_ptr) :: p
+    p = c_loc(a(1))
+  END FUNCTION imcom_gpu_addr
 
   ! Reference-style error stop: print "IMCOM ERROR: <library message>" and stop.
   SUBROUTINE imcom_gpu_check(ctx, status, where)

@@ -32,7 +32,7 @@ EXPORTS = [
     "imcom_gpu_build_Blookup", "imcom_gpu_calc_C", "imcom_gpu_lookup_A", "imcom_gpu_set_A",
     "imcom_gpu_eigen_A", "imcom_gpu_tridiagonalize", "imcom_gpu_tridiag_eig",
     "imcom_gpu_set_QL", "imcom_gpu_lookup_B", "imcom_gpu_set_B", "imcom_gpu_build_P",
-    "imcom_gpu_set_P", "imcom_gpu_bisect_kappa", "imcom_gpu_build_T", "imcom_gpu_build_S",
+    "imcom_gpu_set_P", "imcom_gpu_set_T", "imcom_gpu_bisect_kappa", "imcom_gpu_build_T", "imcom_gpu_build_S",
     "imcom_gpu_build_U", "imcom_gpu_build_image", "imcom_gpu_set_profiling",
     "imcom_gpu_stage_times", "imcom_gpu_mlp_nparams", "imcom_gpu_mlp_set_params",
     "imcom_gpu_mlp_get_params", "imcom_gpu_kappa_features", "imcom_gpu_mlp_infer",
@@ -109,6 +109,7 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_set_P": (_int, [_vp, _dp]),
         "imcom_gpu_bisect_kappa": (_int, [_vp, _int, _dbl, _dbl, _dbl, _dbl, _int, _fp, _dp, _dp, _ip]),
         "imcom_gpu_build_T": (_int, [_vp, _int, _dp]),
+        "imcom_gpu_set_T": (_int, [_vp, _dp]),
         "imcom_gpu_build_S": (_int, [_vp, _dp]),
         "imcom_gpu_build_U": (_int, [_vp, _dp]),
         "imcom_gpu_build_image": (_int, [_vp, _dp, _dp]),
@@ -219,7 +220,8 @@ class ImcomGPU:
     def set_tile(self, tile, tables="host"):
         """Hand over the imcom_data globals of one tile.  tables="host"
         uploads the tile's Alookup/Blookup; "device" builds them on the GPU
-        from the tile's spectra Gt_rot and Gammat (build_Alookup/Blookup)."""
+        from the tile's spectra Gt_rot and Gammat (build_Alookup/Blookup);
+        "none" hands over no tables (a T read from TFILE needs none)."""
         self.n, self.m = tile.n, tile.m
         self._keep = [_vec(tile.x_i), _vec(tile.y_i), _vec(tile.exp_i, np.int32),
                       _vec(tile.X_a), _vec(tile.Y_a), _vec(tile.Ndiag)]
@@ -232,12 +234,14 @@ class ImcomGPU:
                 raise ImcomError("set_tile(tables='device') needs the tile's spectra")
             self.build_Alookup(tile.psfconst, tile.Gt_rot, tile.npad)
             self.build_Blookup(tile.psfconst, tile.Gammat, tile.Gt_rot, tile.npad)
+        elif tables == "none":
+            pass
         elif tables == "host":
             A, B = _mat(tile.Alookup), _mat(tile.Blookup)
             self._ck(self.lib.imcom_gpu_set_tables(self.h, tile.psfconst, tile.nexppsf, A.shape[0],
                                                    A.shape[1], _ptr(A), _ptr(B)), "set_tables")
         else:
-            raise ValueError("tables must be 'host' or 'device'")
+            raise ValueError("tables must be 'host', 'device' or 'none'")
         self.tile = tile
 
     # ---- lookup-table set-up (imcom_proc.f90:184-250,433-475, imcom_matrix.f90:361-552)
@@ -425,6 +429,11 @@ class ImcomGPU:
         T = np.empty((self.n, self.m), order="F") if out else None
         self._ck(self.lib.imcom_gpu_build_T(self.h, precision, _ptr(T)), "build_T")
         return T
+
+    def set_T(self, T):
+        """imcom_get_T cache hit: T read from TFILE (imcom_bisect.f90:47-62)."""
+        T = _mat(T)
+        self._ck(self.lib.imcom_gpu_set_T(self.h, _ptr(T)), "set_T")
 
     def build_S(self):
         S = np.empty(self.m)

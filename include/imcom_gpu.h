@@ -165,6 +165,9 @@ int imcom_gpu_bisect_kappa(imcom_gpu_ctx* ctx, int UnotS, double US_max, double 
 /* imcom_build_T  (imcom_matrix.f90:716-750): T = -1/2 Q (P / (L + K_a));
  * T_out (n*m) may be NULL. */
 int imcom_gpu_build_T(imcom_gpu_ctx* ctx, int precision, double* T_out);
+/* imcom_get_T cache hit: T read from TFILE instead of re-solved when forceT
+ * is 0 (imcom_bisect.f90:47-62); build_image and build_S then use it. */
+int imcom_gpu_set_T(imcom_gpu_ctx* ctx, const double* T);
 /* imcom_build_S  (imcom_matrix.f90:839-877): S_a = sum_i T_ia^2 Ndiag(i). */
 int imcom_gpu_build_S(imcom_gpu_ctx* ctx, double* S_out);
 /* imcom_build_U  (imcom_matrix.f90:795-835): U_a = U(K_a). */

@@ -147,13 +147,29 @@ def test_driver_example_matches_oracle(tmp_path, monkeypatch):
     assert np.all(out["U"][flip] < Umax) and np.all(ref["U"][flip] < Umax)
     assert np.all(dT <= t_tol)
     assert H_eq <= 1e-8
-    # a second run reads A, Q, L, B and P back from the stage files it wrote
-    # (imcom_get_A/QL/B/P cache hits) and reproduces the first run exactly
+    # a second run with forceT = 1 solves T again, reading A, Q, L, B and P
+    # back from the stage files it wrote (imcom_get_A/QL/B/P cache hits), and
+    # reproduces the first run exactly
     logs = []
-    again = D.run("config_example", "U", 1e-6, 1e-9, log=logs.append)
+    again = D.run("config_example", "U", 1e-6, 1e-9, forceT=1, log=logs.append)
     assert sum("Reading" in s for s in logs) == 4
+    assert not again["T_read"]
     for k in ("K", "U", "S", "H"):
         assert np.array_equal(again[k], out[k]), k
+    # a third run with forceT = 0 finds TFILE: imcom_get_T reads T instead of
+    # solving and the system is not built (imcom.f90:151, imcom_bisect.f90:47-64);
+    # only H is formed and written, equal to the first run's bit for bit
+    stamps = {f: os.stat(f"{f}.example.fits").st_mtime_ns for f in "KUST"}
+    os.remove("H.example.fits")
+    logs = []
+    third = D.run("config_example", "U", 1e-6, 1e-9, log=logs.append)
+    assert third["T_read"] and third["K"] is None
+    assert any("Reading T matrix" in s for s in logs)
+    assert not any("Reading A" in s or "Reading Q" in s for s in logs)
+    assert np.array_equal(third["H"], out["H"])
+    assert np.array_equal(fits.readfits("H.example.fits").ravel(order="F"), out["H"])
+    for f, t0 in stamps.items():
+        assert os.stat(f"{f}.example.fits").st_mtime_ns == t0, f
 
 
 @pytest.mark.gpu

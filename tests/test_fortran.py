@@ -1,7 +1,11 @@
-"""The Fortran side of the C ABI: the ISO_C_BINDING module and a driver that
-runs the reference's get_A, get_QL, get_B, get_P, get_T, build_image
-sequence (imcom.f90:151-169) through it, checked against the oracle."""
+"""The Fortran side of the C ABI: the ISO_C_BINDING module, the drop-in
+imcom_matrix / imcom_bisect modules (the unchanged reference driver compiles
+against them), and a driver that runs the reference's get_A, get_QL, get_B,
+get_P, get_T, build_image sequence (imcom.f90:151-169) through the C ABI,
+checked against the oracle."""
+import ctypes
 import os
+import re
 import shutil
 import struct
 import subprocess
@@ -13,9 +17,54 @@ from imcom_amd import build as B
 from imcom_amd import synth
 from oracle import oracle as O
 
-needs_flang = pytest.mark.skipif(
-    shutil.which("flang") is None and shutil.which("flang-new") is None,
-    reason="flang not installed")
+FC = shutil.which("flang") or shutil.which("flang-new")
+needs_flang = pytest.mark.skipif(FC is None, reason="flang not installed")
+REF = "/root/reference"
+FORTRAN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "imcom_amd", "fortran")
+
+
+@needs_flang
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "imcom.f90")),
+                    reason="reference sources not present (read only where they are)")
+def test_reference_driver_compiles_against_gpu_modules(tmp_path):
+    """The unchanged reference driver imcom.f90 compiles against the drop-in
+    imcom_matrix / imcom_bisect modules (imcom_matrix_gpu.f90,
+    imcom_bisect_gpu.f90), over the reference's own imcom_data, imcom_proc
+    and imcom_io.  Compile only: linking the reference needs FFTW and LAPACK,
+    which this image lacks.  Two GNU spellings flang rejects are changed in
+    temporary copies of imcom_proc.f90 (zexp -> exp) and imcom_io.f90
+    (iargc() -> command_argument_count()); nothing from the reference is kept.
+    Every imcom_gpu_* symbol the modules call must be exported by the library."""
+    for name in ("imcom_data.f90", "imcom_proc.f90", "imcom_io.f90"):
+        src = open(os.path.join(REF, name)).read()
+        if name == "imcom_proc.f90":
+            src = src.replace("zexp(", "exp(")
+        if name == "imcom_io.f90":
+            src = src.replace("iargc()", "command_argument_count()")
+        (tmp_path / name).write_text(src)
+    units = [str(tmp_path / "imcom_data.f90"), str(tmp_path / "imcom_proc.f90"),
+             str(tmp_path / "imcom_io.f90"), os.path.join(FORTRAN, "imcom_gpu_iface.f90"),
+             os.path.join(FORTRAN, "imcom_matrix_gpu.f90"),
+             os.path.join(FORTRAN, "imcom_bisect_gpu.f90"), os.path.join(REF, "imcom.f90")]
+    objs = []
+    for u in units:
+        obj = str(tmp_path / (os.path.basename(u) + ".o"))
+        r = subprocess.run([FC, "-c", "-O0", "-J", str(tmp_path), "-I", str(tmp_path), u, "-o", obj],
+                           capture_output=True, text=True, cwd=tmp_path)
+        assert r.returncode == 0, f"{os.path.basename(u)}:\n{r.stderr[-3000:]}"
+        objs.append(obj)
+    called = set()
+    for obj in objs[3:6]:
+        out = subprocess.run(["nm", "-u", obj], capture_output=True, text=True).stdout
+        called |= set(re.findall(r"\b(imcom_gpu_\w+)\b", out))
+    assert {"imcom_gpu_init", "imcom_gpu_set_tile", "imcom_gpu_lookup_A", "imcom_gpu_eigen_A",
+            "imcom_gpu_build_P", "imcom_gpu_bisect_kappa", "imcom_gpu_build_T",
+            "imcom_gpu_set_T", "imcom_gpu_build_image"} <= called
+    if os.path.exists(B.LIB):
+        lib = ctypes.CDLL(B.LIB)
+        for name in sorted(called):
+            assert hasattr(lib, name), name
 
 
 def _write_input(path, t):
