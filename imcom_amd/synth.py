@@ -36,13 +36,14 @@ except Exception:  # pragma: no cover
     _j1 = None
 
 
-def gaussian_psf_image(npix, sigma_px, psfscale):
+def gaussian_psf_image(npix, sigma_px, psfscale, shift=(0.0, 0.0)):
     """Circular Gaussian sampled at PSF pixel centres, peak at 1-based pixel
-    npix/2+1."""
+    npix/2+1 displaced by `shift` input pixels (first axis, second axis)."""
     c = npix // 2
     s = sigma_px / psfscale
-    i = np.arange(npix) - c
-    g = np.exp(-0.5 * (i[:, None] ** 2 + i[None, :] ** 2) / s ** 2)
+    i = np.arange(npix) - c - shift[0] / psfscale
+    j = np.arange(npix) - c - shift[1] / psfscale
+    g = np.exp(-0.5 * (i[:, None] ** 2 + j[None, :] ** 2) / s ** 2)
     return np.asfortranarray(g)
 
 
@@ -125,6 +126,33 @@ def large(seed=20261205, **kw) -> Tile:
                 psf_npix=256, nout=96, outscale=0.25, noise_sigma=1e-3, seed=seed, name="large")
     args.update(kw)
     return make_tile(**args)
+
+
+def displaced_psf_tile(shift=(0.75, -0.5), seed=3, npix_in=8, nout=8, psf_sigma=1.2,
+                       gamma_sigma=1.5, psfscale=0.25, psf_npix=64) -> Tile:
+    """Two exposures with different PSF files (psfconst = 0, no rotation):
+    exposure 1 sees a centred Gaussian, exposure 2 the same Gaussian
+    displaced by `shift` input pixels.  The lookup results then have closed
+    forms that are not point-symmetric, which fixes the sign and the axis of
+    the table arguments x_j - x_i (A) and x_i - X_a (B)."""
+    rng = np.random.default_rng(seed)
+    nexp = 2
+    dither = rng.uniform(0.0, 1.0, size=(nexp, 2))
+    psfs = [gaussian_psf_image(psf_npix, psf_sigma, psfscale),
+            gaussian_psf_image(psf_npix, psf_sigma, psfscale, shift)]
+    gam = gaussian_psf_image(psf_npix, gamma_sigma, psfscale)
+    xc = 0.5 * (npix_in - 1) + dither[:, 0].mean()
+    yc = 0.5 * (npix_in - 1) + dither[:, 1].mean()
+    half = 0.25 * (nout - 1)
+    I = rng.normal(0.0, 1e-3, size=(npix_in, npix_in, nexp))
+    return assemble_tile(
+        I_unmasked=np.asfortranarray(I), gimxscale=np.ones(nexp), gimyscale=np.ones(nexp),
+        dither=dither, psf_img=psfs, rotangdeg=np.zeros(nexp), gamma_img=gam,
+        psfxscale=psfscale, psfyscale=psfscale, noise=np.full(nexp, 1e-6), n1out=nout,
+        n2out=nout, outxscale=0.5, outyscale=0.5, outpos1=xc - half, outpos2=yc - half,
+        psffile=["psf1.fits", "psf2.fits"],
+        meta=dict(name="displaced_psf", shift=tuple(shift), psf_sigma=psf_sigma,
+                  gamma_sigma=gamma_sigma, psfscale=psfscale))
 
 
 def tiny(seed=7, nexp=2, npix_in=8, nout=8, psf_npix=64, **kw) -> Tile:

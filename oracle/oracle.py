@@ -103,6 +103,22 @@ def lookup_A(tile):
     return A
 
 
+def lookup_A_rows(tile, i0, i1):
+    """Rows [i0, i1) of A (entries j >= i, as imcom_lookup_A fills them,
+    imcom_matrix.f90:223-257), returned as an (i1 - i0) x n array."""
+    n = tile.n
+    A = np.zeros((n, n), order="F")
+    tab = _f(tile.Alookup)
+    ex = np.ascontiguousarray(tile.exp_i, dtype=np.int32)
+    x, y = np.ascontiguousarray(tile.x_i), np.ascontiguousarray(tile.y_i)
+    lib().or_lookup_A_rows(ctypes.c_int64(n), ctypes.c_int64(i0), ctypes.c_int64(i1), _p(x), _p(y),
+                           ex.ctypes.data_as(_ip), ctypes.c_int(tile.psfconst),
+                           ctypes.c_int(tab.shape[0]), ctypes.c_int(tab.shape[1]), _p(tab),
+                           ctypes.c_int(tile.npad), ctypes.c_int(tile.npoly),
+                           ctypes.c_double(tile.psfxscale), ctypes.c_double(tile.psfyscale), _p(A))
+    return np.array(A[i0:i1, :])
+
+
 def lookup_B(tile):
     n, m = tile.n, tile.m
     B = np.zeros((n, m), order="F")

@@ -18,6 +18,7 @@ import pytest
 
 from imcom_amd import synth
 from oracle import oracle as O
+from oracle import tolerances as TOL
 
 pytestmark = pytest.mark.gpu
 
@@ -30,22 +31,13 @@ def _tiles():
 
 
 def u_tol(r, t):
-    """1e-12 + 100 x sum_k |dU/dlambda_k| n eps lambda_max at the oracle's kappa,
-    dU/dlambda_k = P_k^2 (lambda_k + 3 kappa) / (4 C (lambda_k + kappa)^3)."""
-    L, P2, K = r["L"], r["P2"], r["K"]
-    w = np.abs(L[:, None] + 3 * K[None, :]) / np.abs(L[:, None] + K[None, :]) ** 3
-    d = t.n * np.finfo(np.float64).eps * np.max(np.abs(L))
-    return 1e-12 + 100.0 * d / (4 * t.C_a) * np.sum(P2 * w, axis=0)
+    """Per-pixel U bound at the oracle's kappa (oracle/tolerances.py)."""
+    return TOL.u_tol(r["L"], r["P2"], r["K"], t.C_a, t.n)
 
 
 def t_tol(r, t):
-    """Per-pixel bound on |delta T_ia|: 1e-10 max|T| + 100 x the first-order
-    change of T = -1/2 Q P/(L + kappa) under eigenvalue errors of
-    n*eps*lambda_max, 1/2 sum_k |P_ka| d/(lambda_k + kappa_a)^2 (|Q| <= 1)."""
-    L, P, K = r["L"], r["P"], r["K"]
-    d = t.n * np.finfo(np.float64).eps * np.max(np.abs(L))
-    g = 0.5 * np.sum(np.abs(P) * d / (L[:, None] + K[None, :]) ** 2, axis=0)
-    return 1e-10 * np.max(np.abs(r["T"])) + 100.0 * g
+    """Per-pixel T bound at the oracle's kappa (oracle/tolerances.py)."""
+    return TOL.t_tol(r["L"], r["P"], r["K"], t.n, np.max(np.abs(r["T"])))
 
 
 @pytest.fixture(scope="module")

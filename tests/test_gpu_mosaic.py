@@ -1,0 +1,134 @@
+"""Parity on the 64-tile mosaic (2-6 exposures of 16-32 px squares, n from
+1024 to 6144, m from 256 to 2304, PSF sigma in [1.0, 1.6], noise sigma
+log-uniform in [1e-4, 1e-2]), solved as three concurrent batched solves
+(imcom_amd.batch.StreamedBatch) replayed from HIP graphs.  These launches are
+large enough for the grouped 128x128 FP64 products.
+
+Both paths the benchmark times are checked:
+  plain  FP64 products, full-range bisection
+  fast   learned kappa bracket, BF16x6 split product for T (P and the
+         bisection stay FP64)
+For every pixel of all 64 tiles: U < U_max, and a second solve reproduces
+the first bit for bit.  The oracle solves the smallest, the median and the
+largest tile (LAPACK DSYEVR, its C restatement of the lookup and the
+bisection).  On those tiles:
+  plain  kappa equal to the oracle's in >= 98% of pixels; U within the
+         conditioning bound u_tol of the oracle's (eigenvalue errors of
+         n eps lambda_max, as in test_gpu_batch); T, S and H at 48 sampled
+         pixels within t_tol of the oracle's at the same kappa
+  fast   U below U_max and within tol of it wherever the oracle's is (the
+         learned bracket may end on another kappa of the same tolerance band);
+         T at 48 sampled pixels against the oracle's T at the fast path's own
+         kappa within t_tol + 2e-6 max|T| (the BF16x6 product's error on T is
+         5.5e-7 of max|T| on the small tile); S and H from that T alike
+"""
+import numpy as np
+import pytest
+
+from imcom_amd import kappa_net, synth
+from imcom_amd.gpu import IMCOM_PREC_BF16X6, IMCOM_PREC_FP64
+from oracle import oracle as O
+from oracle import tolerances as TOL
+
+pytestmark = pytest.mark.gpu
+EPS = 2.0 ** -53
+
+
+def _u_tol(t, L, P2, K):
+    return TOL.u_tol(L, P2, K, t.C_a, t.n)
+
+
+def _t_tol(t, L, P, K, Tmax):
+    return TOL.t_tol(L, P, K, t.n, Tmax)
+
+
+@pytest.fixture(scope="module")
+def mosaic():
+    tiles = synth.batch()
+    ns = np.array([t.n for t in tiles])
+    order = np.argsort(ns, kind="stable")
+    picked = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    refs = {}
+    for i in picked:
+        t = tiles[i]
+        print(f"[case] mosaic oracle tile {i} n={t.n} m={t.m}", flush=True)
+        A = O.lookup_A(t)
+        Q, L, _ = O.eigen_A(A)
+        del A
+        B = O.lookup_B(t)
+        P = np.asfortranarray(Q.T @ B)
+        K, U, ev = O.bisect_kappa(L, P * P, t.C_a, t.US_max, t.kappa_min, t.kappa_max, t.US_tol,
+                                  t.maxNbis, t.USB, t.Ndiag)
+        refs[i] = dict(Q=Q, L=L, P=P, K=K, U=U, ev=ev)
+    return tiles, picked, refs
+
+
+def _solve_twice(tiles, picked, **kw):
+    from imcom_amd.batch import StreamedBatch
+    with StreamedBatch(tiles, weights=kw.pop("weights", None)) as sb:
+        sb.solve(use_graph=True, **kw)
+        first = [sb.result(i, want_T=i in picked) for i in range(len(tiles))]
+        sb.solve(use_graph=True, **kw)
+        for i in range(len(tiles)):
+            again = sb.result(i)
+            for k in ("K", "U", "S", "H", "evals"):
+                assert np.array_equal(again[k], first[i][k]), (i, k)
+    return first
+
+
+def _check_maps(t, r, out, idx, K, extra):
+    """T, S and H at pixels idx against the oracle's T at kappa K."""
+    Tr = O.build_T(r["Q"], r["P"][:, idx], r["L"], K)
+    tol = _t_tol(t, r["L"], r["P"][:, idx], K, np.abs(Tr).max()) + extra * np.abs(Tr).max()
+    dT = np.abs(out["T"][:, idx] - Tr).max(axis=0)
+    assert np.all(dT <= tol), float(np.max(dT / tol))
+    Sr = np.sum(Tr * Tr * t.Ndiag[:, None], axis=0)
+    assert np.all(np.abs(out["S"][idx] - Sr) <= TOL.S_tol(tol, Tr, t.Ndiag, Sr))
+    Hr = Tr.T @ t.I_i
+    assert np.all(np.abs(out["H"][idx] - Hr) <= TOL.H_tol(tol, Hr, t.I_i))
+    return float(np.max(dT / np.abs(Tr).max()))
+
+
+def test_mosaic_plain_fp64(mosaic):
+    tiles, picked, refs = mosaic
+    print("[case] mosaic plain FP64, 64 tiles, three streamed batched solves", flush=True)
+    outs = _solve_twice(tiles, picked, precision_P=IMCOM_PREC_FP64, precision_T=IMCOM_PREC_FP64,
+                        learned=False)
+    bad = sum(int(np.sum(~(o["U"] < t.US_max))) for o, t in zip(outs, tiles))
+    assert bad == 0, f"{bad} pixels with U >= U_max"
+    rng = np.random.default_rng(64)
+    for i in picked:
+        t, r, out = tiles[i], refs[i], outs[i]
+        close = np.abs(out["K"] / r["K"] - 1.0) <= 1e-12
+        assert close.mean() >= 0.98, (i, close.mean())
+        ok = r["ev"] > 1
+        assert np.all((np.abs(out["U"] - r["U"]) <= _u_tol(t, r["L"], r["P"] ** 2, r["K"]))[ok])
+        idx = np.sort(rng.choice(np.flatnonzero(close & ok), min(48, int(np.sum(close & ok))),
+                                 replace=False))
+        rel = _check_maps(t, r, out, idx, r["K"][idx], 0.0)
+        print(f"[mosaic plain] tile {i} n={t.n} m={t.m}: kappa equal {close.mean():.3f}, "
+              f"max rel dT {rel:.2e}", flush=True)
+
+
+def test_mosaic_fast_path(mosaic):
+    tiles, picked, refs = mosaic
+    weights = kappa_net.load_weights()
+    assert weights is not None, "trained bracket weights missing"
+    print("[case] mosaic fast path (learned bracket, BF16x6 T), 64 tiles", flush=True)
+    outs = _solve_twice(tiles, picked, precision_T=IMCOM_PREC_BF16X6, learned=True,
+                        weights=weights)
+    bad = sum(int(np.sum(~(o["U"] < t.US_max))) for o, t in zip(outs, tiles))
+    assert bad == 0, f"{bad} pixels with U >= U_max"
+    hits = np.concatenate([o["hits"] for o in outs])
+    ev = np.concatenate([o["evals"] for o in outs])
+    print(f"[mosaic fast] brackets held in {hits.mean():.3f} of pixels, mean U evaluations "
+          f"{ev.mean():.2f}", flush=True)
+    rng = np.random.default_rng(65)
+    for i in picked:
+        t, r, out = tiles[i], refs[i], outs[i]
+        reached = t.US_max - r["U"] <= t.US_tol
+        assert np.all((t.US_max - out["U"] <= t.US_tol)[reached])
+        idx = np.sort(rng.choice(t.m, min(48, t.m), replace=False))
+        rel = _check_maps(t, r, out, idx, out["K"][idx], 2e-6)
+        print(f"[mosaic fast] tile {i} n={t.n} m={t.m}: max rel dT (BF16x6, own kappa) {rel:.2e}",
+              flush=True)

@@ -77,6 +77,29 @@ def test_lookup_A_B_tiny(gpu_ctx):
     assert np.max(np.abs(B - Br)) <= 1e-13 * np.max(np.abs(Br))
 
 
+def test_lookup_A_B_displaced_psf(gpu_ctx):
+    """The device lookups on a tile whose second exposure's PSF is displaced
+    by s = (0.75, -0.5) px (different PSF per exposure, psfconst = 0): equal
+    to the oracle to 1e-13 and to the closed forms with the shifted arguments
+    (oracle/closed_forms.py), which fix the sign and axis of x_j - x_i and
+    x_i - X_a."""
+    from oracle import closed_forms as CF
+    _say("lookup A/B displaced PSF")
+    t = synth.displaced_psf_tile()
+    mt = t.meta
+    shifts = [(0.0, 0.0), mt["shift"]]
+    gpu_ctx.set_tile(t)
+    A = gpu_ctx.lookup_A(out=True)
+    B = gpu_ctx.lookup_B(out=True)
+    Ar, Br = O.lookup_A(t), O.lookup_B(t)
+    assert np.max(np.abs(A - Ar)) <= 1e-13 * np.max(np.abs(Ar))
+    assert np.max(np.abs(B - Br)) <= 1e-13 * np.max(np.abs(Br))
+    Ac = CF.A_closed_form(t, shifts, mt["psf_sigma"], mt["psfscale"])
+    Bc = CF.B_closed_form(t, shifts, mt["psf_sigma"], mt["gamma_sigma"], mt["psfscale"])
+    assert np.abs(A - Ac).max() <= 1e-8 * Ac.max()
+    assert np.abs(B - Bc).max() <= 1e-6 * np.abs(Bc).max()
+
+
 def test_lookup_A_B_small(gpu_ctx, small_tile, small_ref):
     _say("lookup A/B small n=1024 m=576")
     gpu_ctx.set_tile(small_tile)
@@ -297,10 +320,12 @@ def test_solve_small_end_to_end(gpu_ctx, small_tile, small_ref):
     """Whole path on the GPU against the oracle (LAPACK eigensolver): T, U, S,
     H compared through eigenvector-free quantities."""
     _say("solve small end-to-end")
+    from oracle import tolerances as TOL
     t, r = small_tile, small_ref
     out = gpu_ctx.solve(t)
     assert np.all(out["U"] < t.US_max)
-    assert np.max(np.abs(out["U"] - r["U"])) <= 1e-11
+    ok = r["evals"] > 1
+    assert np.all((np.abs(out["U"] - r["U"]) <= TOL.u_tol(r["L"], r["P2"], r["K"], t.C_a, t.n))[ok])
     # kappa: same bisection decisions in almost every pixel
     close = np.abs(out["K"] / r["K"] - 1.0) <= 1e-12
     assert close.mean() >= 0.95
@@ -308,7 +333,12 @@ def test_solve_small_end_to_end(gpu_ctx, small_tile, small_ref):
     A, B, T = r["A"], r["B"], out["T"]
     Uid = (t.C_a + np.einsum("ia,ia->a", T, A @ T) + np.einsum("ia,ia->a", B, T)) / t.C_a
     assert np.max(np.abs(Uid - out["U"])) <= 1e-10
-    same = close
-    relT = np.max(np.abs(T[:, same] - r["T"][:, same])) / np.max(np.abs(r["T"]))
-    assert relT <= 1e-8, relT
-    assert np.max(np.abs(out["S"] - r["S"]) / r["S"]) <= 1e-8
+    # T, S, H per pixel within the conditioning bounds where kappa agrees
+    same = close & ok
+    tT = TOL.t_tol(r["L"], r["P"], r["K"], t.n, np.abs(r["T"]).max())
+    dT = np.max(np.abs(T - r["T"]), axis=0)
+    print(f"[small e2e] kappa equal {close.mean():.3f}, max dT/t_tol {np.max((dT / tT)[same]):.3e}",
+          flush=True)
+    assert np.all((dT <= tT)[same])
+    assert np.all((np.abs(out["S"] - r["S"]) <= TOL.S_tol(tT, r["T"], t.Ndiag, r["S"]))[same])
+    assert np.all((np.abs(out["H"] - r["H"]) <= TOL.H_tol(tT, r["H"], t.I_i))[same])

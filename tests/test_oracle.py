@@ -134,6 +134,29 @@ def test_table_orientation_shifted_psf():
     assert abs(peak - ps ** 2 / (4 * np.pi * 1.2 ** 2)) <= 1e-9 * peak
 
 
+def test_lookup_sign_and_axis_displaced_psf():
+    """lookup_A / lookup_B on a tile whose second exposure's PSF is displaced
+    by s = (0.75, -0.5) px: A and B follow the closed forms with the shifted
+    arguments (oracle/closed_forms.py), which fixes the sign and the axis of
+    x_j - x_i and x_i - X_a.  A sign error or a swapped axis moves the peak by
+    2 s and misses by more than half the peak value."""
+    from oracle import closed_forms as CF
+    t = synth.displaced_psf_tile()
+    mt = t.meta
+    shifts = [(0.0, 0.0), mt["shift"]]
+    A, B = O.lookup_A(t), O.lookup_B(t)
+    Ac = CF.A_closed_form(t, shifts, mt["psf_sigma"], mt["psfscale"])
+    Bc = CF.B_closed_form(t, shifts, mt["psf_sigma"], mt["gamma_sigma"], mt["psfscale"])
+    assert t.psfconst == 0 and set(np.unique(t.exp_i)) == {1, 2}
+    assert np.abs(A - Ac).max() <= 1e-8 * Ac.max()
+    assert np.abs(B - Bc).max() <= 1e-6 * np.abs(Bc).max()
+    for wrong in ([(0.0, 0.0), (-0.75, 0.5)], [(0.0, 0.0), (-0.5, 0.75)]):
+        assert np.abs(A - CF.A_closed_form(t, wrong, mt["psf_sigma"], mt["psfscale"])).max() \
+            > 0.3 * Ac.max()
+        assert np.abs(B - CF.B_closed_form(t, wrong, mt["psf_sigma"], mt["gamma_sigma"],
+                                           mt["psfscale"])).max() > 0.3 * np.abs(Bc).max()
+
+
 def test_pad_size_and_ndiag():
     x = np.array([0.0, 15.3])
     X = np.array([2.0, 13.0])
