@@ -7,9 +7,15 @@ build_S -> build_image, the driver's sequence (imcom.f90:151-169).
 
 Workloads:
   large  (default) one tile: 4 exposures of 64x64 (n = 16384 input pixels),
-         Airy (obscuration 0.3, lambda/D = 0.5 px) x pixel PSF, 96x96 output
-         pixels at 0.5 px (m = 9216), U_max = 1e-6
-  small  one tile: 4 x 16x16 (n = 1024), Gaussian PSF, 24x24 output (m = 576)
+         Airy (obscuration 0.3, lambda/D = 1 input px) x unit-pixel PSF on a
+         256^2 image at 0.25 px (lookup tables 1536^2), 96x96 output pixels at
+         0.25 px (m = 9216, a 24 px square centred in the 64 px footprint),
+         U_max = 1e-6.  The pitch and lambda/D are this benchmark's choice:
+         at lambda/D = 0.5 px the aliased Airy power leaves a leakage floor
+         near 2e-4, and a 0.5 px pitch puts the grid's edge 8 px from the
+         footprint's, too close for the Airy wings to reach U = 1e-6.
+  small  one tile: 4 x 16x16 (n = 1024), Gaussian PSF sigma 1.2 px, 24x24
+         output at 0.5 px (m = 576)
   batch  64 tiles with n in [1024, 6144] and m in [256, 2304]; the fast path
          is three concurrent batched solves per step (HIP-graph replay)
 
@@ -18,12 +24,16 @@ Two paths are timed: "plain" (full-range bisection, FP64 GEMMs) and "fast"
 P stays FP64 because U ~ 1e-6 is a difference of O(1) terms).  `value` is
 the fast path's output pixels per second; the plain path's number is in
 config.  After the timed steps every pixel's U is checked against U_max.
+The default (large) run also measures the 64-tile mosaic (config.batch),
+where the batched fast path is compared with the plain single-tile path.
 
 The JSON line carries a roofline entry for the dominant kernel (sy_symv, the
 matrix-vector kernel of the tridiagonal reduction; HBM-bound; HIP events
-around each launch on the context's stream in one extra profiled step) and a
-CPU baseline: the oracle's C/LAPACK restatement timed on bounded samples of
-the same tile on this host's cores, scaled stage by stage to the whole tile.
+around each launch on the context's stream in one extra profiled step), with
+the per-stage rooflines of that step beside it, and a CPU baseline: the
+oracle's C/LAPACK restatement on this host's cores, on the whole tile for the
+small workload and on bounded samples scaled stage by stage (with the scale
+factors checked by a second sample) for the large one.
 """
 from __future__ import annotations
 
@@ -40,6 +50,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6     # dense FP64 (vector and MFMA)
+BF16_PEAK_TFS = 2500.0   # dense BF16 MFMA
 METRIC = "output pixels/sec at U≤1e-6 (A/B+eig+κ+T), learned+split vs plain FP64"
 
 
@@ -53,6 +65,143 @@ def symv_alg_bytes(n: int) -> float:
     matrix once (8 bytes per element); n-1 launches."""
     s = np.arange(1, n, dtype=np.float64)
     return float(np.sum(s * (s + 1) / 2.0) * 8.0)
+
+
+def describe(workload: str, tiles) -> str:
+    """The workload's shape and sizes, as the JSON line states them."""
+    t = tiles[0]
+    N = t.npad * int(t.meta.get("n1pad", 0))
+    if workload == "large":
+        return (f"large: 1 tile, 4 x 64^2 inputs (n={t.n}), Airy (obscuration 0.3, lambda/D = 1 "
+                f"input px) x unit-pixel PSF on a 256^2 image at 0.25 px ({N}^2 lookup tables), "
+                f"96^2 outputs at 0.25 px pitch (m={t.m}), Gaussian target sigma 1.8 px, "
+                f"U_max={t.US_max}")
+    if workload == "small":
+        return (f"small: 1 tile, 4 x 16^2 inputs (n={t.n}), Gaussian PSF sigma 1.2 px on a 128^2 "
+                f"image at 0.25 px ({N}^2 lookup tables), 24^2 outputs at 0.5 px pitch "
+                f"(m={t.m}), Gaussian target sigma 1.5 px, U_max={t.US_max}")
+    return (f"batch: {len(tiles)} tiles, n in [{min(x.n for x in tiles)}, "
+            f"{max(x.n for x in tiles)}], sum m={sum(x.m for x in tiles)}, Gaussian PSFs sigma "
+            f"in [1.0, 1.6] px, U_max={t.US_max}")
+
+
+def stage_rooflines(n: int, m: int, st: dict, t_bf16_ms: float) -> list:
+    """Roofline of each stage of one profiled plain step (HIP-event device
+    times, no downloads inside them) from its algorithmic work:
+      lookup_A  8 n^2 bytes written (A, both triangles)     HBM
+      lookup_B  8 n m bytes written                          HBM
+      eigen_A   (10/3) n^3 flops (reduction + D&C + back-transform; the
+                one-stage reduction is itself HBM-bound: see roofline)  FP64
+      build_P   2 n^2 m flops                                FP64 MFMA
+      bisect    8 n m bytes read (P once)                    HBM
+      build_T   2 n^2 m flops (plus W = P / (L + K))         FP64 MFMA
+      build_T BF16x6  2 n^2 m FP64-equivalent flops, and 6 x 2 n^2 m flops
+                of BF16 MFMA products against the BF16 dense peak"""
+    def row(stage, ms, work, unit, peak, bound, what):
+        if ms <= 0:
+            return None
+        ach = work / (ms * 1e-3) / (1e9 if unit == "GB/s" else 1e12)
+        return {"stage": stage, "bound": bound, "work": what, "ms": round(ms, 3),
+                "achieved": round(ach, 2), "peak": peak, "unit": unit,
+                "frac": round(ach / peak, 4)}
+    eig_ms = st["sytrd"][0] + st["stedc"][0] + st["ormtr"][0]
+    rows = [
+        row("lookup_A", st["lookup_A"][0], 8.0 * n * n, "GB/s", HBM_PEAK_GBS, "hbm", "8 n^2 B written"),
+        row("lookup_B", st["lookup_B"][0], 8.0 * n * m, "GB/s", HBM_PEAK_GBS, "hbm", "8 n m B written"),
+        row("eigen_A", eig_ms, 10.0 / 3.0 * n ** 3, "TFLOP/s", FP64_PEAK_TFS, "fp64",
+            "(10/3) n^3 flops"),
+        row("build_P", st["build_P"][0], 2.0 * n * n * m, "TFLOP/s", FP64_PEAK_TFS, "mfma",
+            "2 n^2 m flops"),
+        row("bisect_kappa", st["bisect"][0], 8.0 * n * m, "GB/s", HBM_PEAK_GBS, "hbm",
+            "8 n m B read"),
+        row("build_T", st["build_T"][0], 2.0 * n * n * m, "TFLOP/s", FP64_PEAK_TFS, "mfma",
+            "2 n^2 m flops"),
+        row("build_T_bf16x6", t_bf16_ms, 12.0 * n * n * m, "TFLOP/s", BF16_PEAK_TFS, "mfma",
+            "6 x 2 n^2 m BF16 flops (2 n^2 m FP64-equivalent)"),
+    ]
+    return [r for r in rows if r is not None]
+
+
+def batch_block(steps: int, warmup: int, weights, skip_plain: bool) -> dict:
+    """The 64-tile mosaic (synth.batch): the fast path as three concurrent
+    batched solves replayed from HIP graphs (learned bracket, BF16x6 T), the
+    plain path as the single-tile FP64 path tile after tile, each tile's
+    inputs resident in its own context before the timed region.  Every
+    pixel's U is checked against U_max after the timed steps."""
+    import torch
+    from imcom_amd import autotune, synth
+    from imcom_amd.batch import DEFAULT_STREAMS, StreamedBatch
+    from imcom_amd.gpu import IMCOM_PREC_BF16X6, ImcomGPU
+    tiles = synth.batch()
+    m_total = sum(t.m for t in tiles)
+    Umax = np.concatenate([np.full(t.m, t.US_max) for t in tiles])
+    learned = weights is not None
+
+    def timed(step, k, w):
+        for _ in range(w):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(k):
+            U, ev = step()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t1) / k, int(np.sum(~(U < Umax))), float(ev.mean())
+
+    # fast: opened before any other context (an idle context beside the three
+    # solves slows them)
+    with StreamedBatch(tiles, DEFAULT_STREAMS, weights=weights) as sb:
+        def fast():
+            sb.solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
+            res = [sb.result(i) for i in range(len(tiles))]
+            return (np.concatenate([r["U"] for r in res]),
+                    np.concatenate([r["evals"] for r in res]))
+        f_sec, f_bad, f_ev = timed(fast, steps, warmup)
+    log(f"batch fast: {f_sec * 1e3:.1f} ms/step, {m_total / f_sec:.1f} px/s, pixels with "
+        f"U >= U_max: {f_bad}, mean U evaluations {f_ev:.2f}")
+    out = {"workload": (f"{len(tiles)} tiles, n in [{min(t.n for t in tiles)}, "
+                        f"{max(t.n for t in tiles)}], sum m = {m_total}, U_max = 1e-6"),
+           "fast_path": ("three concurrent batched solves (one stream each, HIP-graph replay), "
+                         + ("learned kappa bracket + " if learned else "")
+                         + "BF16x6 split GEMM for T; P and the bisection FP64"),
+           "fast_px_per_s": round(m_total / f_sec, 2), "fast_ms_per_step": round(f_sec * 1e3, 2),
+           "fast_pixels_U_ge_Umax": f_bad, "fast_mean_U_evaluations": round(f_ev, 2),
+           "steps": steps, "warmup": warmup}
+    if skip_plain:
+        return out
+    singles = []
+    try:
+        for t in tiles:
+            gs = ImcomGPU(0)
+            singles.append(gs)
+            autotune.apply(gs, t.n)
+            gs.set_tile(t)
+
+        def plain():
+            Us, evs = [], []
+            for gs, t in zip(singles, tiles):
+                gs.lookup_A(t.npoly, t.npad)
+                gs.eigen_A(out=False)
+                gs.lookup_B(t.npoly, t.npad)
+                gs.build_P()
+                _, U, ev = gs.bisect_kappa()
+                gs.build_T(out=False)
+                gs.build_S()
+                gs.build_image()
+                Us.append(U)
+                evs.append(ev)
+            return np.concatenate(Us), np.concatenate(evs)
+        p_sec, p_bad, p_ev = timed(plain, 1, 1)
+    finally:
+        for gs in singles:
+            gs.close()
+    log(f"batch plain: {p_sec * 1e3:.1f} ms/step, {m_total / p_sec:.1f} px/s, pixels with "
+        f"U >= U_max: {p_bad}, mean U evaluations {p_ev:.2f}")
+    out.update({"plain_path": "single-tile FP64 path, tile after tile, full-range bisection",
+                "plain_px_per_s": round(m_total / p_sec, 2),
+                "plain_ms_per_step": round(p_sec * 1e3, 2), "plain_pixels_U_ge_Umax": p_bad,
+                "plain_mean_U_evaluations": round(p_ev, 2), "plain_steps": 1,
+                "fast_over_plain": round(p_sec / f_sec, 3)})
+    return out
 
 
 def vendor_eig(g, tile):
@@ -113,6 +262,8 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "small", "batch"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="large workload: leave out the 64-tile mosaic block (config.batch)")
     ap.add_argument("--vendor-eig", action="store_true",
                     help="also time rocSOLVER's dsyevd on the same A (single-tile workloads)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -274,7 +425,8 @@ def main():
         pcie = time.perf_counter() - t1
         log(f"fast step with host hand-over (inputs up, T/K/U/S/H down): {pcie * 1e3:.1f} ms")
 
-    # one profiled plain step (single tile): per-stage and per-launch device times
+    # one profiled plain step (single tile): per-stage and per-launch device
+    # times (no stage output leaves the device inside it), then the BF16x6 T
     roofline = None
     Lh = Ph = None
     if args.workload != "batch":
@@ -282,13 +434,18 @@ def main():
         g.lookup_A(tile.npoly, tile.npad)
         _, Lh, _ = g.eigen_A(out_Q=False, out_L=True)
         g.lookup_B(tile.npoly, tile.npad)
-        Ph = g.build_P(out=True)
+        g.build_P()
         g.bisect_kappa()
         g.build_T(out=False)
         g.build_S()
         g.build_image()
         st = g.stage_times()
+        g.set_profiling(1)
+        g.build_T(IMCOM_PREC_BF16X6, out=False)
+        t_bf16 = g.stage_times()["build_T"][0]
         g.set_profiling(0)
+        Ph = g.fetch_async("P")
+        g.fetch_wait("P")
         log("stage times (ms, launches): " + ", ".join(f"{k}={v[0]:.1f}/{v[1]}" for k, v in st.items()))
         symv_ms, symv_n = st["symv"]
         per_launch = symv_alg_bytes(tile.n) / max(symv_n, 1)
@@ -303,7 +460,8 @@ def main():
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "alg_bytes_per_launch": round(per_launch),
                     "avg_launch_ms": round(avg_ms, 5), "launches": symv_n,
-                    "stage_ms": {k: round(v[0], 2) for k, v in st.items()}}
+                    "stage_ms": {k: round(v[0], 2) for k, v in st.items()},
+                    "stages": stage_rooflines(tile.n, tile.m, st, t_bf16)}
 
     # the lookup-table set-up on the device (build_Alookup/Blookup from the
     # tile's spectra) beside the host's numpy set-up; not part of the step
@@ -339,26 +497,44 @@ def main():
         tc = time.time()
         tot = 0.0
         stages = {}
-        for t in (tiles if args.workload != "batch" else tiles[:4]):
-            s_t, st_t = O.sampled_cpu_time(t, L=Lh if t is tile else None, P=Ph if t is tile else None)
-            tot += s_t
-            for k, v in st_t.items():
-                stages[k] = stages.get(k, 0.0) + v
-        m_sampled = m_total if args.workload != "batch" else sum(t.m for t in tiles[:4])
+        checks = {}
         cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        if args.workload == "small":
+            # the whole tile: every stage of the oracle's solve, timed
+            t1 = time.perf_counter()
+            O.solve(tile)
+            tot = time.perf_counter() - t1
+            m_sampled = m_total
+            sample = (f"oracle C+LAPACK(OpenBLAS) solve of the whole tile (n={tile.n}, "
+                      f"m={tile.m}): lookup_A, DSYEVR, lookup_B, DGEMM, bisection, T, S, H")
+        else:
+            for t in (tiles if args.workload != "batch" else tiles[:4]):
+                s_t, st_t, ck = O.sampled_cpu_time(t, L=Lh if t is tile else None,
+                                                   P=Ph if t is tile else None,
+                                                   checks=args.workload == "large")
+                tot += s_t
+                checks.update(ck)
+                for k, v in st_t.items():
+                    stages[k] = stages.get(k, 0.0) + v
+            m_sampled = m_total if args.workload != "batch" else sum(t.m for t in tiles[:4])
+            if args.workload == "large":
+                sample = ("oracle C+LAPACK(OpenBLAS) on bounded samples of this tile, scaled per "
+                          "stage: A rows 0-191, B rows 0-95, DSYEVR on the leading 4096^2 block "
+                          "(x (n/4096)^3; the exponent fitted against 2048^2 is in scale_checks), "
+                          "DGEMM on 128 output columns, bisection on 48 and T on 16 output pixels")
+            else:
+                sample = ("oracle C+LAPACK(OpenBLAS) on bounded samples of the first 4 tiles, "
+                          "scaled per stage: A rows 0-95, B rows 0-95, DSYEVR on the leading "
+                          "2048^2 block (x (n/2048)^3), DGEMM on 128 output columns, bisection "
+                          "on 48 and T on 8 output pixels")
         cpu = {"value": round(m_sampled / tot, 4), "unit": "output pixels/sec", "cores": cores,
-               "kind": "port",
-               "sample": ("oracle C+LAPACK(OpenBLAS) on bounded samples of "
-                          + ("this tile" if args.workload != "batch" else "the first 4 tiles")
-                          + ", scaled per stage: A rows 0-95, B rows 0-95, DSYEVR on the "
-                          "leading 2048^2 block (x (n/2048)^3), DGEMM on 128 output columns, "
-                          "bisection on 48 and T on 8 output pixels; "
-                          f"{time.time() - tc:.0f}s of CPU work"),
+               "kind": "port", "sample": sample + f"; {time.time() - tc:.0f}s of CPU work",
                "stages_s": {k: round(v, 2) for k, v in stages.items()}}
+        if checks:
+            cpu["scale_checks"] = checks
         log(f"cpu baseline: {tot:.0f} s for {m_sampled} px -> {cpu['value']} px/s ({cores} threads)")
 
-    cfg = {"workload": (f"{args.workload}: {len(tiles)} tile(s), n={tile.n if len(tiles) == 1 else 'mixed'}"
-                        f" input px, sum m={m_total} output px, U_max={tile.US_max}"),
+    cfg = {"workload": describe(args.workload, tiles),
            "path": (("batched tiles (three concurrent batched solves, one stream each, each "
                      "replayed from HIP graphs) + " if args.workload == "batch" else "")
                     + ("learned kappa bracket + BF16x6 split GEMM for T" if learned
@@ -378,6 +554,13 @@ def main():
         cfg["plain_fp64_ms_per_step"] = round(plain[0] * 1e3, 2)
         cfg["plain_mean_U_evaluations"] = round(plain[2], 2)
         cfg["fast_over_plain"] = round(plain[0] / sec, 3)
+    if g is not None:
+        g.close()
+        g = None
+    if args.workload == "large" and not args.no_batch:
+        # the 64-tile mosaic beside the large tile (fewer steps: a plain
+        # step solves the 64 tiles one after another, about 4 s)
+        cfg["batch"] = batch_block(2, 1, weights, args.skip_plain)
     out = {
         "metric": METRIC, "value": round(m_total / sec, 2), "unit": "output pixels/sec",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),

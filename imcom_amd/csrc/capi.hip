@@ -180,12 +180,14 @@ int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   const int64_t n = ctx->n;
   CK(ctx->A.reserve((size_t)n * n * 8), "alloc A");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
-  StageTimer tm(ctx, PS_LOOKUP_A);
-  CK(launch_lookup_A(ctx->stream, ctx->device, n, ctx->x.as<double>(), ctx->y.as<double>(),
-                     ctx->ex.as<int32_t>(), ctx->psfconst, ctx->Atab.as<double>(),
-                     (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
-                     psfyscale, ctx->A.as<double>()),
-     "lookup_A");
+  {
+    StageTimer tm(ctx, PS_LOOKUP_A);
+    CK(launch_lookup_A(ctx->stream, ctx->device, n, ctx->x.as<double>(), ctx->y.as<double>(),
+                       ctx->ex.as<int32_t>(), ctx->psfconst, ctx->Atab.as<double>(),
+                       (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
+                       psfyscale, ctx->A.as<double>()),
+       "lookup_A");
+  }
   ctx->have_A = true;
   CK(download(ctx, A_out, ctx->A, (size_t)n * n * 8), "download A");
   return IMCOM_OK;
@@ -212,12 +214,15 @@ int imcom_gpu_lookup_B(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->B.reserve((size_t)n * m * 8), "alloc B");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
-  StageTimer tm(ctx, PS_LOOKUP_B);
-  CK(launch_lookup_B(ctx->stream, ctx->device, n, m, ctx->x.as<double>(), ctx->y.as<double>(),
-                     ctx->ex.as<int32_t>(), ctx->Xa.as<double>(), ctx->Ya.as<double>(),
-                     ctx->psfconst, ctx->Btab.as<double>(), (int64_t)ctx->n1 * ctx->n2, ctx->n1,
-                     ctx->n2, np / 2, npad, psfxscale, psfyscale, ctx->B.as<double>()),
-     "lookup_B");
+  {
+    StageTimer tm(ctx, PS_LOOKUP_B);
+    CK(launch_lookup_B(ctx->stream, ctx->device, n, m, ctx->x.as<double>(), ctx->y.as<double>(),
+                       ctx->ex.as<int32_t>(), ctx->Xa.as<double>(), ctx->Ya.as<double>(),
+                       ctx->psfconst, ctx->Btab.as<double>(), (int64_t)ctx->n1 * ctx->n2,
+                       ctx->n1, ctx->n2, np / 2, npad, psfxscale, psfyscale,
+                       ctx->B.as<double>()),
+       "lookup_B");
+  }
   ctx->have_B = true;
   CK(download(ctx, B_out, ctx->B, (size_t)n * m * 8), "download B");
   return IMCOM_OK;
@@ -372,10 +377,12 @@ int imcom_gpu_build_P(imcom_gpu_ctx* ctx, int precision, double* P_out) {
   CK(fetch_guard(ctx, 1u << IMCOM_FETCH_P), "build_P: wait for fetches");
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->P.reserve((size_t)n * m * 8), "alloc P");
-  StageTimer tm(ctx, PS_BUILD_P);
-  int rc = gemm_prec(ctx, precision, true, (int)n, (int)m, (int)n, 1.0, ctx->Q.as<double>(), n,
-                     ctx->B.as<double>(), n, ctx->P.as<double>(), n, "P = Q^T B");
-  if (rc) return rc;
+  {
+    StageTimer tm(ctx, PS_BUILD_P);
+    int rc = gemm_prec(ctx, precision, true, (int)n, (int)m, (int)n, 1.0, ctx->Q.as<double>(), n,
+                       ctx->B.as<double>(), n, ctx->P.as<double>(), n, "P = Q^T B");
+    if (rc) return rc;
+  }
   ctx->have_P = true;
   ctx->have_K = ctx->have_T = false;
   CK(download(ctx, P_out, ctx->P, (size_t)n * m * 8), "download P");
@@ -411,12 +418,14 @@ int imcom_gpu_bisect_kappa(imcom_gpu_ctx* ctx, int UnotS, double US_max, double 
     CK(upload(ctx, ctx->bracket, bracket, (size_t)m * 2 * sizeof(float)), "upload bracket");
     dbr = ctx->bracket.as<float>();
   }
-  StageTimer tm(ctx, PS_BISECT);
-  CK(launch_bisect(ctx->stream, n, m, ctx->L.as<double>(), ctx->P.as<double>(), n,
-                   ctx->Nd.as<double>(), ctx->C_a, US_max, tol, k_min, k_max, maxnits,
-                   UnotS ? 1 : 0, dbr, ctx->K.as<double>(), ctx->US.as<double>(),
-                   ctx->evals.as<int32_t>(), nullptr),
-     "bisect");
+  {
+    StageTimer tm(ctx, PS_BISECT);
+    CK(launch_bisect(ctx->stream, n, m, ctx->L.as<double>(), ctx->P.as<double>(), n,
+                     ctx->Nd.as<double>(), ctx->C_a, US_max, tol, k_min, k_max, maxnits,
+                     UnotS ? 1 : 0, dbr, ctx->K.as<double>(), ctx->US.as<double>(),
+                     ctx->evals.as<int32_t>(), nullptr),
+       "bisect");
+  }
   ctx->have_K = true;
   ctx->have_T = false;
   CK(download(ctx, K_out, ctx->K, m * 8), "download K");
@@ -434,13 +443,15 @@ int imcom_gpu_build_T(imcom_gpu_ctx* ctx, int precision, double* T_out) {
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->W.reserve((size_t)n * m * 8), "alloc W");
   CK(ctx->T.reserve((size_t)n * m * 8), "alloc T");
-  StageTimer tm(ctx, PS_BUILD_T);
-  CK(launch_scale_W(ctx->stream, n, m, ctx->P.as<double>(), n, ctx->L.as<double>(),
-                    ctx->K.as<double>(), ctx->W.as<double>(), n),
-     "W = P / (L + K)");
-  int rc = gemm_prec(ctx, precision, false, (int)n, (int)m, (int)n, -0.5, ctx->Q.as<double>(), n,
-                     ctx->W.as<double>(), n, ctx->T.as<double>(), n, "T = -Q W / 2");
-  if (rc) return rc;
+  {
+    StageTimer tm(ctx, PS_BUILD_T);
+    CK(launch_scale_W(ctx->stream, n, m, ctx->P.as<double>(), n, ctx->L.as<double>(),
+                      ctx->K.as<double>(), ctx->W.as<double>(), n),
+       "W = P / (L + K)");
+    int rc = gemm_prec(ctx, precision, false, (int)n, (int)m, (int)n, -0.5, ctx->Q.as<double>(),
+                       n, ctx->W.as<double>(), n, ctx->T.as<double>(), n, "T = -Q W / 2");
+    if (rc) return rc;
+  }
   ctx->have_T = true;
   CK(download(ctx, T_out, ctx->T, (size_t)n * m * 8), "download T");
   return IMCOM_OK;

@@ -223,7 +223,7 @@ def build_S(T, Ndiag):
 
 
 def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, cols_P=128,
-                     pix_bisect=48, pix_T=8):
+                     pix_bisect=48, pix_T=8, checks=False):
     """Time the oracle's solve path on bounded samples of one tile and scale
     each stage to the whole tile (the CPU-baseline leg of bench.py):
 
@@ -235,7 +235,12 @@ def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, col
       build_T    pix_T DGEMVs at full n, scaled by m
 
     L and P (the tile's eigenvalues and projections, any source) feed the
-    bisection sample.  Returns (seconds for the whole tile, dict of stages).
+    bisection sample.  With checks=True the scale factors are tested by a
+    second sample of twice the size: DSYEVR at 2 n_eig (the fitted exponent of
+    n is reported and the larger sample is the one scaled), A rows 2 rows_A
+    and T on 2 pix_T pixels (the ratios to the first samples are reported;
+    linear scaling gives 2).  Returns (seconds for the whole tile, dict of
+    stages, dict of checks).
     """
     import time
     n, m = tile.n, tile.m
@@ -254,6 +259,20 @@ def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, col
     dt = time.perf_counter() - t0
     done = rows_A * n - rows_A * (rows_A - 1) / 2
     stages["lookup_A"] = dt * (n * (n + 1) / 2) / done
+    chk = {}
+    if checks and 2 * rows_A <= n:
+        r2 = 2 * rows_A
+        t0 = time.perf_counter()
+        lib().or_lookup_A_rows(ctypes.c_int64(n), ctypes.c_int64(0), ctypes.c_int64(r2), _p(x),
+                               _p(y), ex.ctypes.data_as(_ip), ctypes.c_int(tile.psfconst),
+                               ctypes.c_int(tab.shape[0]), ctypes.c_int(tab.shape[1]), _p(tab),
+                               ctypes.c_int(tile.npad), ctypes.c_int(tile.npoly),
+                               ctypes.c_double(tile.psfxscale), ctypes.c_double(tile.psfyscale),
+                               _p(A))
+        dt2 = time.perf_counter() - t0
+        done2 = r2 * n - r2 * (r2 - 1) / 2
+        chk["lookup_A_time_ratio_2x_entries"] = round((dt2 / dt) * (done / done2) * 2, 3)
+        stages["lookup_A"] = dt2 * (n * (n + 1) / 2) / done2
     # leading block of A for the eigen sample (computed, not timed)
     ne = min(n_eig, n)
     sub = tile.__class__(**{**tile.__dict__})
@@ -270,7 +289,20 @@ def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, col
     stages["lookup_B"] = (time.perf_counter() - t0) * n / rb
     t0 = time.perf_counter()
     eigen_A(Ae)
-    stages["eigen_A"] = (time.perf_counter() - t0) * (n / ne) ** 3
+    te = time.perf_counter() - t0
+    stages["eigen_A"] = te * (n / ne) ** 3
+    if checks and 2 * ne <= n:
+        sub2 = tile.__class__(**{**tile.__dict__})
+        sub2.n = 2 * ne
+        sub2.x_i, sub2.y_i, sub2.exp_i = tile.x_i[:2 * ne], tile.y_i[:2 * ne], tile.exp_i[:2 * ne]
+        Ae2 = lookup_A(sub2)
+        t0 = time.perf_counter()
+        eigen_A(Ae2)
+        te2 = time.perf_counter() - t0
+        del Ae2
+        chk["eigen_A_fitted_exponent"] = round(float(np.log2(te2 / te)), 3)
+        chk["eigen_A_sizes"] = [ne, 2 * ne]
+        stages["eigen_A"] = te2 * (n / (2 * ne)) ** 3
     rng = np.random.default_rng(0)
     Qr = np.asfortranarray(rng.standard_normal((n, n)))
     cp = min(cols_P, m)
@@ -292,8 +324,17 @@ def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, col
     T = build_T(Qr, np.asfortranarray(np.asarray(P)[:, :pt]), L, K[:pt] if pt <= pb else np.full(pt, K[0]))
     build_S(T, tile.Ndiag)
     build_image(T, tile.I_i)
-    stages["build_T"] = (time.perf_counter() - t0) * m / pt
-    return float(sum(stages.values())), stages
+    tt = time.perf_counter() - t0
+    stages["build_T"] = tt * m / pt
+    if checks and 2 * pt <= min(m, pb):
+        t0 = time.perf_counter()
+        T = build_T(Qr, np.asfortranarray(np.asarray(P)[:, :2 * pt]), L, K[:2 * pt])
+        build_S(T, tile.Ndiag)
+        build_image(T, tile.I_i)
+        tt2 = time.perf_counter() - t0
+        chk["build_T_time_ratio_2x_pixels"] = round(tt2 / tt, 3)
+        stages["build_T"] = tt2 * m / (2 * pt)
+    return float(sum(stages.values())), stages, chk
 
 
 def solve(tile, Q=None, L=None):
