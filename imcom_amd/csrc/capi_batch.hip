@@ -6,11 +6,12 @@
 //
 // Execution: lookup_A of every tile, then the tridiagonal reduction of all
 // tiles in lockstep (blockIdx.z = tile, one grouped GEMM per panel for the
-// trailing updates), the divide-and-conquer of all tiles level by level, the
+// trailing updates), the divide-and-conquer of all tiles level by level (a
+// launch plan fixed at batch_set; deflation counts stay on the device), the
 // back-transform (grouped GEMMs per block), and per-tile lookup_B, P, bisection
-// and T.  With use_graph the two launch sequences around the divide-and-conquer
-// (which reads deflation counts on the host once per level) are captured once
-// per batch into HIP graphs and replayed.
+// and T.  With use_graph the whole sequence is captured once per batch (and
+// per precision / bracket setting) into one HIP graph and replayed: one
+// hipGraphLaunch per solve.
 #include "ctx.h"
 
 #include <algorithm>
@@ -31,18 +32,24 @@ struct BatchState {
   int launch_P = -1, launch_T = -1;
   int64_t nmax = 0;
   int nb = 32, nbb = 128;
-  hipGraphExec_t g1 = nullptr, g2 = nullptr;
-  int g2_key = -1;
-  // split-precision workspace address baked into g2's products at capture;
-  // the single-tile entry points may reallocate that buffer in between
-  const void* g2_split = nullptr;
-  ~BatchState() { clear(); }
+  hipGraphExec_t graph = nullptr;
+  int graph_key = -1;
+  // split-precision workspace address baked into the graph's products at
+  // capture; the single-tile entry points may reallocate that buffer
+  const void* graph_split = nullptr;
+  // the batch's own divide-and-conquer workspace and launch plan (the
+  // single-tile path's plan changes with its tile size; a captured graph
+  // keeps pointing at this one)
+  DcWork* dc = nullptr;
+  ~BatchState() {
+    clear();
+    dc_destroy(dc);
+  }
   void drop_graphs() {
-    if (g1) (void)hipGraphExecDestroy(g1);
-    if (g2) (void)hipGraphExecDestroy(g2);
-    g1 = g2 = nullptr;
-    g2_key = -1;
-    g2_split = nullptr;
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    graph_key = -1;
+    graph_split = nullptr;
   }
   void clear() {
     drop_graphs();
@@ -141,6 +148,17 @@ int imcom_gpu_batch_set(imcom_gpu_ctx* ctx, int ntiles, const imcom_gpu_tile* ti
     B->nmax = std::max(B->nmax, n);
   }
   CK(upload(ctx, B->d_sy, host.data(), host.size() * sizeof(SyTile)), "upload tile descriptors");
+  {
+    if (!B->dc) B->dc = dc_create();
+    std::vector<int64_t> ns;
+    std::vector<double*> zp;
+    for (BatchTile* t : B->tiles) {
+      ns.push_back(t->in.n);
+      zp.push_back(t->ew.sZ.as<double>());
+    }
+    CK(dc_prepare(ctx->stream, B->dc, (int)ns.size(), ns.data(), zp.data()),
+       "divide-and-conquer plan");
+  }
   sy_make_plan(host, B->nmax, B->nb, B->nbb, B->plan);
   CK(B->plan.gp.upload(ctx->stream), "upload reduction plan");
   std::vector<GemmDesc> gP, gT;
@@ -244,13 +262,37 @@ static hipError_t batch_seq2(imcom_gpu_ctx* ctx, int prec_P, int prec_T, int lea
   return hipGetLastError();
 }
 
-static hipError_t run_graph(imcom_gpu_ctx* ctx, hipGraphExec_t* exec, int which, int prec_P,
-                            int prec_T, int learned, float zsig) {
+// the divide-and-conquer of every tile's tridiagonal, level by level
+static hipError_t batch_dc(imcom_gpu_ctx* ctx) {
+  BatchState* B = ctx->batch;
+  std::vector<int64_t> ns;
+  std::vector<const double*> dp, ep;
+  std::vector<double*> lp, zp;
+  for (BatchTile* t : B->tiles) {
+    ns.push_back(t->in.n);
+    dp.push_back(t->ew.sd.as<double>());
+    ep.push_back(t->ew.se.as<double>());
+    lp.push_back(t->ew.slam.as<double>());
+    zp.push_back(t->ew.sZ.as<double>());
+  }
+  return dc_solve_batched(ctx->stream, B->dc, (int)ns.size(), ns.data(), dp.data(), ep.data(),
+                          lp.data(), zp.data());
+}
+
+// the whole solve: reduction, divide-and-conquer, back-transform to the maps
+static hipError_t batch_all(imcom_gpu_ctx* ctx, int prec_P, int prec_T, int learned, float zsig) {
+  IMCOM_HIP_TRY(batch_seq1(ctx));
+  IMCOM_HIP_TRY(batch_dc(ctx));
+  return batch_seq2(ctx, prec_P, prec_T, learned, zsig);
+}
+
+static hipError_t run_graph(imcom_gpu_ctx* ctx, hipGraphExec_t* exec, int prec_P, int prec_T,
+                            int learned, float zsig) {
   hipStream_t st = ctx->stream;
   if (!*exec) {
     hipGraph_t g = nullptr;
     IMCOM_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    hipError_t e = which == 1 ? batch_seq1(ctx) : batch_seq2(ctx, prec_P, prec_T, learned, zsig);
+    hipError_t e = batch_all(ctx, prec_P, prec_T, learned, zsig);
     hipError_t e2 = hipStreamEndCapture(st, &g);
     if (e != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
@@ -287,43 +329,29 @@ int imcom_gpu_batch_solve(imcom_gpu_ctx* ctx, int precision_P, int precision_T, 
   const int key = (learned ? 1 + (int)(zsig * 1000.0f) : 0) * 16 + precision_P * 4 + precision_T;
   // re-capture when the controls change or when the split workspace moved
   // (grown here, or reallocated by imcom_gpu_build_P / _build_T since)
-  if (B->g2 && (B->g2_key != key || B->g2_split != ctx->split_work.p)) {
-    (void)hipGraphExecDestroy(B->g2);
-    B->g2 = nullptr;
+  if (B->graph && (B->graph_key != key || B->graph_split != ctx->split_work.p)) {
+    (void)hipGraphExecDestroy(B->graph);
+    B->graph = nullptr;
   }
-  B->g2_key = key;
-  B->g2_split = ctx->split_work.p;
-  {
+  B->graph_key = key;
+  B->graph_split = ctx->split_work.p;
+  if (use_graph) {
+    // one graph: the whole solve lands in the reduction's time slot
     StageTimer tm(ctx, PS_SYTRD);
-    if (use_graph)
-      CK(run_graph(ctx, &B->g1, 1, precision_P, precision_T, learned, zsig),
-         "batch reduction (graph)");
-    else
+    CK(run_graph(ctx, &B->graph, precision_P, precision_T, learned, zsig), "batch solve (graph)");
+  } else {
+    {
+      StageTimer tm(ctx, PS_SYTRD);
       CK(batch_seq1(ctx), "batch reduction");
-  }
-  {
-    StageTimer tm(ctx, PS_STEDC);
-    std::vector<int64_t> ns;
-    std::vector<const double*> dp, ep;
-    std::vector<double*> lp, zp;
-    for (BatchTile* t : B->tiles) {
-      ns.push_back(t->in.n);
-      dp.push_back(t->ew.sd.as<double>());
-      ep.push_back(t->ew.se.as<double>());
-      lp.push_back(t->ew.slam.as<double>());
-      zp.push_back(t->ew.sZ.as<double>());
     }
-    CK(dc_solve_batched(ctx->stream, ctx->dc, (int)ns.size(), ns.data(), dp.data(), ep.data(),
-                        lp.data(), zp.data()),
-       "batch divide-and-conquer");
-  }
-  {
-    StageTimer tm(ctx, PS_ORMTR);
-    if (use_graph)
-      CK(run_graph(ctx, &B->g2, 2, precision_P, precision_T, learned, zsig),
-         "batch back-transform to T (graph)");
-    else
+    {
+      StageTimer tm(ctx, PS_STEDC);
+      CK(batch_dc(ctx), "batch divide-and-conquer");
+    }
+    {
+      StageTimer tm(ctx, PS_ORMTR);
       CK(batch_seq2(ctx, precision_P, precision_T, learned, zsig), "batch back-transform to T");
+    }
   }
   CK(hipStreamSynchronize(ctx->stream), "batch_solve sync");
   return IMCOM_OK;

@@ -219,6 +219,10 @@ DcWork* dc_create();
 void dc_destroy(DcWork* w);
 hipError_t dc_solve(hipStream_t st, DcWork* w, int64_t n, const double* d, const double* e,
                     double* lam, double* Z);
+// Builds the launch plan of dc_solve_batched for these tile sizes and
+// eigenvector buffers (host work and uploads; synchronises st): call before
+// capturing a dc_solve_batched into a HIP graph.
+hipError_t dc_prepare(hipStream_t st, DcWork* w, int ntiles, const int64_t* ns, double* const* Z_t);
 hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t* ns,
                             const double* const* d_t, const double* const* e_t,
                             double* const* lam_t, double* const* Z_t);

@@ -41,15 +41,16 @@ struct DcMerge {
   int loff;          // offset of the merge inside its tile
   double rho;    // |beta|
   double sigma;  // sign(beta)
-  int64_t u_off;   // offset of this merge's k x k U block
+  int64_t u_off;   // offset of this merge's k x k U block (inside a K x K slot)
   int64_t zg_off;  // offset of this merge's K x K gather block (and K x k output)
-  int root_off;    // offset of this merge's roots in the level's root arrays
 };
 
 struct DcCount {
   int k;     // non-deflated
   int nrot;  // deflation rotations
 };
+
+struct DcPlan;
 
 struct DcWork {
   int64_t n = 0;    // capacity of the concatenated vectors
@@ -63,7 +64,7 @@ struct DcWork {
   DcMerge* d_merges = nullptr;
   DcCount* d_counts = nullptr;
   GemmDesc* d_descs = nullptr;
-  DcCount* h_counts = nullptr;  // pinned
+  DcPlan* plan = nullptr;  // launch plan for the last tile sizes (kept across dc_free)
 };
 
 static void dc_free(DcWork* w) {
@@ -73,14 +74,18 @@ static void dc_free(DcWork* w) {
                   w->rn,     w->org, w->d_prefix, w->d_merges, w->d_counts, w->d_descs};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
-  if (w->h_counts) (void)hipHostFree(w->h_counts);
+  DcPlan* plan = w->plan;
   *w = DcWork();
+  w->plan = plan;
 }
+
+static void dc_plan_delete(DcPlan* p);
 
 DcWork* dc_create() { return new DcWork(); }
 void dc_destroy(DcWork* w) {
   if (!w) return;
   dc_free(w);
+  dc_plan_delete(w->plan);
   delete w;
 }
 
@@ -104,7 +109,6 @@ static hipError_t dc_reserve(DcWork* w, int64_t n, int64_t nn) {
   IMCOM_HIP_TRY(hipMalloc(&w->d_merges, (size_t)n * sizeof(DcMerge)));
   IMCOM_HIP_TRY(hipMalloc(&w->d_counts, (size_t)n * sizeof(DcCount)));
   IMCOM_HIP_TRY(hipMalloc(&w->d_descs, (size_t)n * sizeof(GemmDesc)));
-  IMCOM_HIP_TRY(hipHostMalloc(&w->h_counts, (size_t)n * sizeof(DcCount)));
   w->n = n;
   w->nn = nn;
   return hipSuccess;
@@ -630,7 +634,44 @@ __global__ void dc_unscale_kernel(int64_t n, const double* __restrict__ D,
   if (i < n) lam[i] = D[i] * scale[0];
 }
 
-// ---- driver -----------------------------------------------------------------
+// ---- plan-time set-up kernels --------------------------------------------------
+// fill (in place of a memset, which a captured graph must not hold)
+__global__ void dc_fill_kernel(int64_t n, double v, double* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// coupling of every merge of every level: beta = e(off + n1 - 1) / scale
+__global__ void dc_rho_kernel(DcMerge* __restrict__ mg, int nm, const double* __restrict__ e,
+                              const double* __restrict__ scale) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  const double beta = e[mg[i].off + mg[i].n1 - 1] / scale[0];
+  mg[i].rho = fabs(beta);
+  mg[i].sigma = beta < 0.0 ? -1.0 : 1.0;
+}
+
+// the level's eigenvector products Zo_m (K x k) = Zg_m(:, 0:k) U_m (k x k,
+// ld k) take their k from the deflation counts on the device: descriptors
+// built for k = K are cut down here; blocks beyond a product's tiles exit
+__global__ void dc_fix_descs_kernel(GemmDesc* __restrict__ d, const int* __restrict__ dmerge,
+                                    int nd, const DcCount* __restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nd) return;
+  const int k = counts[dmerge[i]].k;
+  d[i].N = k;
+  d[i].K = k;
+  d[i].ldb = k > 0 ? k : 1;
+}
+
+// ---- plan -------------------------------------------------------------------
+// Everything the levels need that depends only on the tile sizes (the merge
+// trees, offsets, launch grids) is fixed on the host once, before any stream
+// capture; what depends on the matrix (couplings, deflation counts, product
+// sizes) is formed on the device.  A level's roots, U blocks and products are
+// laid out for the worst case (no deflation: k = K), and the kernels skip the
+// roots j >= k, so the whole solve is a fixed launch sequence with no host
+// synchronisation: a HIP graph can hold it.
 struct Node {
   int tile, off, size, height;
 };
@@ -646,144 +687,226 @@ static int build_tree(int tile, int off, int size, std::vector<std::vector<Node>
   return h;
 }
 
+// merges whose worst-case product has at least this many 128x128 tiles go to
+// the level's 128-tile grouped launch, the rest to its 64-tile launch
+constexpr int kDc128Min = 512;
+
+struct DcLevel {
+  int merge_off, nm, maxK, roots;
+  int desc_off[2], ndesc[2], blocks[2];  // [0]: 64x64 tiles, [1]: 128x128
+};
+
+struct DcPlan {
+  std::vector<int64_t> ns;
+  std::vector<double*> Z;
+  std::vector<int64_t> base;
+  int64_t N = 0;
+  std::vector<DcLevel> levels;
+  int nmerges = 0, ndescs = 0;
+  DcMerge* d_merges = nullptr;  // all levels
+  int* d_rprefix = nullptr;     // per level: nm + 1 root prefixes over K
+  GemmDesc* d_descs = nullptr;  // per level and tile size
+  int* d_dmerge = nullptr;      // descriptor -> merge index within its level
+  int* d_gprefix = nullptr;     // grouped-launch block prefixes
+  std::vector<int> rprefix_off, gprefix_off[2];
+  void release() {
+    void* p[] = {d_merges, d_rprefix, d_descs, d_dmerge, d_gprefix};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+    *this = DcPlan();
+  }
+};
+
+DcPlan* dc_plan_of(DcWork* w) {
+  if (!w->plan) w->plan = new DcPlan();
+  return w->plan;
+}
+
+static void dc_plan_delete(DcPlan* p) {
+  if (!p) return;
+  p->release();
+  delete p;
+}
+
+static hipError_t dc_build_plan(hipStream_t st, DcWork* w, DcPlan& P, int ntiles,
+                                const int64_t* ns, double* const* Z_t) {
+  P.release();
+  P.ns.assign(ns, ns + ntiles);
+  P.Z.assign(Z_t, Z_t + ntiles);
+  P.base.assign(ntiles + 1, 0);
+  int64_t nn = 0;
+  for (int t = 0; t < ntiles; ++t) {
+    P.base[t + 1] = P.base[t] + ns[t];
+    nn += ns[t] * ns[t];
+  }
+  P.N = P.base[ntiles];
+  IMCOM_HIP_TRY(dc_reserve(w, P.N, nn));
+  std::vector<std::vector<Node>> levels;
+  for (int t = 0; t < ntiles; ++t) build_tree(t, 0, (int)ns[t], levels);
+  std::vector<DcMerge> hm;
+  std::vector<int> rpre, gpre, dmerge;
+  std::vector<GemmDesc> descs;
+  for (std::vector<Node>& lv : levels) {
+    std::sort(lv.begin(), lv.end(), [&](const Node& a, const Node& b) {
+      return P.base[a.tile] + a.off < P.base[b.tile] + b.off;
+    });
+    DcLevel L{};
+    L.merge_off = (int)hm.size();
+    L.nm = (int)lv.size();
+    P.rprefix_off.push_back((int)rpre.size());
+    rpre.push_back(0);
+    int64_t zoff = 0;
+    std::vector<GemmDesc> dl[2];
+    std::vector<int> ml[2];
+    for (int i = 0; i < L.nm; ++i) {
+      const Node& nd = lv[i];
+      DcMerge m{};
+      m.n1 = (nd.size + 1) / 2;
+      m.n2 = nd.size - m.n1;
+      m.off = (int)(P.base[nd.tile] + nd.off);
+      m.Z = Z_t[nd.tile];
+      m.ldz = ns[nd.tile];
+      m.loff = nd.off;
+      m.zg_off = zoff;
+      m.u_off = zoff;  // a K x K block holds the k x k U
+      hm.push_back(m);
+      rpre.push_back(rpre.back() + nd.size);
+      const int K = nd.size;
+      zoff += (int64_t)K * K;
+      L.maxK = std::max(L.maxK, K);
+      const int big = (int64_t)((K + 127) / 128) * ((K + 127) / 128) >= kDc128Min ? 1 : 0;
+      dl[big].push_back(GemmDesc{K, K, K, 1.0, 0.0, w->Zg + m.zg_off, K, w->U + m.u_off, K,
+                                 w->Zo + m.zg_off, K});
+      ml[big].push_back(i);
+    }
+    L.roots = rpre.back() - rpre[P.rprefix_off.back()];
+    for (int b = 0; b < 2; ++b) {
+      const int tile = b ? 128 : 64;
+      L.desc_off[b] = (int)descs.size();
+      L.ndesc[b] = (int)dl[b].size();
+      P.gprefix_off[b].push_back((int)gpre.size());
+      gpre.push_back(0);
+      for (size_t q = 0; q < dl[b].size(); ++q) {
+        descs.push_back(dl[b][q]);
+        dmerge.push_back(ml[b][q]);
+        const int K = dl[b][q].M;
+        gpre.push_back(gpre.back() + ((K + tile - 1) / tile) * ((K + tile - 1) / tile));
+      }
+      L.blocks[b] = gpre.back();
+    }
+    P.levels.push_back(L);
+  }
+  // root prefixes are per level (relative to the level's first entry)
+  for (size_t l = 0; l < P.levels.size(); ++l) {
+    const int o = P.rprefix_off[l];
+    const int first = rpre[o];
+    for (int i = 0; i <= P.levels[l].nm; ++i) rpre[o + i] -= first;
+  }
+  P.nmerges = (int)hm.size();
+  P.ndescs = (int)descs.size();
+  auto up = [&](void** dst, const void* src, size_t bytes) -> hipError_t {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e != hipSuccess) return e;
+    return src ? hipMemcpyAsync(*dst, src, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+  };
+  IMCOM_HIP_TRY(up((void**)&P.d_merges, hm.data(), hm.size() * sizeof(DcMerge)));
+  IMCOM_HIP_TRY(up((void**)&P.d_rprefix, rpre.data(), rpre.size() * sizeof(int)));
+  IMCOM_HIP_TRY(up((void**)&P.d_descs, descs.empty() ? nullptr : descs.data(),
+                   descs.size() * sizeof(GemmDesc)));
+  IMCOM_HIP_TRY(up((void**)&P.d_dmerge, dmerge.empty() ? nullptr : dmerge.data(),
+                   dmerge.size() * sizeof(int)));
+  IMCOM_HIP_TRY(up((void**)&P.d_gprefix, gpre.data(), gpre.size() * sizeof(int)));
+  // the host vectors must outlive the copies; this runs before any capture
+  return hipStreamSynchronize(st);
+}
+
+static bool dc_plan_matches(const DcPlan& P, int ntiles, const int64_t* ns, double* const* Z_t) {
+  if ((int)P.ns.size() != ntiles || P.d_merges == nullptr) return false;
+  for (int t = 0; t < ntiles; ++t)
+    if (P.ns[t] != ns[t] || P.Z[t] != Z_t[t]) return false;
+  return true;
+}
+
+hipError_t dc_prepare(hipStream_t st, DcWork* w, int ntiles, const int64_t* ns, double* const* Z_t) {
+  DcPlan& P = *dc_plan_of(w);
+  if (dc_plan_matches(P, ntiles, ns, Z_t)) return hipSuccess;
+  return dc_build_plan(st, w, P, ntiles, ns, Z_t);
+}
+
 // Tiles' tridiagonals (d_t: n_t, e_t: n_t - 1, device) solved together: their
 // vectors are concatenated (with zero coupling between tiles), each tile keeps
 // its own balanced merge tree, and merges of equal height across all tiles
 // share the launches of a level.  lam_t (n_t) ascending, Z_t (n_t x n_t).
+// The plan for these sizes and eigenvector buffers is built on first use
+// (dc_prepare); inside a stream capture it must exist already.
 hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t* ns,
                             const double* const* d_t, const double* const* e_t,
                             double* const* lam_t, double* const* Z_t) {
-  std::vector<int64_t> base(ntiles + 1, 0);
-  int64_t nn = 0;
-  for (int t = 0; t < ntiles; ++t) {
-    base[t + 1] = base[t] + ns[t];
-    nn += ns[t] * ns[t];
+  DcPlan& P = *dc_plan_of(w);
+  if (!dc_plan_matches(P, ntiles, ns, Z_t)) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    IMCOM_HIP_TRY(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return hipErrorStreamCaptureUnsupported;
+    IMCOM_HIP_TRY(dc_build_plan(st, w, P, ntiles, ns, Z_t));
   }
-  const int64_t N = base[ntiles];
+  const int64_t N = P.N;
   if (N <= 0) return hipSuccess;
-  IMCOM_HIP_TRY(dc_reserve(w, N, nn));
-  IMCOM_HIP_TRY(hipMemsetAsync(w->ecat, 0, (size_t)N * sizeof(double), st));
+  hipLaunchKernelGGL(dc_fill_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, 0.0, w->ecat);
   for (int t = 0; t < ntiles; ++t) {
-    IMCOM_HIP_TRY(hipMemcpyAsync(w->dcat + base[t], d_t[t], ns[t] * sizeof(double),
+    IMCOM_HIP_TRY(hipMemcpyAsync(w->dcat + P.base[t], d_t[t], ns[t] * sizeof(double),
                                  hipMemcpyDeviceToDevice, st));
     if (ns[t] > 1)
-      IMCOM_HIP_TRY(hipMemcpyAsync(w->ecat + base[t], e_t[t], (ns[t] - 1) * sizeof(double),
+      IMCOM_HIP_TRY(hipMemcpyAsync(w->ecat + P.base[t], e_t[t], (ns[t] - 1) * sizeof(double),
                                    hipMemcpyDeviceToDevice, st));
     hipLaunchKernelGGL(dc_identity_kernel, dim3((unsigned)ns[t]), dim3(256), 0, st, ns[t], Z_t[t]);
   }
   hipLaunchKernelGGL(dc_scale_kernel, dim3(1), dim3(1024), 0, st, N, w->dcat, w->ecat, w->scale);
   hipLaunchKernelGGL(dc_leaves_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->dcat, w->ecat,
                      w->scale, w->D);
-  std::vector<std::vector<Node>> levels;
-  for (int t = 0; t < ntiles; ++t) build_tree(t, 0, (int)ns[t], levels);
-  std::vector<double> he(N, 0.0);
-  double hscale = 1.0;
-  IMCOM_HIP_TRY(hipMemcpyAsync(he.data(), w->ecat, N * sizeof(double), hipMemcpyDeviceToHost, st));
-  IMCOM_HIP_TRY(hipMemcpyAsync(&hscale, w->scale, sizeof(double), hipMemcpyDeviceToHost, st));
-  IMCOM_HIP_TRY(hipStreamSynchronize(st));
-  std::vector<DcMerge> hm;
-  std::vector<int> rprefix, bprefix;
-  std::vector<GemmDesc> descs;
-  for (size_t L = 0; L < levels.size(); ++L) {
-    std::vector<Node>& lv = levels[L];
-    std::sort(lv.begin(), lv.end(), [&](const Node& a, const Node& b) {
-      return base[a.tile] + a.off < base[b.tile] + b.off;
-    });
-    const int nm = (int)lv.size();
-    hm.resize(nm);
-    int64_t zoff = 0;
-    int maxK = 0;
-    for (int i = 0; i < nm; ++i) {
-      const Node& nd = lv[i];
-      const int n1 = (nd.size + 1) / 2;
-      const int64_t g = base[nd.tile] + nd.off;
-      const double beta = he[g + n1 - 1] / hscale;
-      hm[i].off = (int)g;
-      hm[i].n1 = n1;
-      hm[i].n2 = nd.size - n1;
-      hm[i].Z = Z_t[nd.tile];
-      hm[i].ldz = ns[nd.tile];
-      hm[i].loff = nd.off;
-      hm[i].rho = fabs(beta);
-      hm[i].sigma = beta < 0.0 ? -1.0 : 1.0;
-      hm[i].zg_off = zoff;
-      hm[i].u_off = 0;
-      hm[i].root_off = 0;
-      zoff += (int64_t)nd.size * nd.size;
-      maxK = std::max(maxK, nd.size);
+  if (P.nmerges > 0)
+    hipLaunchKernelGGL(dc_rho_kernel, dim3(ceil_div(P.nmerges, 256)), dim3(256), 0, st, P.d_merges,
+                       P.nmerges, w->ecat, w->scale);
+  for (size_t l = 0; l < P.levels.size(); ++l) {
+    const DcLevel& L = P.levels[l];
+    const int nm = L.nm;
+    const DcMerge* mg = P.d_merges + L.merge_off;
+    DcCount* counts = w->d_counts + L.merge_off;
+    const int* rpre = P.d_rprefix + P.rprefix_off[l];
+    hipLaunchKernelGGL(dc_sort_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, mg, nm, w->D,
+                       (const double*)nullptr, w->ds, w->zs, w->cs);
+    hipLaunchKernelGGL(dc_deflate_kernel, dim3(ceil_div(nm, 64)), dim3(64), 0, st, mg, nm, w->ds,
+                       w->zs, w->cs, w->dl, w->wz, w->ndcol, w->dfd, w->dfcol, w->rp, w->rn, w->rc,
+                       w->rs, counts);
+    hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, mg, nm,
+                       (const DcCount*)counts, w->rp, w->rn, w->rc, w->rs, (double*)nullptr);
+    hipLaunchKernelGGL(dc_secular_kernel, dim3(ceil_div(L.roots, 4)), dim3(256), 0, st, mg,
+                       (const DcCount*)counts, rpre, nm, L.roots, w->dl, w->wz, w->org, w->tau,
+                       w->lam);
+    hipLaunchKernelGGL(dc_zhat_kernel, dim3(ceil_div(L.roots, 64)), dim3(64), 0, st, mg,
+                       (const DcCount*)counts, rpre, nm, L.roots, w->dl, w->wz, w->org, w->tau,
+                       w->zhat);
+    hipLaunchKernelGGL(dc_vec_kernel, dim3(L.roots), dim3(256), 0, st, mg, (const DcCount*)counts,
+                       rpre, nm, w->dl, w->org, w->tau, w->zhat, w->U);
+    const int tpb = std::min(256, std::max(64, (L.maxK + 63) / 64 * 64));
+    hipLaunchKernelGGL(dc_gather_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, mg, nm,
+                       (const DcCount*)counts, w->ndcol, w->dfcol, (const double*)nullptr, w->Zg);
+    // Zo_m = Zg_m(:, 0:k) U_m, with k from the device counts
+    for (int b = 0; b < 2; ++b) {
+      if (L.ndesc[b] == 0) continue;
+      GemmDesc* d = P.d_descs + L.desc_off[b];
+      hipLaunchKernelGGL(dc_fix_descs_kernel, dim3(ceil_div(L.ndesc[b], 64)), dim3(64), 0, st, d,
+                         P.d_dmerge + L.desc_off[b], L.ndesc[b], (const DcCount*)counts);
+      IMCOM_HIP_TRY(dgemm_grouped(st, false, false, d, P.d_gprefix + P.gprefix_off[b][l],
+                                  L.ndesc[b], L.blocks[b], b ? 128 : 64));
     }
-    IMCOM_HIP_TRY(hipMemcpyAsync(w->d_merges, hm.data(), nm * sizeof(DcMerge), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(dc_sort_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->d_merges, nm,
-                       w->D, (const double*)nullptr, w->ds, w->zs, w->cs);
-    hipLaunchKernelGGL(dc_deflate_kernel, dim3(ceil_div(nm, 64)), dim3(64), 0, st, w->d_merges, nm,
-                       w->ds, w->zs, w->cs, w->dl, w->wz, w->ndcol, w->dfd, w->dfcol, w->rp, w->rn,
-                       w->rc, w->rs, w->d_counts);
-    IMCOM_HIP_TRY(hipMemcpyAsync(w->h_counts, w->d_counts, nm * sizeof(DcCount),
-                                 hipMemcpyDeviceToHost, st));
-    IMCOM_HIP_TRY(hipStreamSynchronize(st));
-    rprefix.assign(nm + 1, 0);
-    int64_t uoff = 0;
-    bool anyrot = false;
-    for (int i = 0; i < nm; ++i) {
-      const int k = w->h_counts[i].k;
-      rprefix[i + 1] = rprefix[i] + k;
-      hm[i].u_off = uoff;
-      hm[i].root_off = rprefix[i];
-      uoff += (int64_t)k * k;
-      anyrot = anyrot || w->h_counts[i].nrot > 0;
-    }
-    const int total = rprefix[nm];
-    IMCOM_HIP_TRY(hipMemcpyAsync(w->d_merges, hm.data(), nm * sizeof(DcMerge), hipMemcpyHostToDevice, st));
-    IMCOM_HIP_TRY(hipMemcpyAsync(w->d_prefix, rprefix.data(), (nm + 1) * sizeof(int),
-                                 hipMemcpyHostToDevice, st));
-    if (anyrot)
-      hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->d_merges,
-                         nm, w->d_counts, w->rp, w->rn, w->rc, w->rs, (double*)nullptr);
-    if (total > 0) {
-      hipLaunchKernelGGL(dc_secular_kernel, dim3(ceil_div(total, 4)), dim3(256), 0, st, w->d_merges,
-                         w->d_counts, w->d_prefix, nm, total, w->dl, w->wz, w->org, w->tau, w->lam);
-      hipLaunchKernelGGL(dc_zhat_kernel, dim3(ceil_div(total, 64)), dim3(64), 0, st, w->d_merges,
-                         w->d_counts, w->d_prefix, nm, total, w->dl, w->wz, w->org, w->tau, w->zhat);
-      hipLaunchKernelGGL(dc_vec_kernel, dim3(total), dim3(256), 0, st, w->d_merges, w->d_counts,
-                         w->d_prefix, nm, w->dl, w->org, w->tau, w->zhat, w->U);
-    }
-    const int tpb = std::min(256, std::max(64, (maxK + 63) / 64 * 64));
-    hipLaunchKernelGGL(dc_gather_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, w->d_merges, nm,
-                       w->d_counts, w->ndcol, w->dfcol, (const double*)nullptr, w->Zg);
-    // Zo_m (K x k) = Zg_m(:, 0:k) (K x k) * U_m (k x k): large merges as
-    // single GEMMs (128x128 tiles when they fill the GPU), the rest as one
-    // grouped launch of 64x64 tiles
-    descs.clear();
-    bprefix.assign(1, 0);
-    for (int i = 0; i < nm; ++i) {
-      const int K = lv[i].size, k = w->h_counts[i].k;
-      if (k == 0) continue;
-      if ((int64_t)((K + 127) / 128) * ((k + 127) / 128) >= 512) {
-        IMCOM_HIP_TRY(dgemm(st, false, false, K, k, k, 1.0, w->Zg + hm[i].zg_off, K,
-                            w->U + hm[i].u_off, k, 0.0, w->Zo + hm[i].zg_off, K));
-        continue;
-      }
-      descs.push_back(GemmDesc{K, k, k, 1.0, 0.0, w->Zg + hm[i].zg_off, K, w->U + hm[i].u_off, k,
-                               w->Zo + hm[i].zg_off, K});
-      bprefix.push_back(bprefix.back() + gemm_tiles64(K, k));
-    }
-    const int ng = (int)descs.size();
-    if (ng > 0) {
-      IMCOM_HIP_TRY(hipMemcpyAsync(w->d_descs, descs.data(), ng * sizeof(GemmDesc),
-                                   hipMemcpyHostToDevice, st));
-      IMCOM_HIP_TRY(hipMemcpyAsync(w->rp, bprefix.data(), (ng + 1) * sizeof(int),
-                                   hipMemcpyHostToDevice, st));
-      IMCOM_HIP_TRY(dgemm_grouped(st, false, false, w->d_descs, w->rp, ng, bprefix.back()));
-    }
-    hipLaunchKernelGGL(dc_place_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, w->d_merges, nm,
-                       w->d_counts, w->lam, w->dfd, w->Zg, w->Zo, w->D, (double*)nullptr);
-    // host vectors (hm, descs, prefixes) must outlive the async copies
-    IMCOM_HIP_TRY(hipStreamSynchronize(st));
+    hipLaunchKernelGGL(dc_place_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, mg, nm,
+                       (const DcCount*)counts, w->lam, w->dfd, w->Zg, w->Zo, w->D, (double*)nullptr);
   }
   hipLaunchKernelGGL(dc_unscale_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, w->D, w->scale,
                      w->lam);
   for (int t = 0; t < ntiles; ++t)
-    IMCOM_HIP_TRY(hipMemcpyAsync(lam_t[t], w->lam + base[t], ns[t] * sizeof(double),
+    IMCOM_HIP_TRY(hipMemcpyAsync(lam_t[t], w->lam + P.base[t], ns[t] * sizeof(double),
                                  hipMemcpyDeviceToDevice, st));
   return hipGetLastError();
 }

@@ -229,8 +229,9 @@ int imcom_gpu_bisect_kappa_learned_noalloc(imcom_gpu_ctx* ctx, void* stream, int
  * imcom_gpu_bisect_kappa, and the interpolation controls of
  * imcom_gpu_lookup_A/B).  imcom_gpu_batch_set uploads them;
  * imcom_gpu_batch_solve runs get_A, get_QL, get_B, get_P, get_T and
- * build_image for every tile (use_graph: replay the launch sequences as HIP
- * graphs, captured on the first call); imcom_gpu_batch_result returns one
+ * build_image for every tile (use_graph: the whole solve is captured into
+ * one HIP graph on the first call with these precision and bracket settings
+ * and replayed, one graph launch per solve); imcom_gpu_batch_result returns one
  * tile's K, U or S at the root (US_out), the other map (SU_out: S in U mode,
  * U in S mode), H, evaluation counts, bracket hits and T (any may be NULL). */
 typedef struct imcom_gpu_tile {

@@ -121,3 +121,28 @@ def test_batch_equals_single_tile_path(gpu_ctx, batch_tiles):
         same = close & (s["evals"] > 1)  # pixels that reached the target, same kappa
         assert np.all((np.abs(b["U"] - s["U"]) <= u_tol(r, t))[same])
         assert np.all((np.max(np.abs(b["T"] - s["T"]), axis=0) <= t_tol(r, t))[same])
+
+
+@pytest.mark.parametrize("prec_T", [0, 2])
+def test_graph_replays_equal_eager(gpu_ctx, batch_tiles, prec_T):
+    """The whole batched solve as one HIP graph, replayed three times with
+    the same context's single-tile work in between (which rebuilds its own
+    divide-and-conquer plan and, for BF16x6, grows the split-precision
+    workspace the graph's products use): every replay equals the eager
+    batched run bit for bit."""
+    print(f"[case] batch graph replays vs eager, precision_T={prec_T}", flush=True)
+    ts, _ = batch_tiles
+    gpu_ctx.batch_set(ts)
+    gpu_ctx.batch_solve(precision_T=prec_T, use_graph=False)
+    eager = [gpu_ctx.batch_result(i, want_T=True) for i in range(len(ts))]
+    others = [synth.tiny(seed=5, nexp=2, npix_in=9, nout=9), synth.small()]
+    for r in range(3):
+        gpu_ctx.batch_solve(precision_T=prec_T, use_graph=True)
+        for i in range(len(ts)):
+            out = gpu_ctx.batch_result(i, want_T=True)
+            for k in ("K", "U", "S", "H", "evals", "T"):
+                assert np.array_equal(out[k], eager[i][k]), (r, i, k)
+        # eager single-tile work on the same context between replays; the
+        # larger tile's BF16x6 T reallocates the split workspace
+        t = others[r % 2]
+        gpu_ctx.solve(t, precision_T=prec_T)
