@@ -38,6 +38,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace imcom {
@@ -68,7 +69,9 @@ __global__ __launch_bounds__(RB) void sy_colnorm(const SyTile* __restrict__ tile
 constexpr int SB = 64;  // symmetric matrix-vector block
 constexpr int SEG = 4;  // blocks of one block-column per workgroup
 constexpr int DCH = 2048;  // rows per dot-product chunk (kDotChunks * DCH >= 32768)
-constexpr int64_t kSymvNtRows = 8704;  // trailing size from which sy_symv streams (4 s^2 B > 300 MB)
+// bytes of the trailing triangle's tail that sy_symv keeps in the 256 MB
+// last-level cache from one column to the next (the rest it streams)
+constexpr int64_t kSymvKeepBytes = (int64_t)224 << 20;
 
 // The lower-triangle 64x64 blocks of A into the block layout At
 // (tri_block_offset), before the first panel.  Later panels' blocks are
@@ -117,16 +120,21 @@ __global__ __launch_bounds__(256) void sy_tile_init(const SyTile* __restrict__ t
 // sigma = [V W]^T x over rows [c+1, n) in chunks of DCH rows, one chunk per
 // wavefront (partials in dpart[q][chunk], summed by sy_mid): a whole column
 // per wavefront would leave one long serial wavefront as the launch's tail.
-// NT: the blocks are read with nontemporal loads.  Measured at n = 16384:
-// +7% while the trailing triangle exceeds the 256 MB last-level cache, -8%
-// once it fits there (the next column re-reads it from that cache).
+// NT: block-columns J < jkeep are read with nontemporal loads, the tail
+// J >= jkeep (a lower triangle of at most kSymvKeepBytes) with ordinary ones,
+// so the tail stays in the 256 MB last-level cache for the next column while
+// the rest streams past it.  Each workgroup takes one of the two compiled
+// loops (a per-load select merges them into ordinary loads).  Measured at
+// n = 16384, symv total: all ordinary loads while the triangle exceeds the
+// cache 1165 ms, nontemporal above s = 8704 1086 ms, cached tail of 96 / 160
+// / 224 / 256 MB 1048 / 1036 / 1027 / 1032 ms.
 // Two columns per step (eight block loads in flight per lane) was measured
 // against 4, 8 and 16 at n = 16384 (bitwise the same results): 4 and 8 are
 // no faster, and 16 (the whole strip in one step, 183 VGPRs, two waves per
 // SIMD) is 10-20% slower at every trailing size above 2000.
 template <bool NT>
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
-                                               int jj) {
+                                               int jj, int jkeep) {
   __shared__ double xJ[SB];
   __shared__ double csum[SB];
   __shared__ double rred[4][SEG * SB];
@@ -180,6 +188,10 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
     __syncthreads();  // xJ
     double qacc = 0.0;
     const bool diag = (I0 == J);
+    // the block loop, compiled once per load kind; the workgroup's
+    // block-column picks one (uniform branch)
+    auto blocks = [&](auto ntc) {
+    constexpr bool NTL = decltype(ntc)::value;
 #pragma unroll 1
     for (int q0 = 0; q0 < 16; q0 += 2) {
       double a[2][SEG];
@@ -197,7 +209,7 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
           const __attribute__((address_space(1))) double* src =
               (const __attribute__((address_space(1))) double*)t.At +
               tri_block_offset(I0 + u, J, t.nbt) + cl * SB + lane;
-          a[p][u] = ok ? (NT ? __builtin_nontemporal_load(src) : *src) : 0.0;
+          a[p][u] = ok ? (NTL ? __builtin_nontemporal_load(src) : *src) : 0.0;
         }
       }
 #pragma unroll
@@ -216,6 +228,11 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
         if (lane == 0) csum[cl] = cs;
       }
     }
+    };
+    if (NT && J < jkeep)
+      blocks(std::true_type{});
+    else
+      blocks(std::false_type{});
 #pragma unroll
     for (int u = 0; u < SEG; ++u) rred[w][u * SB + lane] = racc[u];
     __syncthreads();
@@ -593,6 +610,17 @@ __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ til
     t.Tb[o + idx % len + (int64_t)(idx / len) * nbb] = 0.0;
 }
 
+// First block-column of the trailing triangle's tail that sy_symv reads with
+// ordinary loads while it streams the rest: the last K block-columns, whose
+// lower triangle (K (K + 1) / 2 blocks of 32 KB) is at most keep bytes, stay
+// in the last-level cache from one column to the next.
+static int symv_keep_from(int64_t nmax) {
+  const int nbt = (int)((nmax + SB - 1) / SB);
+  int K = 0;
+  while (K < nbt && (int64_t)(K + 1) * (K + 2) / 2 * (SB * SB * 8) <= kSymvKeepBytes) ++K;
+  return nbt - K;
+}
+
 // The pb columns of the panel starting at column k: the first column's norm
 // partials, then three launches per column.
 static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
@@ -608,16 +636,16 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
     const int gy = nbl + ceil_div(ndot, gx);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (prof) IMCOM_HIP_TRY(prof->events(PS_SYMV, 2, &e0, &e1));
-    // nontemporal block reads while the lower triangle (4 s^2 bytes) is well
-    // beyond the 256 MB last-level cache
-    const int64_t s = nmax - c - 1;
+    // nontemporal reads of the block-columns left of the cached tail while
+    // the trailing triangle is larger than that tail
+    const int jkeep = symv_keep_from(nmax);
     const dim3 grid(gx, gy, ntiles);
-    if (s >= kSymvNtRows)
+    if (jkeep > (int)((c + 1) / SB))
       hipExtLaunchKernelGGL(sy_symv<true>, grid, dim3(256), 0, st, e0, e1, 0, d_tiles, (int)k,
-                            (int)c, jj);
+                            (int)c, jj, jkeep);
     else
       hipExtLaunchKernelGGL(sy_symv<false>, grid, dim3(256), 0, st, e0, e1, 0, d_tiles, (int)k,
-                            (int)c, jj);
+                            (int)c, jj, 0);
     hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
                        (int)c, jj, nnorm, gx * gy);
     const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);
