@@ -7,6 +7,7 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -27,6 +28,12 @@ __global__ void reverse_cols_kernel(int64_t n, const double* __restrict__ Z, dou
   if (blockIdx.x == 0 && threadIdx.x == 0) L[j] = lam[src];
 }
 
+// contexts open per device: they share its last-level cache
+static std::atomic<int> g_live[64];
+int live_contexts(int device) {
+  return (device >= 0 && device < 64) ? std::max(1, g_live[device].load()) : 1;
+}
+
 namespace imcom {
 hipError_t launch_reverse_cols(hipStream_t st, int64_t n, const double* Zin, double* Qout,
                                const double* lam_in, double* L_out) {
@@ -44,7 +51,7 @@ int imcom_gpu_init(imcom_gpu_ctx** out, int device) {
   int ndev = 0;
   hipError_t e = hipGetDeviceCount(&ndev);
   if (e != hipSuccess || ndev <= 0) return IMCOM_EHIP;
-  if (device < 0 || device >= ndev) return IMCOM_EARG;
+  if (device < 0 || device >= ndev || device >= 64) return IMCOM_EARG;
   e = hipSetDevice(device);
   if (e != hipSuccess) return IMCOM_EHIP;
   e = noalloc_scratch_setup(device);
@@ -58,6 +65,7 @@ int imcom_gpu_init(imcom_gpu_ctx** out, int device) {
   }
   ctx->dc = dc_create();
   ctx->prof.st = ctx->stream;
+  g_live[device].fetch_add(1);
   *out = ctx;
   return IMCOM_OK;
 }
@@ -88,6 +96,7 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
   if (ctx->fetch_ready) (void)hipEventDestroy(ctx->fetch_ready);
   if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  g_live[ctx->device].fetch_sub(1);
   delete ctx;
 }
 
@@ -261,7 +270,8 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
   // 1. A = H T H^T (reflectors left in A's lower part)
   {
     StageTimer tm(ctx, PS_SYTRD);
-    CK(sytrd_run(ctx->stream, dt, 1, n, host, ctx->nb, &ctx->prof), "tridiagonal reduction");
+    CK(sytrd_run(ctx->stream, dt, 1, n, host, ctx->nb, live_contexts(ctx->device), &ctx->prof),
+       "tridiagonal reduction");
   }
   // 2. T = Z diag(lam) Z^T, ascending
   {
@@ -308,7 +318,8 @@ int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out) {
   CK(hipMemcpyAsync(ctx->ew.stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
      "upload tile descriptor");
   std::vector<SyTile> host(1, t);
-  CK(sytrd_run(ctx->stream, ctx->ew.stile.as<SyTile>(), 1, n, host, ctx->nb, &ctx->prof),
+  CK(sytrd_run(ctx->stream, ctx->ew.stile.as<SyTile>(), 1, n, host, ctx->nb,
+               live_contexts(ctx->device), &ctx->prof),
      "tridiagonal reduction");
   ctx->have_A = false;
   CK(download(ctx, d_out, ctx->ew.sd, (size_t)n * 8), "download d");

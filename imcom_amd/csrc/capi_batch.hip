@@ -189,7 +189,7 @@ static hipError_t batch_seq1(imcom_gpu_ctx* ctx) {
                                   s.psfyscale, t->A.as<double>()));
   }
   return sytrd_run_batched(st, B->d_sy.as<SyTile>(), (int)B->tiles.size(), B->nmax, B->nb, B->plan,
-                           nullptr);
+                           live_contexts(ctx->device), nullptr);
 }
 
 // descending reorder, back-transform, lookup_B, P, bisection, T and maps

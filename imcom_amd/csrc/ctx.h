@@ -210,6 +210,10 @@ void imcom_batch_free(imcom_gpu_ctx* ctx);
 // point's scratch array (called by imcom_gpu_init, device current)
 hipError_t noalloc_scratch_setup(int device);
 
+// contexts open on the device (at least 1): the tridiagonal reduction gives
+// each an equal share of the last-level cache
+int live_contexts(int device);
+
 // stage timing helper: events recorded only when profiling is on
 struct StageTimer {
   imcom_gpu_ctx* c;

@@ -196,8 +196,11 @@ constexpr int kQParts = 70000;
 constexpr int kNormParts = 2048;
 constexpr int kDotChunks = 16;
 
+// `sharers`: contexts whose reductions may run at once on the device; each
+// keeps 1/sharers of the matrix-vector product's cached share of the
+// last-level cache.
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                     const std::vector<SyTile>& host, int nb, Prof* prof);
+                     const std::vector<SyTile>& host, int nb, int sharers, Prof* prof);
 hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nbb);
 
@@ -208,7 +211,7 @@ struct SyPlan {
 };
 void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb, SyPlan& plan);
 hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                             int nb, const SyPlan& plan, Prof* prof);
+                             int nb, const SyPlan& plan, int sharers, Prof* prof);
 hipError_t ormtr_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                              int nbb, const SyPlan& plan);
 

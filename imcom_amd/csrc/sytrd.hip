@@ -127,14 +127,15 @@ __global__ __launch_bounds__(256) void sy_tile_init(const SyTile* __restrict__ t
 // loops (a per-load select merges them into ordinary loads).  Measured at
 // n = 16384, symv total: all ordinary loads while the triangle exceeds the
 // cache 1165 ms, nontemporal above s = 8704 1086 ms, cached tail of 96 / 160
-// / 224 / 256 MB 1048 / 1036 / 1027 / 1032 ms.
+// / 224 / 256 MB 1048 / 1036 / 1027 / 1032 ms.  kkeep: block-columns of
+// each tile's cached tail.
 // Two columns per step (eight block loads in flight per lane) was measured
 // against 4, 8 and 16 at n = 16384 (bitwise the same results): 4 and 8 are
 // no faster, and 16 (the whole strip in one step, 183 VGPRs, two waves per
 // SIMD) is 10-20% slower at every trailing size above 2000.
 template <bool NT>
 __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles, int k, int c,
-                                               int jj, int jkeep) {
+                                               int jj, int kkeep) {
   __shared__ double xJ[SB];
   __shared__ double csum[SB];
   __shared__ double rred[4][SEG * SB];
@@ -146,6 +147,7 @@ __global__ __launch_bounds__(256) void sy_symv(const SyTile* __restrict__ tiles,
   const int64_t c1 = c + 1;
   const int j0 = (int)(c1 / SB);
   const int nbl = (int)((n + SB - 1) / SB) - j0;  // local block count
+  const int jkeep = t.nbt - kkeep;  // first block-column of the cached tail
   const int Jl = blockIdx.y;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int I0l = Jl + blockIdx.x * SEG;
@@ -610,21 +612,25 @@ __global__ __launch_bounds__(256) void bt_build_T(const SyTile* __restrict__ til
     t.Tb[o + idx % len + (int64_t)(idx / len) * nbb] = 0.0;
 }
 
-// First block-column of the trailing triangle's tail that sy_symv reads with
-// ordinary loads while it streams the rest: the last K block-columns, whose
-// lower triangle (K (K + 1) / 2 blocks of 32 KB) is at most keep bytes, stay
-// in the last-level cache from one column to the next.
-static int symv_keep_from(int64_t nmax) {
-  const int nbt = (int)((nmax + SB - 1) / SB);
+// Block-columns K at the end of each tile that sy_symv reads with ordinary
+// loads while it streams the rest: the tiles' tails (K (K + 1) / 2 blocks of
+// 32 KB each) stay in the last-level cache from one column to the next.
+// Their total is this context's share of kSymvKeepBytes: reductions of
+// `sharers` contexts may run at once.  Measured on 64 tiles (n from 1024 to
+// 5766) as three concurrent batched solves: all loads ordinary 852 ms per
+// solve, a tail of 0 / 40 / 75 / 110 / 150 / 224 MB per solve 825 / 803 /
+// 793 / 802 / 819 / 834 ms.
+static int symv_keep_blocks(int ntiles, int sharers) {
+  const int64_t keep = kSymvKeepBytes / std::max(1, sharers);
   int K = 0;
-  while (K < nbt && (int64_t)(K + 1) * (K + 2) / 2 * (SB * SB * 8) <= kSymvKeepBytes) ++K;
-  return nbt - K;
+  while ((int64_t)ntiles * (K + 1) * (K + 2) / 2 * (SB * SB * 8) <= keep) ++K;
+  return K;
 }
 
 // The pb columns of the panel starting at column k: the first column's norm
 // partials, then three launches per column.
 static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                                   int64_t k, int pb, Prof* prof) {
+                                   int64_t k, int pb, int kkeep, Prof* prof) {
   int nnorm = std::max(ceil_div(nmax - k - 2, RB), 1);
   hipLaunchKernelGGL(sy_colnorm, dim3(nnorm, 1, ntiles), dim3(RB), 0, st, d_tiles, (int)k);
   for (int jj = 0; jj < pb; ++jj) {
@@ -638,11 +644,10 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
     if (prof) IMCOM_HIP_TRY(prof->events(PS_SYMV, 2, &e0, &e1));
     // nontemporal reads of the block-columns left of the cached tail while
     // the trailing triangle is larger than that tail
-    const int jkeep = symv_keep_from(nmax);
     const dim3 grid(gx, gy, ntiles);
-    if (jkeep > (int)((c + 1) / SB))
+    if (nbl > kkeep)
       hipExtLaunchKernelGGL(sy_symv<true>, grid, dim3(256), 0, st, e0, e1, 0, d_tiles, (int)k,
-                            (int)c, jj, jkeep);
+                            (int)c, jj, kkeep);
     else
       hipExtLaunchKernelGGL(sy_symv<false>, grid, dim3(256), 0, st, e0, e1, 0, d_tiles, (int)k,
                             (int)c, jj, 0);
@@ -661,13 +666,14 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
 }
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                     const std::vector<SyTile>& host, int nb, Prof* prof) {
+                     const std::vector<SyTile>& host, int nb, int sharers, Prof* prof) {
   size_t tok = 0;
+  const int kkeep = symv_keep_blocks(ntiles, sharers);
   const int nbt = ceil_div(nmax, SB);
   hipLaunchKernelGGL(sy_tile_init, dim3(nbt, nbt, ntiles), dim3(256), 0, st, d_tiles);
   for (int64_t k = 0; k < nmax - 1; k += nb) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
-    IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, prof));
+    IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, kkeep, prof));
     // trailing update A22 -= [V W] [W V]^T over rows/cols [k+nb, n)
     const int64_t r0 = k + nb;
     if (r0 < nmax) {
@@ -797,14 +803,15 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
 }
 
 hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                             int nb, const SyPlan& plan, Prof* prof) {
+                             int nb, const SyPlan& plan, int sharers, Prof* prof) {
   size_t tok = 0;
+  const int kkeep = symv_keep_blocks(ntiles, sharers);
   const int nbt = ceil_div(nmax, SB);
   hipLaunchKernelGGL(sy_tile_init, dim3(nbt, nbt, ntiles), dim3(256), 0, st, d_tiles);
   int panel = 0;
   for (int64_t k = 0; k < nmax - 1; k += nb, ++panel) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
-    IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, prof));
+    IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, kkeep, prof));
     const int64_t r0 = k + nb;
     if (r0 < nmax) {
       hipLaunchKernelGGL(sy_pack, dim3(ceil_div(nmax - r0, 256), nb, ntiles), dim3(256), 0, st,
