@@ -326,7 +326,9 @@ void or_bisect_kappa(int UnotS, int64_t n, int64_t m, const double *L, const dou
 /* imcom_matrix.f90:716-750: T(:,a) = Q (P(:,a) / (L + K_a(a))), T = -T/2. */
 void or_build_T(int64_t n, int64_t m, const double *Q, const double *P, const double *L,
                 const double *K_a, double *T) {
-#pragma omp parallel for schedule(dynamic, 4)
+  /* one pixel per chunk: a sample of as many pixels as threads keeps every
+     thread busy, as the whole tile does */
+#pragma omp parallel for schedule(dynamic, 1)
   for (int64_t a = 0; a < m; ++a) {
     double *t = T + a * n;
     const double *p = P + a * n;

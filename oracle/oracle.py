@@ -223,7 +223,7 @@ def build_S(T, Ndiag):
 
 
 def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, cols_P=128,
-                     pix_bisect=48, pix_T=8, checks=False):
+                     pix_bisect=48, pix_T=16, checks=False, full_eig=False):
     """Time the oracle's solve path on bounded samples of one tile and scale
     each stage to the whole tile (the CPU-baseline leg of bench.py):
 
@@ -234,6 +234,11 @@ def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, col
       bisect     pix_bisect pixels at full n, scaled by m
       build_T    pix_T DGEMVs at full n, scaled by m
 
+    With full_eig=True, lookup_A and DSYEVR run on the whole tile instead
+    (timed, not scaled): DSYEVR's time on a leading or strided block of A does
+    not follow n^3 to the whole matrix (measured at n = 16384 on 16 threads:
+    blocks of 4096 and 8192 take 4.2 and 21.6 s, the whole matrix 68 s),
+    because the block's spectrum is not the whole matrix's.
     L and P (the tile's eigenvalues and projections, any source) feed the
     bisection sample.  With checks=True the scale factors are tested by a
     second sample of twice the size: DSYEVR at 2 n_eig (the fitted exponent of
@@ -248,38 +253,43 @@ def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, col
     tab = _f(tile.Alookup)
     ex = np.ascontiguousarray(tile.exp_i, dtype=np.int32)
     x, y = np.ascontiguousarray(tile.x_i), np.ascontiguousarray(tile.y_i)
-    rows_A = min(rows_A, n)
-    A = np.zeros((n, n), order="F")
-    t0 = time.perf_counter()
-    lib().or_lookup_A_rows(ctypes.c_int64(n), ctypes.c_int64(0), ctypes.c_int64(rows_A), _p(x),
-                           _p(y), ex.ctypes.data_as(_ip), ctypes.c_int(tile.psfconst),
-                           ctypes.c_int(tab.shape[0]), ctypes.c_int(tab.shape[1]), _p(tab),
-                           ctypes.c_int(tile.npad), ctypes.c_int(tile.npoly),
-                           ctypes.c_double(tile.psfxscale), ctypes.c_double(tile.psfyscale), _p(A))
-    dt = time.perf_counter() - t0
-    done = rows_A * n - rows_A * (rows_A - 1) / 2
-    stages["lookup_A"] = dt * (n * (n + 1) / 2) / done
     chk = {}
-    if checks and 2 * rows_A <= n:
-        r2 = 2 * rows_A
+    if full_eig:
         t0 = time.perf_counter()
-        lib().or_lookup_A_rows(ctypes.c_int64(n), ctypes.c_int64(0), ctypes.c_int64(r2), _p(x),
+        Ae = lookup_A(tile)
+        stages["lookup_A"] = time.perf_counter() - t0
+    else:
+        rows_A = min(rows_A, n)
+        A = np.zeros((n, n), order="F")
+        t0 = time.perf_counter()
+        lib().or_lookup_A_rows(ctypes.c_int64(n), ctypes.c_int64(0), ctypes.c_int64(rows_A), _p(x),
                                _p(y), ex.ctypes.data_as(_ip), ctypes.c_int(tile.psfconst),
                                ctypes.c_int(tab.shape[0]), ctypes.c_int(tab.shape[1]), _p(tab),
                                ctypes.c_int(tile.npad), ctypes.c_int(tile.npoly),
-                               ctypes.c_double(tile.psfxscale), ctypes.c_double(tile.psfyscale),
-                               _p(A))
-        dt2 = time.perf_counter() - t0
-        done2 = r2 * n - r2 * (r2 - 1) / 2
-        chk["lookup_A_time_ratio_2x_entries"] = round((dt2 / dt) * (done / done2) * 2, 3)
-        stages["lookup_A"] = dt2 * (n * (n + 1) / 2) / done2
-    # leading block of A for the eigen sample (computed, not timed)
-    ne = min(n_eig, n)
-    sub = tile.__class__(**{**tile.__dict__})
-    sub.n = ne
-    sub.x_i, sub.y_i, sub.exp_i = tile.x_i[:ne], tile.y_i[:ne], tile.exp_i[:ne]
-    Ae = lookup_A(sub)
-    del A
+                               ctypes.c_double(tile.psfxscale), ctypes.c_double(tile.psfyscale), _p(A))
+        dt = time.perf_counter() - t0
+        done = rows_A * n - rows_A * (rows_A - 1) / 2
+        stages["lookup_A"] = dt * (n * (n + 1) / 2) / done
+        if checks and 2 * rows_A <= n:
+            r2 = 2 * rows_A
+            t0 = time.perf_counter()
+            lib().or_lookup_A_rows(ctypes.c_int64(n), ctypes.c_int64(0), ctypes.c_int64(r2), _p(x),
+                                   _p(y), ex.ctypes.data_as(_ip), ctypes.c_int(tile.psfconst),
+                                   ctypes.c_int(tab.shape[0]), ctypes.c_int(tab.shape[1]), _p(tab),
+                                   ctypes.c_int(tile.npad), ctypes.c_int(tile.npoly),
+                                   ctypes.c_double(tile.psfxscale), ctypes.c_double(tile.psfyscale),
+                                   _p(A))
+            dt2 = time.perf_counter() - t0
+            done2 = r2 * n - r2 * (r2 - 1) / 2
+            chk["lookup_A_time_ratio_2x_entries"] = round((dt2 / dt) * (done / done2) * 2, 3)
+            stages["lookup_A"] = dt2 * (n * (n + 1) / 2) / done2
+        # leading block of A for the eigen sample (computed, not timed)
+        ne = min(n_eig, n)
+        sub = tile.__class__(**{**tile.__dict__})
+        sub.n = ne
+        sub.x_i, sub.y_i, sub.exp_i = tile.x_i[:ne], tile.y_i[:ne], tile.exp_i[:ne]
+        Ae = lookup_A(sub)
+        del A
     rb = min(rows_B, n)
     subB = tile.__class__(**{**tile.__dict__})
     subB.n = rb
@@ -287,22 +297,28 @@ def sampled_cpu_time(tile, L=None, P=None, rows_A=96, rows_B=96, n_eig=2048, col
     t0 = time.perf_counter()
     lookup_B(subB)
     stages["lookup_B"] = (time.perf_counter() - t0) * n / rb
-    t0 = time.perf_counter()
-    eigen_A(Ae)
-    te = time.perf_counter() - t0
-    stages["eigen_A"] = te * (n / ne) ** 3
-    if checks and 2 * ne <= n:
-        sub2 = tile.__class__(**{**tile.__dict__})
-        sub2.n = 2 * ne
-        sub2.x_i, sub2.y_i, sub2.exp_i = tile.x_i[:2 * ne], tile.y_i[:2 * ne], tile.exp_i[:2 * ne]
-        Ae2 = lookup_A(sub2)
+    if full_eig:
         t0 = time.perf_counter()
-        eigen_A(Ae2)
-        te2 = time.perf_counter() - t0
-        del Ae2
-        chk["eigen_A_fitted_exponent"] = round(float(np.log2(te2 / te)), 3)
-        chk["eigen_A_sizes"] = [ne, 2 * ne]
-        stages["eigen_A"] = te2 * (n / (2 * ne)) ** 3
+        eigen_A(Ae)
+        stages["eigen_A"] = time.perf_counter() - t0
+        del Ae
+    else:
+        t0 = time.perf_counter()
+        eigen_A(Ae)
+        te = time.perf_counter() - t0
+        stages["eigen_A"] = te * (n / ne) ** 3
+        if checks and 2 * ne <= n:
+            sub2 = tile.__class__(**{**tile.__dict__})
+            sub2.n = 2 * ne
+            sub2.x_i, sub2.y_i, sub2.exp_i = tile.x_i[:2 * ne], tile.y_i[:2 * ne], tile.exp_i[:2 * ne]
+            Ae2 = lookup_A(sub2)
+            t0 = time.perf_counter()
+            eigen_A(Ae2)
+            te2 = time.perf_counter() - t0
+            del Ae2
+            chk["eigen_A_fitted_exponent"] = round(float(np.log2(te2 / te)), 3)
+            chk["eigen_A_sizes"] = [ne, 2 * ne]
+            stages["eigen_A"] = te2 * (n / (2 * ne)) ** 3
     rng = np.random.default_rng(0)
     Qr = np.asfortranarray(rng.standard_normal((n, n)))
     cp = min(cols_P, m)
