@@ -69,7 +69,7 @@ struct EigWork {
     IMCOM_HIP_TRY(sd.reserve(n * d));
     IMCOM_HIP_TRY(se.reserve(n * d));
     IMCOM_HIP_TRY(stau.reserve(n * d));
-    IMCOM_HIP_TRY(spart.reserve(kNormParts * d));
+    IMCOM_HIP_TRY(spart.reserve(2 * kNormParts * d));  // two buffers by column parity
     IMCOM_HIP_TRY(spart2.reserve(kQParts * d));
     IMCOM_HIP_TRY(ss.reserve(2 * nb * d));
     IMCOM_HIP_TRY(sdp.reserve((size_t)2 * nb * kDotChunks * d));

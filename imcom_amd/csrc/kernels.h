@@ -176,7 +176,7 @@ struct SyTile {
   double* d;      // n     tridiagonal diagonal
   double* e;      // n-1   off-diagonal
   double* tau;    // n-1
-  double* part;   // kNormParts   per-workgroup partials of a column's squared norm
+  double* part;   // 2 x kNormParts  per-workgroup partials of a column's squared norm
   double* part2;  // kQParts
   double* s;      // 2 NB dot products
   double* dpart;  // 2 NB x kDotChunks  chunk partials of the dot products

@@ -14,7 +14,8 @@
 // product reads the lower triangle in 64x64 blocks, each block serving its
 // rows and its columns.
 //
-// Three launches per column:
+// Three launches per column (two for one tile, where sy_midwupd does the
+// last two):
 //   sy_symv      partial sums of u = A22 x for the unscaled column x (so it
 //                does not wait for the reflector) and of x^T u over the lower
 //                triangle, the 2jj dot products [V W]^T x
@@ -45,6 +46,12 @@ namespace imcom {
 
 constexpr int RB = 256;  // rows per workgroup in the row-parallel kernels
 
+// Norm partials of column c: two buffers by the column's parity, so that one
+// launch can read column c's partials while it writes column c+1's.
+__device__ __forceinline__ double* norm_parts(const SyTile& t, int64_t c) {
+  return t.part + (c & 1) * kNormParts;
+}
+
 __device__ __forceinline__ double dlapy2(double x, double y) {
   const double xa = fabs(x), ya = fabs(y);
   const double w = fmax(xa, ya), z = fmin(xa, ya);
@@ -63,7 +70,7 @@ __global__ __launch_bounds__(RB) void sy_colnorm(const SyTile* __restrict__ tile
   const int64_t i = c + 2 + (int64_t)blockIdx.x * RB + threadIdx.x;
   const double a = (i < n) ? t.A[i + (int64_t)c * n] : 0.0;
   const double tot = block_sum(a * a, red);
-  if (threadIdx.x == 0) t.part[blockIdx.x] = tot;
+  if (threadIdx.x == 0) norm_parts(t, c)[blockIdx.x] = tot;
 }
 
 constexpr int SB = 64;  // symmetric matrix-vector block
@@ -273,21 +280,30 @@ constexpr int MID_THREADS = 1024;
 // norm, x^T u, row c+1 of u) in one pass with one block reduction, the chunk
 // sums of the dot products, and V(c+1,:), W(c+1,:); one wavefront then forms
 // the jj-term sums.
-__global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__ tiles, int k,
-                                                      int c, int jj, int nnorm, int nq) {
+// The reflector step's results, in LDS: scal[0..4] as the tile's scal, ss as
+// its s.
+struct MidOut {
+  double scal[5];
+  double ss[2 * 64];
+};
+
+// sy_mid's work for one tile (c < n - 1), run by MID_THREADS threads; the
+// results go to `out` (read after a barrier) and, with to_global, to the
+// tile's s, scal, d, e and tau.  `red`: 3 x MID_THREADS/64 doubles of LDS.
+__device__ __forceinline__ void mid_body(const SyTile& t, int k, int c, int jj, int nnorm, int nq,
+                                         double (*red)[MID_THREADS / 64], MidOut& out,
+                                         bool to_global) {
   constexpr int NW = MID_THREADS / 64;
-  __shared__ double red[3][NW];
-  __shared__ double ss[2 * 64];
-  const SyTile t = tiles[blockIdx.z];
+  double* ss = out.ss;
   const int64_t n = t.n;
-  if (c >= n - 1) return;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int64_t c1 = c + 1;
   const int nbl = (int)((n + SB - 1) / SB) - (int)(c1 / SB);
   const int nseg0 = (nbl + SEG - 1) / SEG;
   // phase 1: loads
   double pn = 0.0, pu = 0.0;
-  for (int q = tid; q < nnorm; q += MID_THREADS) pn += t.part[q];
+  const double* __restrict__ part = norm_parts(t, c);
+  for (int q = tid; q < nnorm; q += MID_THREADS) pn += part[q];
   double qs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // eight loads in flight
   {
     int q = tid;
@@ -345,7 +361,7 @@ __global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__
   }
   if (tid < 2 * jj) ss[tid] = scal * (sig - beta * xc1);
   __syncthreads();
-  if (tid < 2 * jj) t.s[tid] = ss[tid];
+  if (to_global && tid < 2 * jj) t.s[tid] = ss[tid];
   if (w == 0) {  // jj <= 63: one lane per panel column
     const double sv = lane < jj ? ss[lane] * ss[jj + lane] : 0.0;
     const double cr = lane < jj ? vc * ss[jj + lane] + wc * ss[lane] : 0.0;
@@ -354,41 +370,57 @@ __global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__
       const double vAv = scal * scal * (xu - 2.0 * beta * u0 + beta * beta * g0);
       const double alpha = -0.5 * tau * tau * (vAv - 2.0 * svw);
       const double y0 = scal * (u0 - beta * g0);
-      t.scal[0] = scal;
-      t.scal[1] = tau;
-      t.scal[2] = alpha;
-      t.scal[3] = beta;
-      t.scal[4] = tau * (y0 - corr) + alpha;
-      t.d[c] = acc;
-      t.e[c] = beta;
-      t.tau[c] = tau;
+      const double sc[5] = {scal, tau, alpha, beta, tau * (y0 - corr) + alpha};
+#pragma unroll
+      for (int q = 0; q < 5; ++q) out.scal[q] = sc[q];
+      if (to_global) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) t.scal[q] = sc[q];
+        t.d[c] = acc;
+        t.e[c] = beta;
+        t.tau[c] = tau;
+      }
     }
   }
+}
+
+__global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__ tiles, int k,
+                                                      int c, int jj, int nnorm, int nq) {
+  __shared__ double red[3][MID_THREADS / 64];
+  __shared__ MidOut mo;
+  const SyTile t = tiles[blockIdx.z];
+  if (c >= t.n - 1) return;
+  mid_body(t, k, c, jj, nnorm, nq, red, mo, true);
 }
 
 constexpr int WR = 16;  // rows per sy_wupd workgroup
 constexpr int WL = 16;  // lanes per row (256 threads)
 
 // Finish column c and start column c+1 over rows [c+1, n), 16 rows per
-// workgroup and 16 lanes per row (lane l sums partials l, l+16, ...):
+// row block and 16 lanes per row (lane l sums partials l, l+16, ...):
 //   y = scal (u - beta A22(:,0)),  w = tau (y - V sW - W sV) + alpha v,
-// stored in W(:, jj); v into A(:, c); then, when column c+1 is in this panel
-// or is the tile's last column, its rank-2(jj+1) update and its norm
-// partials (for the next sy_mid).
-template <bool WIDE>
-__global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles, int k, int c,
-                                               int jj, int in_panel_next) {
-  __shared__ double red[3][4][WR];
-  __shared__ double nred[4];
-  const SyTile t = tiles[blockIdx.z];
+// stored in W(:, jj); v into A(:, c) below row c+1 (the unit at row c+1 is
+// implicit: A(c+1, c) keeps the column's entry before the reflection, which
+// sy_mid reads); then, when column c+1 is in this panel or is the tile's last
+// column, its rank-2(jj+1) update and its norm partials (for the next
+// column's reflector step).
+//
+// wupd_body: row block b (rows c+1+16b ..), 256 threads (tid), the reflector
+// step's results sc[0..4] and sv[0..2jj) (the tile's scal and s, or their
+// LDS copies: SV_LDS), LDS scratch red (3 x 4 x WR) and nred (4); row blocks
+// b >= nw write no norm partial.
+template <bool WIDE, bool SV_LDS>
+__device__ __forceinline__ void wupd_body(const SyTile& t, int k, int c, int jj, int in_panel_next,
+                                          int b, int tid, int nw, const double* sc,
+                                          const double* sv, double (*red)[4][WR], double* nred) {
   const int64_t n = t.n;
-  if (c >= n - 1) return;  // this tile is reduced (uniform over its workgroups)
   const bool upd_next = in_panel_next || (c + 1 == n - 1);
-  const int tid = threadIdx.x, rl = tid & (WR - 1), lane = tid >> 4, w = tid >> 6;
+  const int rl = tid & (WR - 1), lane = tid >> 4, w = tid >> 6;
   const int64_t c1 = c + 1;
   const int j0 = (int)(c1 / SB);
   const int nbl = (int)((n + SB - 1) / SB) - j0;
-  const int64_t i = c1 + (int64_t)blockIdx.x * WR + rl;
+  const int64_t i = c1 + (int64_t)b * WR + rl;
+  auto sval = [&](int q) { return SV_LDS ? sv[q] : ldg(sv + q); };
   // WIDE (one tile): every load is issued before the first sum (straight-line
   // code, one memory round trip): 16 ypart and 4 ycpart partials of u per
   // lane (all of them up to n = 16384; longer rows finish in the loops
@@ -397,8 +429,7 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
   // runs the 36-VGPR form with eight partials in flight per step instead
   // (measured: 8.4 -> 7.0 us per launch for one tile at n = 16384, but
   // 18 -> 22 us for the 64-tile batch).
-  const double scal = t.scal[0], tau = t.scal[1], alpha = t.scal[2], beta = t.scal[3],
-               w1 = t.scal[4];
+  const double scal = sc[0], tau = sc[1], alpha = sc[2], beta = sc[3], w1 = sc[4];
   double g = 0.0, xc = 0.0;  // A22(i, 0) as stored, and A(i, c): used after the reduction
   double qu = 0.0, qc = 0.0, q1 = 0.0;  // lane partials: u, V sW + W sV, update
   if (i < n) {
@@ -431,8 +462,8 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
         const bool ok = p < jj, okn = ok && upd_next;
         pv[q][0] = ok ? ldg(t.A + i + (int64_t)(k + p) * n) : 0.0;
         pv[q][1] = ok ? ldg(t.W + i + (int64_t)p * n) : 0.0;
-        pv[q][2] = ok ? ldg(t.s + jj + p) : 0.0;
-        pv[q][3] = ok ? ldg(t.s + p) : 0.0;
+        pv[q][2] = ok ? sval(jj + p) : 0.0;
+        pv[q][3] = ok ? sval(p) : 0.0;
         pv[q][4] = okn ? ldg(t.W + c1 + (int64_t)p * n) : 0.0;
         pv[q][5] = okn ? ldg(t.A + c1 + (int64_t)(k + p) * n) : 0.0;
       }
@@ -469,7 +500,7 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
         if (p < jj) {
           const double Vip = ldg(t.A + i + (int64_t)(k + p) * n);
           const double Wip = ldg(t.W + i + (int64_t)p * n);
-          qc += Vip * ldg(t.s + jj + p) + Wip * ldg(t.s + p);
+          qc += Vip * sval(jj + p) + Wip * sval(p);
           if (upd_next)
             q1 += Vip * ldg(t.W + c1 + (int64_t)p * n) + Wip * ldg(t.A + c1 + (int64_t)(k + p) * n);
         }
@@ -506,12 +537,56 @@ __global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles,
     }
     // each row's column-c, column-(c+1) and W(:, jj) entries belong to this thread
     t.W[i + (int64_t)jj * n] = wv;
-    t.A[i + (int64_t)c * n] = v;
+    if (i > c1) t.A[i + (int64_t)c * n] = v;
   }
-  if (upd_next && c1 < n - 1) {
-    const double tot = block_sum(sq, nred);
-    if (tid == 0) t.part[blockIdx.x] = tot;
+  if (upd_next && c1 < n - 1) {  // the 4 wavefronts' sums in order, as block_sum
+    const double ws = wave_sum(sq);
+    __syncthreads();
+    if ((tid & 63) == 0) nred[w] = ws;
+    __syncthreads();
+    double tot = 0.0;
+    for (int q = 0; q < 4; ++q) tot += nred[q];
+    if (tid == 0 && b < nw) norm_parts(t, c1)[b] = tot;
   }
+}
+
+template <bool WIDE>
+__global__ __launch_bounds__(256) void sy_wupd(const SyTile* __restrict__ tiles, int k, int c,
+                                               int jj, int in_panel_next) {
+  __shared__ double red[3][4][WR];
+  __shared__ double nred[4];
+  const SyTile t = tiles[blockIdx.z];
+  if (c >= t.n - 1) return;  // this tile is reduced (uniform over its workgroups)
+  wupd_body<WIDE, false>(t, k, c, jj, in_panel_next, blockIdx.x, threadIdx.x, gridDim.x, t.scal,
+                         t.s, red, nred);
+}
+
+// sy_mid and sy_wupd in one launch, for columns whose partial sums are few:
+// every workgroup forms the reflector itself (the same code and summation
+// order as sy_mid, so the results are bitwise those of the two launches),
+// then its four groups of 256 threads run sy_wupd's row blocks 4b .. 4b+3.
+// Workgroup 0 also stores the reflector step's results.  nw: sy_wupd's row
+// block count.  One tile only: measured at n = 16384 the reduction went 1401
+// -> 1380 ms (the repeated reflector step reads ~17000 partials per
+// workgroup at the first columns and still gains), at n = 1024 the step 24.9
+// -> 24.2 ms; in the 64-tile batch, where every workgroup of every tile
+// repeats it, fusing above trailing size 2000 cost 789 -> 888-903 ms per
+// solve, and below it gained nothing.
+template <bool WIDE>
+__global__ __launch_bounds__(MID_THREADS) void sy_midwupd(const SyTile* __restrict__ tiles, int k,
+                                                          int c, int jj, int nnorm, int nq,
+                                                          int in_panel_next, int nw) {
+  __shared__ double red[3][MID_THREADS / 64];
+  __shared__ MidOut mo;
+  __shared__ double wred[MID_THREADS / 256][3][4][WR];
+  __shared__ double nred[MID_THREADS / 256][4];
+  const SyTile t = tiles[blockIdx.z];
+  if (c >= t.n - 1) return;
+  mid_body(t, k, c, jj, nnorm, nq, red, mo, blockIdx.x == 0);
+  __syncthreads();
+  const int g = threadIdx.x >> 8;
+  wupd_body<WIDE, true>(t, k, c, jj, in_panel_next, blockIdx.x * (MID_THREADS / 256) + g,
+                        threadIdx.x & 255, nw, mo.scal, mo.ss, wred[g], nred[g]);
 }
 
 // Copy the panel's V (with explicit zeros above the unit) and W into the
@@ -522,7 +597,7 @@ __global__ void sy_pack(const SyTile* __restrict__ tiles, int k, int nb, int64_t
   const int64_t i = r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int p = blockIdx.y;
   if (i >= n || k + p >= n - 1) return;
-  const double v = (i > k + p) ? t.A[i + (int64_t)(k + p) * n] : 0.0;
+  const double v = (i == k + p + 1) ? 1.0 : (i > k + p + 1 ? t.A[i + (int64_t)(k + p) * n] : 0.0);
   const double w = t.W[i + (int64_t)p * n];
   t.VW[i + (int64_t)p * n] = v;
   t.VW[i + (int64_t)(nb + p) * n] = w;
@@ -651,15 +726,17 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
     else
       hipExtLaunchKernelGGL(sy_symv<false>, grid, dim3(256), 0, st, e0, e1, 0, d_tiles, (int)k,
                             (int)c, jj, 0);
-    hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
-                       (int)c, jj, nnorm, gx * gy);
     const int nw = std::max(ceil_div(nmax - c - 1, WR), 1);
-    if (ntiles == 1)
-      hipLaunchKernelGGL(sy_wupd<true>, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
-                         (int)c, jj, (jj + 1 < pb) ? 1 : 0);
-    else
+    const int nxt = (jj + 1 < pb) ? 1 : 0;
+    if (ntiles == 1) {  // one launch: sy_mid's work repeated by every workgroup
+      hipLaunchKernelGGL(sy_midwupd<true>, dim3(ceil_div(nw, MID_THREADS / 256)), dim3(MID_THREADS),
+                         0, st, d_tiles, (int)k, (int)c, jj, nnorm, gx * gy, nxt, nw);
+    } else {
+      hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
+                         (int)c, jj, nnorm, gx * gy);
       hipLaunchKernelGGL(sy_wupd<false>, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
-                         (int)c, jj, (jj + 1 < pb) ? 1 : 0);
+                         (int)c, jj, nxt);
+    }
     nnorm = nw;  // sy_wupd leaves the next column's norm partials, one per workgroup
   }
   return hipGetLastError();
