@@ -287,13 +287,13 @@ struct MidOut {
   double ss[2 * 64];
 };
 
-// sy_mid's work for one tile (c < n - 1), run by MID_THREADS threads; the
-// results go to `out` (read after a barrier) and, with to_global, to the
-// tile's s, scal, d, e and tau.  `red`: 3 x MID_THREADS/64 doubles of LDS.
+// sy_mid's work for one tile (c < n - 1), run by NTH threads; the results
+// go to `out` (read after a barrier) and, with to_global, to the tile's s,
+// scal, d, e and tau.  `red`: 3 x NTH/64 doubles of LDS.
+template <int NTH>
 __device__ __forceinline__ void mid_body(const SyTile& t, int k, int c, int jj, int nnorm, int nq,
-                                         double (*red)[MID_THREADS / 64], MidOut& out,
-                                         bool to_global) {
-  constexpr int NW = MID_THREADS / 64;
+                                         double (*red)[NTH / 64], MidOut& out, bool to_global) {
+  constexpr int NW = NTH / 64;
   double* ss = out.ss;
   const int64_t n = t.n;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -303,18 +303,18 @@ __device__ __forceinline__ void mid_body(const SyTile& t, int k, int c, int jj, 
   // phase 1: loads
   double pn = 0.0, pu = 0.0;
   const double* __restrict__ part = norm_parts(t, c);
-  for (int q = tid; q < nnorm; q += MID_THREADS) pn += part[q];
+  for (int q = tid; q < nnorm; q += NTH) pn += part[q];
   double qs[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // eight loads in flight
   {
     int q = tid;
-    for (; q + 7 * MID_THREADS < nq; q += 8 * MID_THREADS) {
+    for (; q + 7 * NTH < nq; q += 8 * NTH) {
 #pragma unroll
-      for (int r = 0; r < 8; ++r) qs[r] += t.qpart[q + r * MID_THREADS];
+      for (int r = 0; r < 8; ++r) qs[r] += t.qpart[q + r * NTH];
     }
-    for (; q < nq; q += MID_THREADS) qs[0] += t.qpart[q];
+    for (; q < nq; q += NTH) qs[0] += t.qpart[q];
   }
   const double q0 = qs[0] + qs[4], q1 = qs[1] + qs[5], q2 = qs[2] + qs[6], q3 = qs[3] + qs[7];
-  for (int sg = tid; sg < nseg0; sg += MID_THREADS) pu += t.ycpart[(int64_t)sg * n + c1];
+  for (int sg = tid; sg < nseg0; sg += NTH) pu += t.ycpart[(int64_t)sg * n + c1];
   double sig = 0.0, xc1 = 0.0;
   if (tid < 2 * jj) {
     const int nch = (int)((n - c1 + DCH - 1) / DCH);
@@ -384,13 +384,20 @@ __device__ __forceinline__ void mid_body(const SyTile& t, int k, int c, int jj, 
   }
 }
 
-__global__ __launch_bounds__(MID_THREADS) void sy_mid(const SyTile* __restrict__ tiles, int k,
-                                                      int c, int jj, int nnorm, int nq) {
-  __shared__ double red[3][MID_THREADS / 64];
+// The batched launches' reflector step: 256 threads, so that each tile's
+// workgroup finds room beside the other streams' matrix-vector workgroups
+// (64-tile mosaic as three concurrent batched solves: 1024 threads 785 ms per
+// solve, 256 760 ms, 128 762 ms).
+constexpr int MID_THREADS_BATCH = 256;
+
+__global__ __launch_bounds__(MID_THREADS_BATCH) void sy_mid(const SyTile* __restrict__ tiles,
+                                                            int k, int c, int jj, int nnorm,
+                                                            int nq) {
+  __shared__ double red[3][MID_THREADS_BATCH / 64];
   __shared__ MidOut mo;
   const SyTile t = tiles[blockIdx.z];
   if (c >= t.n - 1) return;
-  mid_body(t, k, c, jj, nnorm, nq, red, mo, true);
+  mid_body<MID_THREADS_BATCH>(t, k, c, jj, nnorm, nq, red, mo, true);
 }
 
 constexpr int WR = 16;  // rows per sy_wupd workgroup
@@ -582,7 +589,7 @@ __global__ __launch_bounds__(MID_THREADS) void sy_midwupd(const SyTile* __restri
   __shared__ double nred[MID_THREADS / 256][4];
   const SyTile t = tiles[blockIdx.z];
   if (c >= t.n - 1) return;
-  mid_body(t, k, c, jj, nnorm, nq, red, mo, blockIdx.x == 0);
+  mid_body<MID_THREADS>(t, k, c, jj, nnorm, nq, red, mo, blockIdx.x == 0);
   __syncthreads();
   const int g = threadIdx.x >> 8;
   wupd_body<WIDE, true>(t, k, c, jj, in_panel_next, blockIdx.x * (MID_THREADS / 256) + g,
@@ -732,8 +739,8 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
       hipLaunchKernelGGL(sy_midwupd<true>, dim3(ceil_div(nw, MID_THREADS / 256)), dim3(MID_THREADS),
                          0, st, d_tiles, (int)k, (int)c, jj, nnorm, gx * gy, nxt, nw);
     } else {
-      hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS), 0, st, d_tiles, (int)k,
-                         (int)c, jj, nnorm, gx * gy);
+      hipLaunchKernelGGL(sy_mid, dim3(1, 1, ntiles), dim3(MID_THREADS_BATCH), 0, st, d_tiles,
+                         (int)k, (int)c, jj, nnorm, gx * gy);
       hipLaunchKernelGGL(sy_wupd<false>, dim3(nw, 1, ntiles), dim3(256), 0, st, d_tiles, (int)k,
                          (int)c, jj, nxt);
     }
