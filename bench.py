@@ -270,6 +270,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
+    ap.add_argument("--reduction-only", action="store_true",
+                    help="single-tile workloads: build A and run its tridiagonal reduction once "
+                         "(no timing events), print its device time and exit; the run that "
+                         "sy_symv's FETCH_SIZE / WRITE_SIZE counters are collected on")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -309,6 +313,16 @@ def main():
         autotune.apply(g, max(t.n for t in tiles))
         if weights is not None:
             g.mlp_set_params(weights)
+        if args.reduction_only:
+            g.set_tile(tile)
+            g.lookup_A(tile.npoly, tile.npad)
+            torch.cuda.synchronize()
+            tr = time.perf_counter()
+            d, e = g.tridiagonalize()
+            print(json.dumps({"reduction": args.workload, "n": tile.n, "launches": tile.n - 1,
+                              "seconds": round(time.perf_counter() - tr, 4)}), flush=True)
+            g.close()
+            return
 
     if args.workload == "batch":
         # fast: the size-sorted tiles cut into three runs of about equal n^3
