@@ -216,7 +216,7 @@ __global__ void dc_sort_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm
   cs[m.off + rank] = q;
 }
 
-// ---- step 2: deflation scan (dlaed2 rules), one thread per merge -----------
+// ---- step 2: deflation scan (dlaed2 rules), one wavefront per merge -------
 __device__ __forceinline__ double dlapy2_d(double x, double y) {
   const double xa = fabs(x), ya = fabs(y);
   const double w = fmax(xa, ya), z = fmin(xa, ya);
@@ -225,7 +225,33 @@ __device__ __forceinline__ double dlapy2_d(double x, double y) {
   return w * sqrt(1.0 + q * q);
 }
 
-__global__ void dc_deflate_kernel(const DcMerge* __restrict__ mg, int nm,
+// dlaed2's rotation test for the candidate (nd, nz) against the previous
+// candidate (pd, pz): c, s of the rotation that zeroes one z entry, and
+// whether the rotated matrix's off-diagonal entry t c s is negligible.
+// Unfused, so that both code paths below take the same decision.
+__device__ __forceinline__ bool dc_rot_test(double pd, double pz, double nd, double nz, double tol,
+                                            double& c, double& s, double& tau) {
+#pragma clang fp contract(off)
+  s = pz;
+  c = nz;
+  tau = dlapy2_d(c, s);
+  const double t = nd - pd;
+  c = c / tau;
+  s = -s / tau;
+  return fabs(t * c * s) <= tol;
+}
+
+// One wavefront per merge, 64 entries at a time: each lane takes one entry of
+// the sorted (d, z); entries with negligible z deflate, every other entry
+// (a candidate) is tested against the previous candidate.  A chunk with no
+// deflating rotation, whose deflated values keep ascending, is resolved at
+// once (ballots and prefix counts give each output its place); a chunk with a
+// rotation is walked by lane 0 with the scalar rules, since a rotation
+// changes the previous candidate's d and z.  The outputs are those of the
+// scalar scan over the whole merge.  Scan state: the pending candidate
+// (pd, pz, pcol) once one exists, k non-deflated so far, ndf deflated, nrot
+// rotations, lastv the largest deflated value appended.
+__global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restrict__ mg, int nm,
                                   const double* __restrict__ ds, const double* __restrict__ zs,
                                   const int* __restrict__ cs, double* __restrict__ dl,
                                   double* __restrict__ wz, int* __restrict__ ndcol,
@@ -233,93 +259,190 @@ __global__ void dc_deflate_kernel(const DcMerge* __restrict__ mg, int nm,
                                   int* __restrict__ rp, int* __restrict__ rn,
                                   double* __restrict__ rc, double* __restrict__ rs,
                                   DcCount* __restrict__ counts) {
-  const int mi = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double cd[64], cz[64];
+  __shared__ int ccol[64];
+  __shared__ double st_d[3];  // pd, pz, lastv after a walked chunk
+  __shared__ int st_i[5];     // pcol, have_prev, k, ndf, nrot
+  const int mi = blockIdx.x;
   if (mi >= nm) return;
+  const int lane = threadIdx.x;
   const DcMerge m = mg[mi];
   const int K = m.n1 + m.n2;
   const int o = m.off;
   const double rho = 2.0 * m.rho;
   const double eps = 1.1102230246251565e-16;
   double dmax = 0.0, zmax = 0.0;
-#pragma unroll 8
-  for (int j = 0; j < K; ++j) {
+  for (int j = lane; j < K; j += 64) {
     dmax = fmax(dmax, fabs(ds[o + j]));
     zmax = fmax(zmax, fabs(zs[o + j]));
   }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+    zmax = fmax(zmax, __shfl_xor(zmax, off, 64));
+  }
   const double tol = 8.0 * eps * fmax(dmax, zmax);
-  int k = 0, ndf = 0, nrot = 0;
-  double lastv = -HUGE_VAL;  // largest deflated value so far
-  auto push_df = [&](double v, int col) {
-    // ascending insertion (values arrive almost sorted)
-    int p = ndf;
-    if (v < lastv) {
-      while (p > 0 && dfd[o + p - 1] > v) {
-        dfd[o + p] = dfd[o + p - 1];
-        dfcol[o + p] = dfcol[o + p - 1];
-        --p;
+  if (rho * zmax <= tol) {  // every entry deflates, in their (ascending) order
+    for (int j = lane; j < K; j += 64) {
+      dfd[o + j] = ds[o + j];
+      dfcol[o + j] = cs[o + j];
+    }
+    if (lane == 0) {
+      counts[mi].k = 0;
+      counts[mi].nrot = 0;
+    }
+    return;
+  }
+  bool have_prev = false;
+  double pd = 0.0, pz = 0.0, lastv = -HUGE_VAL;
+  int pcol = 0, k = 0, ndf = 0, nrot = 0;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));  // lanes < lane
+  for (int b = 0; b < K; b += 64) {
+    const int j = b + lane;
+    const bool in = j < K;
+    const double nd = in ? ds[o + j] : 0.0, nz = in ? zs[o + j] : 0.0;
+    const int ncol = in ? cs[o + j] : 0;
+    const bool small = in && rho * fabs(nz) <= tol;
+    const bool cand = in && !small;
+    const uint64_t cm = __ballot(cand), sm = __ballot(small);
+    // previous candidate (in this chunk, else the pending one) and previous
+    // deflated value appended (in this chunk, else lastv)
+    const uint64_t cb = cm & below, sb = sm & below;
+    const int pl = cb ? 63 - __clzll(cb) : 0;
+    const int sl = sb ? 63 - __clzll(sb) : 0;
+    const double qd_in = __shfl(nd, pl, 64), qz_in = __shfl(nz, pl, 64);
+    const double sv_in = __shfl(nd, sl, 64);
+    bool rot = false;
+    if (cand && (cb || have_prev)) {
+      double c, s, tau;
+      rot = dc_rot_test(cb ? qd_in : pd, cb ? qz_in : pz, nd, nz, tol, c, s, tau);
+    }
+    const bool unordered = small && nd < (sb ? sv_in : lastv);
+    if (__ballot(rot) == 0 && __ballot(unordered) == 0) {
+      const int r = __popcll(cm);
+      if (r > 0) {
+        int base = k;
+        if (have_prev) {
+          if (lane == 0) {
+            dl[o + k] = pd;
+            wz[o + k] = pz;
+            ndcol[o + k] = pcol;
+          }
+          base = k + 1;
+        }
+        const int idx = __popcll(cb);
+        if (cand && idx < r - 1) {
+          dl[o + base + idx] = nd;
+          wz[o + base + idx] = nz;
+          ndcol[o + base + idx] = ncol;
+        }
+        k = base + r - 1;
+        const int L = 63 - __clzll(cm);
+        pd = __shfl(nd, L, 64);
+        pz = __shfl(nz, L, 64);
+        pcol = __shfl(ncol, L, 64);
+        have_prev = true;
       }
+      const int ns = __popcll(sm);
+      if (small) {
+        const int si = ndf + __popcll(sb);
+        dfd[o + si] = nd;
+        dfcol[o + si] = ncol;
+      }
+      if (ns > 0) lastv = __shfl(nd, 63 - __clzll(sm), 64);
+      ndf += ns;
     } else {
-      lastv = v;
-    }
-    dfd[o + p] = v;
-    dfcol[o + p] = col;
-    ++ndf;
-  };
-  if (rho * zmax <= tol) {
-    for (int j = 0; j < K; ++j) push_df(ds[o + j], cs[o + j]);
-  } else {
-    int j = 0;
-    // first non-negligible entry
-    for (; j < K; ++j) {
-      if (rho * fabs(zs[o + j]) <= tol)
-        push_df(ds[o + j], cs[o + j]);
-      else
-        break;
-    }
-    double pd = ds[o + j], pz = zs[o + j];
-    int pcol = cs[o + j];
-    for (++j; j < K; ++j) {
-      const double nd = ds[o + j], nz = zs[o + j];
-      const int ncol = cs[o + j];
-      if (rho * fabs(nz) <= tol) {
-        push_df(nd, ncol);
-        continue;
+      cd[lane] = nd;
+      cz[lane] = nz;
+      ccol[lane] = ncol;
+      __syncthreads();
+      if (lane == 0) {
+        const int cnt = min(64, K - b);
+        for (int q = 0; q < cnt; ++q) {
+          const double vd = cd[q], vz = cz[q];
+          const int vc = ccol[q];
+          const bool vsmall = rho * fabs(vz) <= tol;
+          auto push_df = [&](double v, int col) {
+            // ascending insertion (values arrive almost sorted)
+            int p = ndf;
+            if (v < lastv) {
+              while (p > 0 && dfd[o + p - 1] > v) {
+                dfd[o + p] = dfd[o + p - 1];
+                dfcol[o + p] = dfcol[o + p - 1];
+                --p;
+              }
+            } else {
+              lastv = v;
+            }
+            dfd[o + p] = v;
+            dfcol[o + p] = col;
+            ++ndf;
+          };
+          if (vsmall) {
+            push_df(vd, vc);
+          } else if (!have_prev) {
+            pd = vd;
+            pz = vz;
+            pcol = vc;
+            have_prev = true;
+          } else {
+            double c, s, tau;
+            if (dc_rot_test(pd, pz, vd, vz, tol, c, s, tau)) {
+              // rotate columns (pcol, vc): x' = c x + s y, y' = c y - s x
+              rp[o + nrot] = pcol;
+              rn[o + nrot] = vc;
+              rc[o + nrot] = c;
+              rs[o + nrot] = s;
+              ++nrot;
+              double tp, tn;
+              {
+#pragma clang fp contract(off)
+                tp = pd * c * c + vd * s * s;
+                tn = pd * s * s + vd * c * c;
+              }
+              push_df(tp, pcol);
+              pd = tn;
+              pz = tau;
+              pcol = vc;
+            } else {
+              dl[o + k] = pd;
+              wz[o + k] = pz;
+              ndcol[o + k] = pcol;
+              ++k;
+              pd = vd;
+              pz = vz;
+              pcol = vc;
+            }
+          }
+        }
+        st_d[0] = pd;
+        st_d[1] = pz;
+        st_d[2] = lastv;
+        st_i[0] = pcol;
+        st_i[1] = have_prev ? 1 : 0;
+        st_i[2] = k;
+        st_i[3] = ndf;
+        st_i[4] = nrot;
       }
-      double s = pz, c = nz;
-      const double tau = dlapy2_d(c, s);
-      const double t = nd - pd;
-      c = c / tau;
-      s = -s / tau;
-      if (fabs(t * c * s) <= tol) {
-        // rotate columns (pcol, ncol): x' = c x + s y, y' = c y - s x
-        rp[o + nrot] = pcol;
-        rn[o + nrot] = ncol;
-        rc[o + nrot] = c;
-        rs[o + nrot] = s;
-        ++nrot;
-        const double tp = pd * c * c + nd * s * s;
-        const double tn = pd * s * s + nd * c * c;
-        push_df(tp, pcol);
-        pd = tn;
-        pz = tau;
-        pcol = ncol;
-      } else {
-        dl[o + k] = pd;
-        wz[o + k] = pz;
-        ndcol[o + k] = pcol;
-        ++k;
-        pd = nd;
-        pz = nz;
-        pcol = ncol;
-      }
+      __syncthreads();
+      pd = st_d[0];
+      pz = st_d[1];
+      lastv = st_d[2];
+      pcol = st_i[0];
+      have_prev = st_i[1] != 0;
+      k = st_i[2];
+      ndf = st_i[3];
+      nrot = st_i[4];
+      __syncthreads();  // before the next walked chunk overwrites the staging arrays
     }
+  }
+  if (lane == 0) {
     dl[o + k] = pd;
     wz[o + k] = pz;
     ndcol[o + k] = pcol;
-    ++k;
+    counts[mi].k = k + 1;
+    counts[mi].nrot = nrot;
   }
-  counts[mi].k = k;
-  counts[mi].nrot = nrot;
 }
 
 // ---- step 3: apply the deflation rotations to Z's columns ------------------
@@ -875,7 +998,8 @@ hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t
     const int* rpre = P.d_rprefix + P.rprefix_off[l];
     hipLaunchKernelGGL(dc_sort_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, mg, nm, w->D,
                        (const double*)nullptr, w->ds, w->zs, w->cs);
-    hipLaunchKernelGGL(dc_deflate_kernel, dim3(ceil_div(nm, 64)), dim3(64), 0, st, mg, nm, w->ds,
+    if (nm > 0)
+      hipLaunchKernelGGL(dc_deflate_kernel, dim3(nm), dim3(64), 0, st, mg, nm, w->ds,
                        w->zs, w->cs, w->dl, w->wz, w->ndcol, w->dfd, w->dfcol, w->rp, w->rn, w->rc,
                        w->rs, counts);
     hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, mg, nm,
