@@ -48,6 +48,8 @@ struct DcMerge {
 struct DcCount {
   int k;     // non-deflated
   int nrot;  // deflation rotations
+  int k1;    // non-deflated columns with entries in the first half's rows only
+  int k2;    // ... in both halves' rows (mixed by a rotation); k - k1 - k2: second half only
 };
 
 struct DcPlan;
@@ -60,7 +62,7 @@ struct DcWork {
          *zhat = nullptr, *U = nullptr, *Zg = nullptr, *Zo = nullptr, *scale = nullptr,
          *dcat = nullptr, *ecat = nullptr;
   int *cs = nullptr, *ndcol = nullptr, *dfcol = nullptr, *rp = nullptr, *rn = nullptr,
-      *org = nullptr, *d_prefix = nullptr;
+      *org = nullptr, *d_prefix = nullptr, *ndpos = nullptr;
   DcMerge* d_merges = nullptr;
   DcCount* d_counts = nullptr;
   GemmDesc* d_descs = nullptr;
@@ -71,7 +73,8 @@ static void dc_free(DcWork* w) {
   void* ptrs[] = {w->D,   w->ds,    w->zs,    w->dl,       w->wz,       w->dfd,    w->rc,
                   w->rs,  w->tau,   w->lam,   w->zhat,     w->U,        w->Zg,     w->Zo,
                   w->scale, w->dcat, w->ecat, w->cs,  w->ndcol, w->dfcol,    w->rp,
-                  w->rn,     w->org, w->d_prefix, w->d_merges, w->d_counts, w->d_descs};
+                  w->rn,     w->org, w->d_prefix, w->d_merges, w->d_counts, w->d_descs,
+                  w->ndpos};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   DcPlan* plan = w->plan;
@@ -103,7 +106,7 @@ static hipError_t dc_reserve(DcWork* w, int64_t n, int64_t nn) {
   IMCOM_HIP_TRY(hipMalloc(&w->U, nnb));
   IMCOM_HIP_TRY(hipMalloc(&w->Zg, nnb));
   IMCOM_HIP_TRY(hipMalloc(&w->Zo, nnb));
-  int** iv[] = {&w->cs, &w->ndcol, &w->dfcol, &w->rp, &w->rn, &w->org};
+  int** iv[] = {&w->cs, &w->ndcol, &w->dfcol, &w->rp, &w->rn, &w->org, &w->ndpos};
   for (int** q : iv) IMCOM_HIP_TRY(hipMalloc(q, ni));
   IMCOM_HIP_TRY(hipMalloc(&w->d_prefix, ni + sizeof(int)));
   IMCOM_HIP_TRY(hipMalloc(&w->d_merges, (size_t)n * sizeof(DcMerge)));
@@ -250,11 +253,25 @@ __device__ __forceinline__ bool dc_rot_test(double pd, double pz, double nd, dou
 // changes the previous candidate's d and z.  The outputs are those of the
 // scalar scan over the whole merge.  Scan state: the pending candidate
 // (pd, pz, pcol) once one exists, k non-deflated so far, ndf deflated, nrot
-// rotations, lastv the largest deflated value appended.
+// rotations, lastv the largest deflated value appended, pmix whether the
+// pending column mixes both halves (a rotation between a first-half and a
+// second-half column, or a column that already did).
+//
+// Each non-deflated column's rows are nonzero in the first half, the second
+// half or both (type 0 / 1 / 2: the first half's columns, the rotated mixed
+// ones, the second half's).  ndpos[q] receives the column's place in the
+// type-grouped order (all type 0, then 1, then 2, each in scan order), and
+// counts the numbers k1 (type 0) and k2 (type 1): the eigenvector product
+// then runs per half over the columns with entries in that half.
+__device__ __forceinline__ int dc_col_type(int col, bool mix, int n1) {
+  return mix ? 1 : (col < n1 ? 0 : 2);
+}
+
 __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restrict__ mg, int nm,
                                   const double* __restrict__ ds, const double* __restrict__ zs,
                                   const int* __restrict__ cs, double* __restrict__ dl,
                                   double* __restrict__ wz, int* __restrict__ ndcol,
+                                  int* __restrict__ ndpos,
                                   double* __restrict__ dfd, int* __restrict__ dfcol,
                                   int* __restrict__ rp, int* __restrict__ rn,
                                   double* __restrict__ rc, double* __restrict__ rs,
@@ -262,12 +279,13 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
   __shared__ double cd[64], cz[64];
   __shared__ int ccol[64];
   __shared__ double st_d[3];  // pd, pz, lastv after a walked chunk
-  __shared__ int st_i[5];     // pcol, have_prev, k, ndf, nrot
+  __shared__ int st_i[6];     // pcol, have_prev, k, ndf, nrot, pmix
   const int mi = blockIdx.x;
   if (mi >= nm) return;
   const int lane = threadIdx.x;
   const DcMerge m = mg[mi];
   const int K = m.n1 + m.n2;
+  const int n1 = m.n1;
   const int o = m.off;
   const double rho = 2.0 * m.rho;
   const double eps = 1.1102230246251565e-16;
@@ -290,10 +308,12 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
     if (lane == 0) {
       counts[mi].k = 0;
       counts[mi].nrot = 0;
+      counts[mi].k1 = 0;
+      counts[mi].k2 = 0;
     }
     return;
   }
-  bool have_prev = false;
+  bool have_prev = false, pmix = false;
   double pd = 0.0, pz = 0.0, lastv = -HUGE_VAL;
   int pcol = 0, k = 0, ndf = 0, nrot = 0;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));  // lanes < lane
@@ -327,6 +347,7 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
             dl[o + k] = pd;
             wz[o + k] = pz;
             ndcol[o + k] = pcol;
+            ndpos[o + k] = dc_col_type(pcol, pmix, n1);
           }
           base = k + 1;
         }
@@ -335,6 +356,7 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
           dl[o + base + idx] = nd;
           wz[o + base + idx] = nz;
           ndcol[o + base + idx] = ncol;
+          ndpos[o + base + idx] = dc_col_type(ncol, false, n1);
         }
         k = base + r - 1;
         const int L = 63 - __clzll(cm);
@@ -342,6 +364,7 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
         pz = __shfl(nz, L, 64);
         pcol = __shfl(ncol, L, 64);
         have_prev = true;
+        pmix = false;
       }
       const int ns = __popcll(sm);
       if (small) {
@@ -385,6 +408,7 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
             pz = vz;
             pcol = vc;
             have_prev = true;
+            pmix = false;
           } else {
             double c, s, tau;
             if (dc_rot_test(pd, pz, vd, vz, tol, c, s, tau)) {
@@ -401,6 +425,7 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
                 tn = pd * s * s + vd * c * c;
               }
               push_df(tp, pcol);
+              pmix = pmix || ((pcol < n1) != (vc < n1));
               pd = tn;
               pz = tau;
               pcol = vc;
@@ -408,10 +433,12 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
               dl[o + k] = pd;
               wz[o + k] = pz;
               ndcol[o + k] = pcol;
+              ndpos[o + k] = dc_col_type(pcol, pmix, n1);
               ++k;
               pd = vd;
               pz = vz;
               pcol = vc;
+              pmix = false;
             }
           }
         }
@@ -423,6 +450,7 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
         st_i[2] = k;
         st_i[3] = ndf;
         st_i[4] = nrot;
+        st_i[5] = pmix ? 1 : 0;
       }
       __syncthreads();
       pd = st_d[0];
@@ -433,6 +461,7 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
       k = st_i[2];
       ndf = st_i[3];
       nrot = st_i[4];
+      pmix = st_i[5] != 0;
       __syncthreads();  // before the next walked chunk overwrites the staging arrays
     }
   }
@@ -440,8 +469,37 @@ __global__ __launch_bounds__(64) void dc_deflate_kernel(const DcMerge* __restric
     dl[o + k] = pd;
     wz[o + k] = pz;
     ndcol[o + k] = pcol;
-    counts[mi].k = k + 1;
+    ndpos[o + k] = dc_col_type(pcol, pmix, n1);
+  }
+  ++k;
+  __syncthreads();  // the types, written by any lane, are read back below
+  // type-grouped places: counts of types 0 and 1, then each column's place
+  int c0 = 0, c1 = 0;
+  for (int b = 0; b < k; b += 64) {
+    const int q = b + lane;
+    const int t = q < k ? ndpos[o + q] : -1;
+    c0 += __popcll(__ballot(t == 0));
+    c1 += __popcll(__ballot(t == 1));
+  }
+  int r0 = 0, r1 = c0, r2 = c0 + c1;  // next place of each type
+  for (int b = 0; b < k; b += 64) {
+    const int q = b + lane;
+    const int t = q < k ? ndpos[o + q] : -1;
+    const uint64_t m0 = __ballot(t == 0), m1 = __ballot(t == 1), m2 = __ballot(t == 2);
+    if (q < k) {
+      const uint64_t mt = t == 0 ? m0 : (t == 1 ? m1 : m2);
+      const int rt = t == 0 ? r0 : (t == 1 ? r1 : r2);
+      ndpos[o + q] = rt + __popcll(mt & below);
+    }
+    r0 += __popcll(m0);
+    r1 += __popcll(m1);
+    r2 += __popcll(m2);
+  }
+  if (lane == 0) {
+    counts[mi].k = k;
     counts[mi].nrot = nrot;
+    counts[mi].k1 = c0;
+    counts[mi].k2 = c1;
   }
 }
 
@@ -649,6 +707,7 @@ __global__ __launch_bounds__(256) void dc_vec_kernel(const DcMerge* __restrict__
                                                      const int* __restrict__ orgall,
                                                      const double* __restrict__ tauall,
                                                      const double* __restrict__ zhat,
+                                                     const int* __restrict__ ndpos,
                                                      double* __restrict__ Uall) {
   __shared__ double red[4];
   const int gr = blockIdx.x;
@@ -669,10 +728,10 @@ __global__ __launch_bounds__(256) void dc_vec_kernel(const DcMerge* __restrict__
   const double tj = tauall[m.off + j];
   double* U = Uall + m.u_off + (int64_t)j * k;
   double ss = 0.0;
-  for (int i = threadIdx.x; i < k; i += blockDim.x) {
+  for (int i = threadIdx.x; i < k; i += blockDim.x) {  // row i goes to its grouped place
     const double del = (dl[i] - dorg) - tj;
     const double u = zhat[m.off + i] / del;
-    U[i] = u;
+    U[ndpos[m.off + i]] = u;
     ss += u * u;
   }
   ss = block_sum(ss, red);
@@ -680,12 +739,13 @@ __global__ __launch_bounds__(256) void dc_vec_kernel(const DcMerge* __restrict__
   for (int i = threadIdx.x; i < k; i += blockDim.x) U[i] *= inv;
 }
 
-// ---- step 7: gather Z columns: non-deflated first, then deflated ------------
-// grid.y = global column g (merge m, local column q); Zg_m is K x K (ld K).
+// ---- step 7: gather Z columns: non-deflated first (in their type-grouped
+// places), then deflated; grid.x = global column g (merge m, local column q);
+// Zg_m is K x K (ld K).
 __global__ void dc_gather_kernel(int64_t n, const DcMerge* __restrict__ mg, int nm,
                                  const DcCount* __restrict__ counts, const int* __restrict__ ndcol,
-                                 const int* __restrict__ dfcol, const double* __restrict__ Z,
-                                 double* __restrict__ Zg) {
+                                 const int* __restrict__ ndpos, const int* __restrict__ dfcol,
+                                 const double* __restrict__ Z, double* __restrict__ Zg) {
   const int64_t g = blockIdx.x;
   const int mi = find_merge(mg, nm, g);
   if (mi < 0) return;
@@ -694,8 +754,9 @@ __global__ void dc_gather_kernel(int64_t n, const DcMerge* __restrict__ mg, int 
   const int k = counts[mi].k;
   const int q = (int)(g - m.off);
   const int src = (q < k) ? ndcol[m.off + q] : dfcol[m.off + (q - k)];
+  const int dst = (q < k) ? ndpos[m.off + q] : q;
   const double* zc = m.Z + m.loff + (int64_t)(m.loff + src) * m.ldz;
-  double* out = Zg + m.zg_off + (int64_t)q * K;
+  double* out = Zg + m.zg_off + (int64_t)dst * K;
   for (int r = threadIdx.x; r < K; r += blockDim.x) out[r] = zc[r];
 }
 
@@ -777,14 +838,30 @@ __global__ void dc_rho_kernel(DcMerge* __restrict__ mg, int nm, const double* __
 // the level's eigenvector products Zo_m (K x k) = Zg_m(:, 0:k) U_m (k x k,
 // ld k) take their k from the deflation counts on the device: descriptors
 // built for k = K are cut down here; blocks beyond a product's tiles exit
+// Each merge's eigenvector product per half: rows [0, n1) take the grouped
+// columns [0, k1 + k2) (types 0 and 1), rows [n1, K) the columns [k1, k)
+// (types 1 and 2), each against the same rows of U.  dmerge[i] = 2 x (merge
+// index in the level) + half.  Formed from the plan's fixed parts on every
+// solve, so replays of a captured solve find them as set here.
 __global__ void dc_fix_descs_kernel(GemmDesc* __restrict__ d, const int* __restrict__ dmerge,
-                                    int nd, const DcCount* __restrict__ counts) {
+                                    int nd, const DcCount* __restrict__ counts,
+                                    const DcMerge* __restrict__ mg, double* __restrict__ Zg,
+                                    double* __restrict__ U) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nd) return;
-  const int k = counts[dmerge[i]].k;
-  d[i].N = k;
-  d[i].K = k;
-  d[i].ldb = k > 0 ? k : 1;
+  const int mi = dmerge[i] >> 1, half = dmerge[i] & 1;
+  const DcCount c = counts[mi];
+  const DcMerge m = mg[mi];
+  const int K = m.n1 + m.n2;
+  d[i].N = c.k;
+  d[i].ldb = c.k > 0 ? c.k : 1;
+  if (half == 0) {
+    d[i].K = c.k1 + c.k2;
+  } else {
+    d[i].K = c.k - c.k1;
+    d[i].A = Zg + m.zg_off + m.n1 + (int64_t)c.k1 * K;
+    d[i].B = U + m.u_off + c.k1;
+  }
 }
 
 // ---- plan -------------------------------------------------------------------
@@ -898,9 +975,13 @@ static hipError_t dc_build_plan(hipStream_t st, DcWork* w, DcPlan& P, int ntiles
       zoff += (int64_t)K * K;
       L.maxK = std::max(L.maxK, K);
       const int big = (int64_t)((K + 127) / 128) * ((K + 127) / 128) >= kDc128Min ? 1 : 0;
-      dl[big].push_back(GemmDesc{K, K, K, 1.0, 0.0, w->Zg + m.zg_off, K, w->U + m.u_off, K,
+      // per half (sizes and the second half's operands set on the device)
+      dl[big].push_back(GemmDesc{m.n1, K, K, 1.0, 0.0, w->Zg + m.zg_off, K, w->U + m.u_off, K,
                                  w->Zo + m.zg_off, K});
-      ml[big].push_back(i);
+      ml[big].push_back(2 * i);
+      dl[big].push_back(GemmDesc{m.n2, K, K, 1.0, 0.0, w->Zg + m.zg_off + m.n1, K, w->U + m.u_off,
+                                 K, w->Zo + m.zg_off + m.n1, K});
+      ml[big].push_back(2 * i + 1);
     }
     L.roots = rpre.back() - rpre[P.rprefix_off.back()];
     for (int b = 0; b < 2; ++b) {
@@ -912,8 +993,8 @@ static hipError_t dc_build_plan(hipStream_t st, DcWork* w, DcPlan& P, int ntiles
       for (size_t q = 0; q < dl[b].size(); ++q) {
         descs.push_back(dl[b][q]);
         dmerge.push_back(ml[b][q]);
-        const int K = dl[b][q].M;
-        gpre.push_back(gpre.back() + ((K + tile - 1) / tile) * ((K + tile - 1) / tile));
+        const int M = dl[b][q].M, Nw = dl[b][q].N;  // worst case: N = K
+        gpre.push_back(gpre.back() + ((M + tile - 1) / tile) * ((Nw + tile - 1) / tile));
       }
       L.blocks[b] = gpre.back();
     }
@@ -1000,8 +1081,8 @@ hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t
                        (const double*)nullptr, w->ds, w->zs, w->cs);
     if (nm > 0)
       hipLaunchKernelGGL(dc_deflate_kernel, dim3(nm), dim3(64), 0, st, mg, nm, w->ds,
-                       w->zs, w->cs, w->dl, w->wz, w->ndcol, w->dfd, w->dfcol, w->rp, w->rn, w->rc,
-                       w->rs, counts);
+                       w->zs, w->cs, w->dl, w->wz, w->ndcol, w->ndpos, w->dfd, w->dfcol, w->rp,
+                       w->rn, w->rc, w->rs, counts);
     hipLaunchKernelGGL(dc_rotate_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, st, N, mg, nm,
                        (const DcCount*)counts, w->rp, w->rn, w->rc, w->rs, (double*)nullptr);
     hipLaunchKernelGGL(dc_secular_kernel, dim3(ceil_div(L.roots, 4)), dim3(256), 0, st, mg,
@@ -1011,16 +1092,18 @@ hipError_t dc_solve_batched(hipStream_t st, DcWork* w, int ntiles, const int64_t
                        (const DcCount*)counts, rpre, nm, L.roots, w->dl, w->wz, w->org, w->tau,
                        w->zhat);
     hipLaunchKernelGGL(dc_vec_kernel, dim3(L.roots), dim3(256), 0, st, mg, (const DcCount*)counts,
-                       rpre, nm, w->dl, w->org, w->tau, w->zhat, w->U);
+                       rpre, nm, w->dl, w->org, w->tau, w->zhat, (const int*)w->ndpos, w->U);
     const int tpb = std::min(256, std::max(64, (L.maxK + 63) / 64 * 64));
     hipLaunchKernelGGL(dc_gather_kernel, dim3((unsigned)N), dim3(tpb), 0, st, N, mg, nm,
-                       (const DcCount*)counts, w->ndcol, w->dfcol, (const double*)nullptr, w->Zg);
+                       (const DcCount*)counts, w->ndcol, w->ndpos, w->dfcol, (const double*)nullptr,
+                       w->Zg);
     // Zo_m = Zg_m(:, 0:k) U_m, with k from the device counts
     for (int b = 0; b < 2; ++b) {
       if (L.ndesc[b] == 0) continue;
       GemmDesc* d = P.d_descs + L.desc_off[b];
       hipLaunchKernelGGL(dc_fix_descs_kernel, dim3(ceil_div(L.ndesc[b], 64)), dim3(64), 0, st, d,
-                         P.d_dmerge + L.desc_off[b], L.ndesc[b], (const DcCount*)counts);
+                         P.d_dmerge + L.desc_off[b], L.ndesc[b], (const DcCount*)counts, mg, w->Zg,
+                         w->U);
       IMCOM_HIP_TRY(dgemm_grouped(st, false, false, d, P.d_gprefix + P.gprefix_off[b][l],
                                   L.ndesc[b], L.blocks[b], b ? 128 : 64));
     }
