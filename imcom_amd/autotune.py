@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "data", "tuning.json")
 
 CLASSES = [1024, 2048, 4096, 8192, 16384]          # upper bounds of the n classes
-CANDIDATES = [(nb, nbb) for nb in (16, 32, 64) for nbb in (64, 128, 256)]
+CANDIDATES = [(nb, nbb) for nb in (16, 32, 64) for nbb in (64, 128, 256, 512)]
 DEFAULT = (32, 128)
 
 

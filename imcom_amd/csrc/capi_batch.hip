@@ -381,7 +381,8 @@ int imcom_gpu_batch_result(imcom_gpu_ctx* ctx, int tile, double* K_out, double* 
 extern "C" int imcom_gpu_set_tuning(imcom_gpu_ctx* ctx, int nb, int nbb) {
   if (!ctx) return IMCOM_EARG;
   NEED(nb == 16 || nb == 32 || nb == 64, IMCOM_EARG, "set_tuning: nb must be 16, 32 or 64");
-  NEED(nbb == 64 || nbb == 128 || nbb == 256, IMCOM_EARG, "set_tuning: nbb must be 64, 128 or 256");
+  NEED(nbb == 64 || nbb == 128 || nbb == 256 || nbb == 512, IMCOM_EARG,
+       "set_tuning: nbb must be 64, 128, 256 or 512");
   ctx->nb = nb;
   ctx->nbb = nbb;
   return IMCOM_OK;

@@ -207,7 +207,8 @@ hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
 struct SyPlan {
   GroupedPlan gp;
   std::vector<int> syr2k;            // grouped launch per panel
-  std::vector<std::array<int, 6>> bt;  // grouped launches per back-transform block
+  std::vector<std::vector<int>> bt;  // grouped launches per back-transform block: S, Y,
+                                     // Y2, Z update, then T's joins (two each)
 };
 void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb, SyPlan& plan);
 hipError_t sytrd_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,

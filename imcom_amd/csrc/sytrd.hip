@@ -783,6 +783,22 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
 // first: Z[kb+1:, :] -= Vb (T (Vb^T Z[kb+1:, :])).  Z is n x n (ld n).  For
 // NBB = 256, T's two 128 diagonal blocks come from bt_build_T and the block
 // above them is T12 = -T1 (V1^T V2) T2 (the DLARFT recursion).
+// The joins that complete T from its kMaxNBB diagonal blocks (bt_build_T):
+// for widths w = kMaxNBB, 2 kMaxNBB, .. < nbb and each pair of neighbouring
+// finished blocks (i, i + w), T(i:i+w, i+w:i+w+w2) = -T1 S12 T2 with
+// T1 = T(i:i+w, i:i+w), T2 = T(i+w:.., i+w:..) and S12 = S(i:i+w, i+w:..);
+// X = S12 T2 goes to the scratch `x` (ld w).  Two products per join, in
+// this order (a join's T2 is finished by the joins of the width before).
+struct TJoin {
+  int i, w, w2;
+};
+static std::vector<TJoin> t_joins(int nbb) {
+  std::vector<TJoin> v;
+  for (int w = kMaxNBB; w < nbb; w *= 2)
+    for (int i = 0; i + w < nbb; i += 2 * w) v.push_back(TJoin{i, w, std::min(w, nbb - i - w)});
+  return v;
+}
+
 hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nbb) {
   if (nmax <= 1) return hipSuccess;
@@ -807,13 +823,12 @@ hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
       if (kb >= h.n - 1) continue;
       const int64_t rows = h.n - kb - 1;
       double* Zr = h.Z + kb + 1;
-      if (nbb > kMaxNBB) {
-        // T(0:128, 128:nbb) = -T1 S12 T2 (the two diagonal blocks joined; X in Y2)
-        const int n2 = nbb - kMaxNBB;
-        IMCOM_HIP_TRY(dgemm(st, false, false, kMaxNBB, n2, n2, 1.0, h.Sb + (int64_t)kMaxNBB * nbb,
-                            nbb, h.Tb + kMaxNBB + (int64_t)kMaxNBB * nbb, nbb, 0.0, h.Y2, kMaxNBB));
-        IMCOM_HIP_TRY(dgemm(st, false, false, kMaxNBB, n2, kMaxNBB, -1.0, h.Tb, nbb, h.Y2, kMaxNBB,
-                            0.0, h.Tb + (int64_t)kMaxNBB * nbb, nbb));
+      for (const TJoin& j : t_joins(nbb)) {  // X in Y2
+        const int64_t c0 = j.i + j.w;  // first column of the block above the diagonal
+        IMCOM_HIP_TRY(dgemm(st, false, false, j.w, j.w2, j.w2, 1.0, h.Sb + j.i + c0 * nbb, nbb,
+                            h.Tb + c0 + c0 * nbb, nbb, 0.0, h.Y2, j.w));
+        IMCOM_HIP_TRY(dgemm(st, false, false, j.w, j.w2, j.w, -1.0, h.Tb + j.i + (int64_t)j.i * nbb,
+                            nbb, h.Y2, j.w, 0.0, h.Tb + j.i + c0 * nbb, nbb));
       }
       // Y = Vb^T Z_r  (nbb x n; split K)
       IMCOM_HIP_TRY(dgemm_splitk(st, true, false, nbb, (int)h.n, (int)rows, 1.0, h.Vb, rows, Zr,
@@ -856,8 +871,9 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
     const int64_t nblk = (nmax - 1 + nbb - 1) / nbb;
     for (int64_t bi = nblk - 1; bi >= 0; --bi) {
       const int kb = (int)(bi * nbb);
-      std::vector<GemmDesc> gs, gy, gy2, gz, gj1, gj2;
-      const int n2 = nbb - kMaxNBB;
+      std::vector<GemmDesc> gs, gy, gy2, gz;
+      const std::vector<TJoin> joins = t_joins(nbb);
+      std::vector<std::vector<GemmDesc>> gj(2 * joins.size());
       for (const SyTile& h : host) {
         if (kb >= h.n - 1) continue;
         const int rows = (int)(h.n - kb - 1);
@@ -867,20 +883,22 @@ void sy_make_plan(const std::vector<SyTile>& host, int64_t nmax, int nb, int nbb
         gy.push_back(GemmDesc{nbb, n, rows, 1.0, 0.0, h.Vb, rows, Zr, h.n, h.Y, nbb});
         gy2.push_back(GemmDesc{nbb, n, nbb, 1.0, 0.0, h.Tb, nbb, h.Y, nbb, h.Y2, nbb});
         gz.push_back(GemmDesc{rows, n, nbb, -1.0, 1.0, h.Vb, rows, h.Y2, nbb, Zr, h.n});
-        if (n2 > 0) {
-          gj1.push_back(GemmDesc{kMaxNBB, n2, n2, 1.0, 0.0, h.Sb + (int64_t)kMaxNBB * nbb, nbb,
-                                 h.Tb + kMaxNBB + (int64_t)kMaxNBB * nbb, nbb, h.Y2, kMaxNBB});
-          gj2.push_back(GemmDesc{kMaxNBB, n2, kMaxNBB, -1.0, 0.0, h.Tb, nbb, h.Y2, kMaxNBB,
-                                 h.Tb + (int64_t)kMaxNBB * nbb, nbb});
+        for (size_t q = 0; q < joins.size(); ++q) {
+          const TJoin& j = joins[q];
+          const int64_t c0 = j.i + j.w;
+          gj[2 * q].push_back(GemmDesc{j.w, j.w2, j.w2, 1.0, 0.0, h.Sb + j.i + c0 * nbb, nbb,
+                                       h.Tb + c0 + c0 * nbb, nbb, h.Y2, j.w});
+          gj[2 * q + 1].push_back(GemmDesc{j.w, j.w2, j.w, -1.0, 0.0,
+                                           h.Tb + j.i + (int64_t)j.i * nbb, nbb, h.Y2, j.w,
+                                           h.Tb + j.i + c0 * nbb, nbb});
         }
       }
-      std::array<int, 6> ids;
-      ids[0] = plan.gp.add(true, false, gs);
-      ids[1] = plan.gp.add(true, false, gy);
-      ids[2] = plan.gp.add(false, false, gy2);
-      ids[3] = plan.gp.add(false, false, gz);
-      ids[4] = plan.gp.add(false, false, gj1);
-      ids[5] = plan.gp.add(false, false, gj2);
+      std::vector<int> ids;
+      ids.push_back(plan.gp.add(true, false, gs));
+      ids.push_back(plan.gp.add(true, false, gy));
+      ids.push_back(plan.gp.add(false, false, gy2));
+      ids.push_back(plan.gp.add(false, false, gz));
+      for (auto& g : gj) ids.push_back(plan.gp.add(false, false, g));
       plan.bt.push_back(ids);
     }
   }
@@ -916,14 +934,13 @@ hipError_t ormtr_run_batched(hipStream_t st, const SyTile* d_tiles, int ntiles, 
   int idx = 0;
   for (int64_t bi = nblk - 1; bi >= 0; --bi, ++idx) {
     const int kb = (int)(bi * nbb);
-    const std::array<int, 6>& ids = plan.bt[idx];
+    const std::vector<int>& ids = plan.bt[idx];
     hipLaunchKernelGGL(bt_gather, dim3(ceil_div(nmax - kb - 1, 256), nbb, ntiles), dim3(256), 0,
                        st, d_tiles, kb, nbb);
     IMCOM_HIP_TRY(plan.gp.run(st, ids[0]));
     hipLaunchKernelGGL(bt_build_T, dim3(ceil_div(nbb, kMaxNBB), 1, ntiles), dim3(256), 0, st,
                        d_tiles, kb, nbb);
-    IMCOM_HIP_TRY(plan.gp.run(st, ids[4]));
-    IMCOM_HIP_TRY(plan.gp.run(st, ids[5]));
+    for (size_t q = 4; q < ids.size(); ++q) IMCOM_HIP_TRY(plan.gp.run(st, ids[q]));  // T's joins
     IMCOM_HIP_TRY(plan.gp.run(st, ids[1]));
     IMCOM_HIP_TRY(plan.gp.run(st, ids[2]));
     IMCOM_HIP_TRY(plan.gp.run(st, ids[3]));

@@ -200,7 +200,8 @@ def test_tridiagonalize_small(gpu_ctx, small_tile, small_ref):
 
 
 @pytest.mark.parametrize("n,nb,nbb", [(5, 32, 128), (130, 32, 128), (700, 32, 128),
-                                       (700, 64, 256), (1100, 16, 256)])
+                                       (700, 64, 256), (1100, 16, 256), (700, 64, 512),
+                                       (1300, 32, 512)])
 def test_eigen_random_spd(gpu_ctx, n, nb, nbb):
     _say(f"eigen_A random n={n} nb={nb} nbb={nbb}")
     rng = np.random.default_rng(3 * n)
