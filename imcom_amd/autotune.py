@@ -63,6 +63,7 @@ def tune(g, classes=(1024, 2048, 4096), candidates=CANDIDATES, repeats=2, seed=0
         A = _spd(n, seed + n)
         t = synth.tiny()
         t.n, t.m = n, 4
+        t.grid = None  # the coordinates below are not on the tile's grids
         t.x_i, t.y_i = np.zeros(n), np.zeros(n)
         t.exp_i = np.ones(n, dtype=np.int32)
         t.X_a, t.Y_a = np.zeros(4), np.zeros(4)

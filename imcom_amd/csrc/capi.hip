@@ -7,6 +7,8 @@
 #include "kernels.h"
 
 #include <algorithm>
+#include <cmath>
+#include <numeric>
 #include <atomic>
 #include <cstdio>
 #include <cstring>
@@ -84,7 +86,8 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
                     &ctx->mlp_bins, &ctx->mlp_hits, &ctx->mlp_X, &ctx->mlp_Y, &ctx->mlp_perm,
                     &ctx->mlp_loss, &ctx->mlp_grad, &ctx->mlp_m, &ctx->mlp_v, &ctx->mlp_out,
                     &ctx->ms_Xn, &ctx->ms_H1, &ctx->ms_dZ1, &ctx->ms_H2, &ctx->ms_dZ2,
-                    &ctx->ms_dY};
+                    &ctx->ms_dY,    &ctx->gkx,  &ctx->gky,   &ctx->latA_desc, &ctx->latB_desc,
+                    &ctx->lat_vals};
   for (DevBuf* b : bufs) b->release();
   ctx->ew.release();
   ctx->tw.release();
@@ -139,6 +142,145 @@ int imcom_gpu_set_tile(imcom_gpu_ctx* ctx, int64_t n, int64_t m, const double* x
   CK(hipStreamSynchronize(ctx->stream), "set_tile sync");
   ctx->have_tile = true;
   ctx->have_A = ctx->have_B = ctx->have_QL = ctx->have_P = ctx->have_K = ctx->have_T = false;
+  ctx->have_grid = ctx->lat_A = ctx->lat_B = false;  // a new tile's grids are declared anew
+  return IMCOM_OK;
+}
+
+// sr / sc = P / Q with coprime P, Q <= 64 (smallest P + Q first), to 1e-13
+// relative: the lattice of two grid pitches.
+static bool pitch_ratio(double sr, double sc, int& P, int& Q) {
+  if (!(sr > 0.0 && sc > 0.0) || !std::isfinite(sr) || !std::isfinite(sc)) return false;
+  for (int tot = 2; tot <= 128; ++tot)
+    for (int p = std::max(1, tot - 64); p <= std::min(64, tot - 1); ++p) {
+      const int q = tot - p;
+      if (std::gcd(p, q) != 1) continue;
+      if (std::fabs(sr * q - sc * p) <= 1e-13 * sr * q) {
+        P = p;
+        Q = q;
+        return true;
+      }
+    }
+  return false;
+}
+
+// The lattice of the subtracted grid (pitch sr, offset dr, Nr points) and the
+// added grid (sc, dc, Nc); false when the pitches have no small ratio.
+static bool lat_axis(double sr, double dr, int Nr, double sc, double dc, int Nc, LatAxis& a) {
+  if (!pitch_ratio(sr, sc, a.P, a.Q)) return false;
+  a.Nr = Nr;
+  a.Nc = Nc;
+  a.L = a.P * (Nr - 1) + a.Q * (Nc - 1) + 1;
+  a.g = sr / a.P;
+  a.doff = dc - dr;
+  return true;
+}
+
+// Lattices beyond this many values (for all pairs of one matrix) are not
+// formed; the per-entry kernels serve such grids.
+static const int64_t kLatMaxValues = (int64_t)1 << 25;
+
+int imcom_gpu_set_grid(imcom_gpu_ctx* ctx, int n1gim, int n2gim, int nexp, const double* gimxscale,
+                       const double* gimyscale, const double* dither, const int32_t* mask_subs_i,
+                       int n1out, int n2out, double outxscale, double outyscale, double outpos1,
+                       double outpos2) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_tile, IMCOM_ESTATE, "set_grid: tile inputs not set (imcom_gpu_set_tile)");
+  ctx->have_grid = ctx->lat_A = ctx->lat_B = false;
+  if (n1gim == 0) return IMCOM_OK;  // grids cleared: per-entry lookups
+  NEED(n1gim >= 1 && n2gim >= 1 && nexp >= 1 && n1out >= 1 && n2out >= 1, IMCOM_EARG,
+       "set_grid: grid sizes must be >= 1");
+  NEED(gimxscale && gimyscale && dither && mask_subs_i, IMCOM_EARG, "set_grid: NULL input");
+  NEED((int64_t)n1out * n2out == ctx->m, IMCOM_EARG, "set_grid: n1out * n2out must equal m");
+  NEED((int64_t)n1gim * n2gim * nexp >= ctx->n, IMCOM_EARG,
+       "set_grid: fewer grid pixels than input pixels");
+  const int64_t n = ctx->n, m = ctx->m;
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  // per-exposure pitches and dithers, mask_subs, the check counter
+  std::vector<double> geo(4 * (size_t)nexp);
+  for (int e = 0; e < nexp; ++e) {
+    geo[e] = gimxscale[e];
+    geo[nexp + e] = gimyscale[e];
+    geo[2 * nexp + e] = dither[e];         // dither(e, 1)
+    geo[3 * nexp + e] = dither[nexp + e];  // dither(e, 2)
+  }
+  DevBuf dgeo, dsubs, dbad;
+  CK(dgeo.reserve(geo.size() * 8), "alloc grid");
+  CK(dsubs.reserve((size_t)n * 4), "alloc mask_subs");
+  CK(dbad.reserve(4), "alloc grid check");
+  CK(ctx->gkx.reserve((size_t)n * 4), "alloc grid x");
+  CK(ctx->gky.reserve((size_t)n * 4), "alloc grid y");
+  int32_t bad = 0;
+  hipError_t e = hipMemcpyAsync(dgeo.p, geo.data(), geo.size() * 8, hipMemcpyHostToDevice,
+                                ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dsubs.p, mask_subs_i, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dbad.p, &bad, 4, hipMemcpyHostToDevice, ctx->stream);
+  const double* g = dgeo.as<double>();
+  if (e == hipSuccess)
+    e = launch_grid_check(ctx->stream, n, m, ctx->x.as<double>(), ctx->y.as<double>(),
+                          ctx->ex.as<int32_t>(), dsubs.as<int32_t>(), n1gim, n2gim, nexp, g,
+                          g + nexp, g + 2 * nexp, n1out, outxscale, outyscale, outpos1, outpos2,
+                          ctx->Xa.as<double>(), ctx->Ya.as<double>(), ctx->gkx.as<int32_t>(),
+                          ctx->gky.as<int32_t>(), dbad.as<int32_t>());
+  if (e == hipSuccess) e = hipMemcpyAsync(&bad, dbad.p, 4, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dgeo.release();
+  dsubs.release();
+  dbad.release();
+  CK(e, "set_grid: grid check");
+  NEED(bad == 0, IMCOM_EARG,
+       "set_grid: " + std::to_string(bad) +
+           " pixels do not lie on the declared grids (x_i, y_i, exp_i, X_a, Y_a)");
+  // A: exposure pairs e <= f (descriptor e + f (f + 1) / 2, 0-based), grid e subtracted
+  std::vector<LatDesc> dA, dB;
+  bool okA = true, okB = true;
+  int64_t tot = 0;
+  for (int f = 0; f < nexp && okA; ++f)
+    for (int e2 = 0; e2 <= f && okA; ++e2) {
+      LatDesc d{};
+      okA = lat_axis(gimxscale[e2], dither[e2], n1gim, gimxscale[f], dither[f], n1gim, d.x) &&
+            lat_axis(gimyscale[e2], dither[nexp + e2], n2gim, gimyscale[f], dither[nexp + f],
+                     n2gim, d.y);
+      d.table = e2 + f * (f + 1) / 2;
+      d.off = tot;
+      tot += (int64_t)d.x.L * d.y.L;
+      dA.push_back(d);
+    }
+  okA = okA && tot <= kLatMaxValues;
+  ctx->lat_totA = tot;
+  // B: exposure e against the output grid (subtracted)
+  tot = 0;
+  for (int e2 = 0; e2 < nexp && okB; ++e2) {
+    LatDesc d{};
+    okB = lat_axis(outxscale, outpos1, n1out, gimxscale[e2], dither[e2], n1gim, d.x) &&
+          lat_axis(outyscale, outpos2, n2out, gimyscale[e2], dither[nexp + e2], n2gim, d.y);
+    d.table = e2;
+    d.off = tot;
+    tot += (int64_t)d.x.L * d.y.L;
+    dB.push_back(d);
+  }
+  okB = okB && tot <= kLatMaxValues;
+  ctx->lat_totB = tot;
+  if (okA) {
+    CK(upload(ctx, ctx->latA_desc, dA.data(), dA.size() * sizeof(LatDesc)), "upload lattice A");
+    ctx->lat_npA = (int)dA.size();
+  }
+  if (okB) {
+    CK(upload(ctx, ctx->latB_desc, dB.data(), dB.size() * sizeof(LatDesc)), "upload lattice B");
+    ctx->lat_npB = (int)dB.size();
+  }
+  CK(hipStreamSynchronize(ctx->stream), "set_grid sync");
+  ctx->grid_n1out = n1out;
+  ctx->have_grid = true;
+  ctx->lat_A = okA;
+  ctx->lat_B = okB;
+  return IMCOM_OK;
+}
+
+int imcom_gpu_lattice_state(imcom_gpu_ctx* ctx, int* A_lattice, int* B_lattice) {
+  if (!ctx) return IMCOM_EARG;
+  if (A_lattice) *A_lattice = ctx->lat_A ? 1 : 0;
+  if (B_lattice) *B_lattice = ctx->lat_B ? 1 : 0;
   return IMCOM_OK;
 }
 
@@ -189,13 +331,26 @@ int imcom_gpu_lookup_A(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   const int64_t n = ctx->n;
   CK(ctx->A.reserve((size_t)n * n * 8), "alloc A");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
+  if (ctx->lat_A) CK(ctx->lat_vals.reserve((size_t)ctx->lat_totA * 8), "alloc lattice");
   {
     StageTimer tm(ctx, PS_LOOKUP_A);
-    CK(launch_lookup_A(ctx->stream, ctx->device, n, ctx->x.as<double>(), ctx->y.as<double>(),
-                       ctx->ex.as<int32_t>(), ctx->psfconst, ctx->Atab.as<double>(),
-                       (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
-                       psfyscale, ctx->A.as<double>()),
-       "lookup_A");
+    if (ctx->lat_A) {  // regular grids: one interpolation per separation, then A
+      CK(launch_lattice_eval(ctx->stream, ctx->device, ctx->latA_desc.as<LatDesc>(), ctx->lat_npA,
+                             ctx->lat_totA, ctx->psfconst, ctx->Atab.as<double>(),
+                             (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
+                             psfyscale, ctx->lat_vals.as<double>()),
+         "lookup_A lattice");
+      CK(launch_lattice_A(ctx->stream, n, ctx->gkx.as<int32_t>(), ctx->gky.as<int32_t>(),
+                          ctx->ex.as<int32_t>(), ctx->latA_desc.as<LatDesc>(),
+                          ctx->lat_vals.as<double>(), ctx->A.as<double>()),
+         "lookup_A");
+    } else {
+      CK(launch_lookup_A(ctx->stream, ctx->device, n, ctx->x.as<double>(), ctx->y.as<double>(),
+                         ctx->ex.as<int32_t>(), ctx->psfconst, ctx->Atab.as<double>(),
+                         (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
+                         psfyscale, ctx->A.as<double>()),
+         "lookup_A");
+    }
   }
   ctx->have_A = true;
   CK(download(ctx, A_out, ctx->A, (size_t)n * n * 8), "download A");
@@ -223,14 +378,28 @@ int imcom_gpu_lookup_B(imcom_gpu_ctx* ctx, int npoly, int npad, double psfxscale
   const int64_t n = ctx->n, m = ctx->m;
   CK(ctx->B.reserve((size_t)n * m * 8), "alloc B");
   int np = npoly % 2 == 0 ? npoly - 1 : npoly;
+  if (ctx->lat_B) CK(ctx->lat_vals.reserve((size_t)ctx->lat_totB * 8), "alloc lattice");
   {
     StageTimer tm(ctx, PS_LOOKUP_B);
-    CK(launch_lookup_B(ctx->stream, ctx->device, n, m, ctx->x.as<double>(), ctx->y.as<double>(),
-                       ctx->ex.as<int32_t>(), ctx->Xa.as<double>(), ctx->Ya.as<double>(),
-                       ctx->psfconst, ctx->Btab.as<double>(), (int64_t)ctx->n1 * ctx->n2,
-                       ctx->n1, ctx->n2, np / 2, npad, psfxscale, psfyscale,
-                       ctx->B.as<double>()),
-       "lookup_B");
+    if (ctx->lat_B) {
+      CK(launch_lattice_eval(ctx->stream, ctx->device, ctx->latB_desc.as<LatDesc>(), ctx->lat_npB,
+                             ctx->lat_totB, ctx->psfconst, ctx->Btab.as<double>(),
+                             (int64_t)ctx->n1 * ctx->n2, ctx->n1, ctx->n2, np / 2, npad, psfxscale,
+                             psfyscale, ctx->lat_vals.as<double>()),
+         "lookup_B lattice");
+      CK(launch_lattice_B(ctx->stream, n, m, ctx->grid_n1out, ctx->gkx.as<int32_t>(),
+                          ctx->gky.as<int32_t>(), ctx->ex.as<int32_t>(),
+                          ctx->latB_desc.as<LatDesc>(), ctx->lat_vals.as<double>(),
+                          ctx->B.as<double>()),
+         "lookup_B");
+    } else {
+      CK(launch_lookup_B(ctx->stream, ctx->device, n, m, ctx->x.as<double>(), ctx->y.as<double>(),
+                         ctx->ex.as<int32_t>(), ctx->Xa.as<double>(), ctx->Ya.as<double>(),
+                         ctx->psfconst, ctx->Btab.as<double>(), (int64_t)ctx->n1 * ctx->n2,
+                         ctx->n1, ctx->n2, np / 2, npad, psfxscale, psfyscale,
+                         ctx->B.as<double>()),
+         "lookup_B");
+    }
   }
   ctx->have_B = true;
   CK(download(ctx, B_out, ctx->B, (size_t)n * m * 8), "download B");

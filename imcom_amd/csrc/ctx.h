@@ -156,6 +156,14 @@ struct imcom_gpu_ctx {
        have_K = false, have_T = false;
   // inputs
   DevBuf x, y, ex, Xa, Ya, Nd, Atab, Btab, I;
+  // regular input and output grids (imcom_gpu_set_grid): each pixel's grid
+  // position, and the separation lattices of A's exposure pairs and of B's
+  // exposures (lat_A / lat_B: the grids' pitches are in small integer ratios,
+  // so lookup_A / lookup_B interpolate each separation once)
+  bool have_grid = false, lat_A = false, lat_B = false;
+  int grid_n1out = 0, lat_npA = 0, lat_npB = 0;
+  int64_t lat_totA = 0, lat_totB = 0;
+  DevBuf gkx, gky, latA_desc, latB_desc, lat_vals;
   // stages
   DevBuf A, B, Q, L, P, K, US, T, W, H, S, U, evals, bracket;
   // eigensolver workspace

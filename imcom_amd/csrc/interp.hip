@@ -183,6 +183,165 @@ hipError_t launch_lookup_B(hipStream_t st, int dev, int64_t n, int64_t m, const 
   return hipGetLastError();
 }
 
+// ---- regular grids (USERXY=0): one interpolation per distinct separation ----
+//
+// With x_i = kx * gimxscale(e) + dither(e, 1) (imcom_build_xy,
+// imcom_proc.f90:56-89) and X_a = ax * outxscale + outpos1 (imcom_build_XaYa,
+// imcom_proc.f90:93-119), the arguments of imcom_lookup_A and imcom_lookup_B
+// (imcom_matrix.f90:230-231, 310-311) take few distinct values: for A, the
+// separation of two pixels depends only on their exposures and grid offsets,
+// (2 n1gim - 1) (2 n2gim - 1) values per exposure pair when the scales are
+// equal; for B, on the exposure and (P kx - Q ax) when the input and output
+// pitches are in the ratio P/Q.  Each such value is interpolated once into a
+// lattice (lattice_eval_kernel), and A and B are written from the lattices
+// with coalesced stores: the matrices' cost becomes their write bandwidth.
+// The lattice argument (u g + doff) equals the reference's (x_j - x_i) in
+// exact arithmetic; rounded, it differs by a few units in the last place of
+// the coordinates.
+
+__global__ __launch_bounds__(256) void grid_check_kernel(
+    int64_t n, int64_t m, const double* __restrict__ x, const double* __restrict__ y,
+    const int32_t* __restrict__ ex, const int32_t* __restrict__ mask_subs, int n1gim, int n2gim,
+    int nexp, const double* __restrict__ gsx, const double* __restrict__ gsy,
+    const double* __restrict__ dith, int n1out, double osx, double osy, double op1, double op2,
+    const double* __restrict__ Xa, const double* __restrict__ Ya, int32_t* __restrict__ kx,
+    int32_t* __restrict__ ky, int32_t* __restrict__ bad) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  bool ok = true;
+  auto near = [](double a, double b) { return fabs(a - b) <= 1e-12 * (1.0 + fabs(b)); };
+  if (t < n) {
+    const int64_t sub = (int64_t)mask_subs[t] - 1;
+    const int64_t plane = (int64_t)n1gim * n2gim;
+    if (sub < 0 || sub >= plane * nexp) {
+      ok = false;
+      kx[t] = ky[t] = 0;
+    } else {
+      const int ix = (int)(sub % n1gim), iy = (int)((sub / n1gim) % n2gim);
+      const int e = (int)(sub / plane);  // 0-based
+      kx[t] = ix;
+      ky[t] = iy;
+      ok = (ex[t] == e + 1) && near((double)ix * gsx[e] + dith[e], x[t]) &&
+           near((double)iy * gsy[e] + dith[nexp + e], y[t]);
+    }
+  } else if (t < n + m) {
+    const int64_t a = t - n;
+    const int ax = (int)(a % n1out), ay = (int)(a / n1out);
+    ok = near((double)ax * osx + op1, Xa[a]) && near((double)ay * osy + op2, Ya[a]);
+  }
+  if (!ok) atomicAdd(bad, 1);
+}
+
+__global__ __launch_bounds__(256) void lattice_eval_kernel(
+    const LatDesc* __restrict__ desc, int npairs, int64_t total, int psfconst,
+    const double* __restrict__ tabs, int64_t tstride, int n1, int n2, int npm, double npad,
+    double sx, double sy, double* __restrict__ vals) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= total) return;
+  int p = 0;
+  while (p + 1 < npairs && desc[p + 1].off <= t) ++p;
+  const LatDesc d = desc[p];
+  const int64_t r = t - d.off;
+  const int ux = (int)(r % d.x.L), uy = (int)(r / d.x.L);
+  const double Dx = (double)(ux - d.x.P * (d.x.Nr - 1)) * d.x.g + d.x.doff;
+  const double Dy = (double)(uy - d.y.P * (d.y.Nr - 1)) * d.y.g + d.y.doff;
+  const double Delx = npad * Dx / sx;
+  const double Dely = npad * Dy / sy;
+  const int tab = psfconst == 1 ? 0 : d.table;
+  vals[t] = interp_lookup(tabs + (int64_t)tab * tstride, n1, n2, npm, Delx, Dely);
+}
+
+constexpr int kLatCols = 16;  // matrix columns per workgroup in the expansions
+
+// Pair (e_lo, e_hi) of A is descriptor (e_lo - 1) + e_hi (e_hi - 1) / 2, the
+// reference's table index ial less one.
+__global__ __launch_bounds__(256) void lattice_A_kernel(int64_t n, const int32_t* __restrict__ kx,
+                                                        const int32_t* __restrict__ ky,
+                                                        const int32_t* __restrict__ ex,
+                                                        const LatDesc* __restrict__ desc,
+                                                        const double* __restrict__ vals,
+                                                        double* __restrict__ A) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t j0 = (int64_t)blockIdx.y * kLatCols;
+  if (i >= n) return;
+  const int kxi = kx[i], kyi = ky[i], ei = ex[i];
+  const int64_t jn = min(n, j0 + kLatCols);
+  for (int64_t j = j0; j < jn; ++j) {
+    const int kxj = kx[j], kyj = ky[j], ej = ex[j];
+    const bool up = i <= j;  // pixel i is the subtracted one
+    const int elo = up ? ei : ej, ehi = up ? ej : ei;
+    const int xlo = up ? kxi : kxj, xhi = up ? kxj : kxi;
+    const int ylo = up ? kyi : kyj, yhi = up ? kyj : kyi;
+    const LatDesc* d = desc + (elo - 1) + ehi * (ehi - 1) / 2;
+    const int ux = d->x.Q * xhi - d->x.P * xlo + d->x.P * (d->x.Nr - 1);
+    const int uy = d->y.Q * yhi - d->y.P * ylo + d->y.P * (d->y.Nr - 1);
+    A[i + j * n] = vals[d->off + ux + (int64_t)uy * d->x.L];
+  }
+}
+
+// B(i, a): the input pixel's grid is the added one, the output grid the
+// subtracted one (x_i - X_a); descriptor e - 1.
+__global__ __launch_bounds__(256) void lattice_B_kernel(int64_t n, int64_t m, int n1out,
+                                                        const int32_t* __restrict__ kx,
+                                                        const int32_t* __restrict__ ky,
+                                                        const int32_t* __restrict__ ex,
+                                                        const LatDesc* __restrict__ desc,
+                                                        const double* __restrict__ vals,
+                                                        double* __restrict__ B) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t a0 = (int64_t)blockIdx.y * kLatCols;
+  if (i >= n) return;
+  const LatDesc* d = desc + (ex[i] - 1);
+  const int bx = d->x.Q * kx[i] + d->x.P * (d->x.Nr - 1);
+  const int by = d->y.Q * ky[i] + d->y.P * (d->y.Nr - 1);
+  const double* v = vals + d->off;
+  const int Lx = d->x.L, Px = d->x.P, Py = d->y.P;
+  const int64_t an = min(m, a0 + kLatCols);
+  for (int64_t a = a0; a < an; ++a) {
+    const int ax = (int)(a % n1out), ay = (int)(a / n1out);
+    B[i + a * n] = v[(bx - Px * ax) + (int64_t)(by - Py * ay) * Lx];
+  }
+}
+
+hipError_t launch_grid_check(hipStream_t st, int64_t n, int64_t m, const double* x, const double* y,
+                             const int32_t* ex, const int32_t* mask_subs, int n1gim, int n2gim,
+                             int nexp, const double* gsx, const double* gsy, const double* dith,
+                             int n1out, double osx, double osy, double op1, double op2,
+                             const double* Xa, const double* Ya, int32_t* kx, int32_t* ky,
+                             int32_t* bad) {
+  const unsigned nb = (unsigned)((n + m + 255) / 256);
+  hipLaunchKernelGGL(grid_check_kernel, dim3(nb), dim3(256), 0, st, n, m, x, y, ex, mask_subs,
+                     n1gim, n2gim, nexp, gsx, gsy, dith, n1out, osx, osy, op1, op2, Xa, Ya, kx,
+                     ky, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_lattice_eval(hipStream_t st, int dev, const LatDesc* desc, int npairs,
+                               int64_t total, int psfconst, const double* tabs, int64_t tstride,
+                               int n1, int n2, int npm, int npad, double sx, double sy,
+                               double* vals) {
+  IMCOM_HIP_TRY(ensure_invden(dev));
+  hipLaunchKernelGGL(lattice_eval_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                     desc, npairs, total, psfconst, tabs, tstride, n1, n2, npm, (double)npad, sx,
+                     sy, vals);
+  return hipGetLastError();
+}
+
+hipError_t launch_lattice_A(hipStream_t st, int64_t n, const int32_t* kx, const int32_t* ky,
+                            const int32_t* ex, const LatDesc* desc, const double* vals, double* A) {
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)((n + kLatCols - 1) / kLatCols));
+  hipLaunchKernelGGL(lattice_A_kernel, grid, dim3(256), 0, st, n, kx, ky, ex, desc, vals, A);
+  return hipGetLastError();
+}
+
+hipError_t launch_lattice_B(hipStream_t st, int64_t n, int64_t m, int n1out, const int32_t* kx,
+                            const int32_t* ky, const int32_t* ex, const LatDesc* desc,
+                            const double* vals, double* B) {
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)((m + kLatCols - 1) / kLatCols));
+  hipLaunchKernelGGL(lattice_B_kernel, grid, dim3(256), 0, st, n, m, n1out, kx, ky, ex, desc, vals,
+                     B);
+  return hipGetLastError();
+}
+
 // imcom_rotate_image (imcom_proc.f90:184-250): output pixel (iout, jout),
 // 1-based, takes the polynomial through the clipped stencil around the
 // rotated position (xs, ys), or 0 when the position's floor/ceiling leave the

@@ -144,6 +144,47 @@ hipError_t launch_lookup_B(hipStream_t st, int dev, int64_t n, int64_t m, const 
                            int psfconst, const double* tabs, int64_t tstride, int n1, int n2,
                            int npm, int npad, double sx, double sy, double* B);
 
+// Separations between two regular 1-D grids, r(k) = k sr + dr (k < Nr, the
+// subtracted grid) and c(l) = l sc + dc (l < Nc), with sr = P g, sc = Q g for
+// coprime P, Q: c(l) - r(k) = (Q l - P k) g + doff, doff = dc - dr, so the
+// separations are the lattice index u = Q l - P k + P (Nr - 1) in [0, L),
+// L = P (Nr - 1) + Q (Nc - 1) + 1.
+struct LatAxis {
+  int P, Q, Nr, Nc, L;
+  double g, doff;
+};
+// One pair of grids (an exposure pair for A, an exposure and the output grid
+// for B): its x and y axes, the lookup table it reads unless all exposures
+// share table 0 (psfconst = 1), and where its Lx * Ly interpolated values
+// start in the value array (x index fastest).
+struct LatDesc {
+  LatAxis x, y;
+  int table;
+  int64_t off;
+};
+// The 0-based grid position of every pixel of a regular (USERXY=0) tile,
+// checked against the coordinates: kx, ky (n each) from the 1-based
+// mask_subs into the (n1gim, n2gim, nexp) order; bad counts the pixels whose
+// exposure or position do not match (and the output pixels off their grid).
+hipError_t launch_grid_check(hipStream_t st, int64_t n, int64_t m, const double* x, const double* y,
+                             const int32_t* ex, const int32_t* mask_subs, int n1gim, int n2gim,
+                             int nexp, const double* gsx, const double* gsy, const double* dith,
+                             int n1out, double osx, double osy, double op1, double op2,
+                             const double* Xa, const double* Ya, int32_t* kx, int32_t* ky,
+                             int32_t* bad);
+// Every pair's lattice values (total entries, descriptors in desc), then A
+// (n x n, both triangles: pixel lo = min(i, j) is the subtracted one, so
+// A(i, j) = A(j, i) exactly) or B (n x m) expanded from them.
+hipError_t launch_lattice_eval(hipStream_t st, int dev, const LatDesc* desc, int npairs,
+                               int64_t total, int psfconst, const double* tabs, int64_t tstride,
+                               int n1, int n2, int npm, int npad, double sx, double sy,
+                               double* vals);
+hipError_t launch_lattice_A(hipStream_t st, int64_t n, const int32_t* kx, const int32_t* ky,
+                            const int32_t* ex, const LatDesc* desc, const double* vals, double* A);
+hipError_t launch_lattice_B(hipStream_t st, int64_t n, int64_t m, int n1out, const int32_t* kx,
+                            const int32_t* ky, const int32_t* ex, const LatDesc* desc,
+                            const double* vals, double* B);
+
 // ---- (d) bisection and per-pixel epilogues --------------------------------
 hipError_t launch_bisect(hipStream_t st, int64_t n, int64_t m, const double* L, const double* P,
                          int64_t ldp, const double* Nd, double C_a, double US_max, double tol,

@@ -42,6 +42,17 @@ MODULE imcom_gpu_iface
       integer(c_int32_t), intent(in) :: exp_i(*)
       real(c_double), value :: C_a
     end function
+    integer(c_int) function imcom_gpu_set_grid(ctx, n1gim, n2gim, nexp, gimxscale, gimyscale, &
+                                               dither, mask_subs_i, n1out, n2out, outxscale, &
+                                               outyscale, outpos1, outpos2) &
+        bind(C, name="imcom_gpu_set_grid")
+      import :: c_int, c_ptr, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: n1gim, n2gim, nexp, n1out, n2out
+      real(c_double), intent(in) :: gimxscale(*), gimyscale(*), dither(*)
+      integer(c_int32_t), intent(in) :: mask_subs_i(*)
+      real(c_double), value :: outxscale, outyscale, outpos1, outpos2
+    end function
     integer(c_int) function imcom_gpu_set_tables(ctx, psfconst, nexppsf, n1, n2, Alookup, Blookup) &
         bind(C, name="imcom_gpu_set_tables")
       import :: c_int, c_ptr, c_double

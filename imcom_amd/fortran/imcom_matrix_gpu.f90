@@ -84,6 +84,17 @@ if (.not.(allocated(x_i).and.allocated(y_i).and.allocated(exp_i).and.allocated(X
 end if
 call imcom_gpu_check(imcom_gpu_context, imcom_gpu_set_tile(imcom_gpu_context, int(n, c_int64_t), &
                      int(m, c_int64_t), x_i, y_i, exp_i, X_a, Y_a, Ndiag, C_a), "set_tile")
+! USERXY = 0 (imcom.f90:64-71): the driver built x_i, y_i, X_a, Y_a from the
+! regular grids, whose pitches and dithers imcom_read_inconfigs allocated
+! only in that case (imcom_io.f90:695-703); the lookups then interpolate each
+! pixel separation once
+if (allocated(gimxscale).and.allocated(gimyscale).and.allocated(dither) &
+    .and.allocated(mask_subs_i)) then
+  call imcom_gpu_check(imcom_gpu_context, imcom_gpu_set_grid(imcom_gpu_context, &
+                       int(n1gim, c_int), int(n2gim, c_int), int(nexp, c_int), gimxscale, &
+                       gimyscale, dither, mask_subs_i, int(n1out, c_int), int(n2out, c_int), &
+                       outxscale, outyscale, outpos1, outpos2), "set_grid")
+end if
 imcom_gpu_tile_ready = .true.
 END SUBROUTINE imcom_gpu_tile
 

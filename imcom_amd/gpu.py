@@ -27,7 +27,8 @@ IMCOM_PREC_BF16X6 = 2
 # every symbol declared in include/imcom_gpu.h
 EXPORTS = [
     "imcom_gpu_init", "imcom_gpu_finalize", "imcom_gpu_last_error", "imcom_gpu_synchronize",
-    "imcom_gpu_set_tile", "imcom_gpu_set_tables", "imcom_gpu_assemble_inputs",
+    "imcom_gpu_set_tile", "imcom_gpu_set_grid", "imcom_gpu_lattice_state",
+    "imcom_gpu_set_tables", "imcom_gpu_assemble_inputs",
     "imcom_gpu_rotate_image", "imcom_gpu_ft_r2c", "imcom_gpu_build_Alookup",
     "imcom_gpu_build_Blookup", "imcom_gpu_calc_C", "imcom_gpu_lookup_A", "imcom_gpu_set_A",
     "imcom_gpu_eigen_A", "imcom_gpu_tridiagonalize", "imcom_gpu_tridiag_eig",
@@ -96,6 +97,9 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_last_error": (ctypes.c_char_p, [_vp]),
         "imcom_gpu_synchronize": (_int, [_vp]),
         "imcom_gpu_set_tile": (_int, [_vp, _i64, _i64, _dp, _dp, _ip, _dp, _dp, _dp, _dbl]),
+        "imcom_gpu_set_grid": (_int, [_vp, _int, _int, _int, _dp, _dp, _dp, _ip, _int, _int,
+                                       _dbl, _dbl, _dbl, _dbl]),
+        "imcom_gpu_lattice_state": (_int, [_vp, _ip, _ip]),
         "imcom_gpu_set_tables": (_int, [_vp, _int, _int, _int, _int, _dp, _dp]),
         "imcom_gpu_lookup_A": (_int, [_vp, _int, _int, _dbl, _dbl, _dp]),
         "imcom_gpu_set_A": (_int, [_vp, _dp]),
@@ -217,11 +221,13 @@ class ImcomGPU:
             raise ImcomError(f"IMCOM ERROR: {what}: {msg} (code {rc})")
 
     # ---- inputs ---------------------------------------------------------
-    def set_tile(self, tile, tables="host"):
+    def set_tile(self, tile, tables="host", grid=True):
         """Hand over the imcom_data globals of one tile.  tables="host"
         uploads the tile's Alookup/Blookup; "device" builds them on the GPU
         from the tile's spectra Gt_rot and Gammat (build_Alookup/Blookup);
-        "none" hands over no tables (a T read from TFILE needs none)."""
+        "none" hands over no tables (a T read from TFILE needs none).
+        grid=True also declares a USERXY=0 tile's regular grids (set_grid),
+        so that lookup_A/B interpolate each pixel separation once."""
         self.n, self.m = tile.n, tile.m
         self._keep = [_vec(tile.x_i), _vec(tile.y_i), _vec(tile.exp_i, np.int32),
                       _vec(tile.X_a), _vec(tile.Y_a), _vec(tile.Ndiag)]
@@ -229,6 +235,11 @@ class ImcomGPU:
         self._ck(self.lib.imcom_gpu_set_tile(self.h, tile.n, tile.m, _ptr(x), _ptr(y), _ptr(e, _ip),
                                              _ptr(X), _ptr(Y), _ptr(N), tile.C_a), "set_tile")
         self._inflight = {}  # set_tile has waited for every copy
+        g = getattr(tile, "grid", None) if grid else None
+        if g:
+            if len(g["mask_subs_i"]) != tile.n or g["n1out"] * g["n2out"] != tile.m:
+                raise ImcomError("set_tile: tile.grid does not describe this tile's pixels")
+            self.set_grid(**g)
         if tables == "device":
             if tile.Gt_rot is None or tile.Gammat is None:
                 raise ImcomError("set_tile(tables='device') needs the tile's spectra")
@@ -243,6 +254,31 @@ class ImcomGPU:
         else:
             raise ValueError("tables must be 'host', 'device' or 'none'")
         self.tile = tile
+
+    def set_grid(self, n1gim, n2gim, nexp, gimxscale, gimyscale, dither, mask_subs_i, n1out,
+                 n2out, outxscale, outyscale, outpos1, outpos2):
+        """The regular grids of a USERXY=0 tile (imcom_build_xy /
+        imcom_build_XaYa, imcom_proc.f90:56-119): dither is (nexp, 2),
+        mask_subs_i the 1-based unmasked indices.  n1gim = 0 clears them."""
+        gx, gy = _vec(gimxscale), _vec(gimyscale)
+        dth = np.asfortranarray(np.asarray(dither, dtype=np.float64).reshape(nexp, 2))
+        ms = _vec(mask_subs_i, np.int32)
+        self._grid_keep = [gx, gy, dth, ms]
+        self._ck(self.lib.imcom_gpu_set_grid(self.h, n1gim, n2gim, nexp, _ptr(gx), _ptr(gy),
+                                             _ptr(dth), _ptr(ms, _ip), n1out, n2out, outxscale,
+                                             outyscale, outpos1, outpos2), "set_grid")
+
+    def clear_grid(self):
+        """Per-entry lookups from here on (the grids are forgotten)."""
+        self._ck(self.lib.imcom_gpu_set_grid(self.h, 0, 0, 0, None, None, None, None, 0, 0, 0.0,
+                                             0.0, 0.0, 0.0), "set_grid")
+
+    def lattice_state(self):
+        """(A, B): whether lookup_A / lookup_B take the grid lattices."""
+        a, b = ctypes.c_int(0), ctypes.c_int(0)
+        self._ck(self.lib.imcom_gpu_lattice_state(self.h, ctypes.byref(a), ctypes.byref(b)),
+                 "lattice_state")
+        return bool(a.value), bool(b.value)
 
     # ---- lookup-table set-up (imcom_proc.f90:184-250,433-475, imcom_matrix.f90:361-552)
     def assemble_inputs(self, I_unmasked, saturation, gimxscale, gimyscale, dither):

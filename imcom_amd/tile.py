@@ -270,6 +270,9 @@ class Tile:
     US_tol: float = 1e-9
     maxNbis: int = 53
     meta: dict = dataclasses.field(default_factory=dict)
+    # USERXY=0 regular grids, the arguments of ImcomGPU.set_grid (None for
+    # USERXY=1 tiles, whose coordinates come from files)
+    grid: dict = None
     Gt_rot: list = None        # PSF spectra (n1pad^2 complex, nexppsf of them)
     Gammat: np.ndarray = None  # target spectrum
 
@@ -316,6 +319,15 @@ def assemble_tile(*, I_unmasked, gimxscale=None, gimyscale=None, dither=None, ps
         Gt.append(ft_r2c(G))
     Gammat = ft_r2c(embed_psf(gamma_img, n1pad))
     n = x_i.shape[0]
+    grid = None
+    if xy_unmasked is None and XaYa_grid is None:
+        dth = np.asarray(dither, dtype=np.float64).reshape(nexp, 2)
+        grid = dict(n1gim=n1gim, n2gim=n2gim, nexp=nexp,
+                    gimxscale=np.asarray(gimxscale, dtype=np.float64).copy(),
+                    gimyscale=np.asarray(gimyscale, dtype=np.float64).copy(),
+                    dither=dth.copy(), mask_subs_i=(mask_subs + 1).astype(np.int32),
+                    n1out=n1out, n2out=n2out, outxscale=float(outxscale),
+                    outyscale=float(outyscale), outpos1=float(outpos1), outpos2=float(outpos2))
     Ndiag = build_Ndiag(n, noise)
     C_a = calc_C(Gammat)
     return Tile(n=n, m=X_a.shape[0], x_i=x_i, y_i=y_i, exp_i=exp_i.astype(np.int32),
@@ -326,4 +338,4 @@ def assemble_tile(*, I_unmasked, gimxscale=None, gimyscale=None, dither=None, ps
                 psfconst=psfconst, nexppsf=nexppsf, npad=npad, npoly=npoly,
                 psfxscale=psfxscale, psfyscale=psfyscale, n1out=n1out, n2out=n2out,
                 USB=USB, US_max=US_max, US_tol=US_tol, meta=dict(meta or {}, n1pad=n1pad),
-                Gt_rot=Gt, Gammat=Gammat)
+                grid=grid, Gt_rot=Gt, Gammat=Gammat)

@@ -68,6 +68,29 @@ int imcom_gpu_set_tile(imcom_gpu_ctx* ctx, int64_t n, int64_t m, const double* x
                        const double* y_i, const int32_t* exp_i, const double* X_a,
                        const double* Y_a, const double* Ndiag, double C_a);
 
+/* The regular grids of a USERXY=0 tile, as the imcom_data globals hold them:
+ * input pixel (k1, k2) of exposure e (0-based k) at
+ * x = k1 * gimxscale(e) + dither(e, 1), y = k2 * gimyscale(e) + dither(e, 2)
+ * (imcom_build_xy, imcom_proc.f90:56-89; dither is (nexp, 2) column-major),
+ * mask_subs_i (n entries, 1-based into the (n1gim, n2gim, nexp) order) the
+ * unmasked pixels, and output pixel (a1, a2) at a1 * outxscale + outpos1,
+ * a2 * outyscale + outpos2 (imcom_build_XaYa, imcom_proc.f90:93-119).  Call
+ * after imcom_gpu_set_tile; every pixel's coordinates are checked against the
+ * grids (IMCOM_EARG if one differs by more than 1e-12 relative).  With grids
+ * whose pitches are in small integer ratios, imcom_gpu_lookup_A / _B
+ * interpolate each distinct pixel separation once and write the matrices from
+ * those values (the same as the per-entry lookups to the rounding of the
+ * separations).  n1gim = 0 clears the grids; imcom_gpu_set_tile clears them
+ * too.  No reference counterpart: imcom_lookup_A/B (imcom_matrix.f90:206-357)
+ * read only x_i, y_i, X_a, Y_a. */
+int imcom_gpu_set_grid(imcom_gpu_ctx* ctx, int n1gim, int n2gim, int nexp, const double* gimxscale,
+                       const double* gimyscale, const double* dither, const int32_t* mask_subs_i,
+                       int n1out, int n2out, double outxscale, double outyscale, double outpos1,
+                       double outpos2);
+/* Whether imcom_gpu_lookup_A / _B take the grid lattices (1) or evaluate
+ * every entry (0). */
+int imcom_gpu_lattice_state(imcom_gpu_ctx* ctx, int* A_lattice, int* B_lattice);
+
 /* Lookup tables built on the host by imcom_build_Alookup / imcom_build_Blookup
  * (imcom_matrix.f90:361-493): Alookup is (n1, n2, nexppsf*(nexppsf+1)/2),
  * Blookup is (n1, n2, nexppsf), psfconst as imcom_test_psfconst returns it. */

@@ -4,7 +4,9 @@ whole FP64 path through the C ABI.  The oracle cannot solve this tile in
 seconds, so every stage is checked where it can be, at the GPU's own inputs:
 
   every pixel         U < U_max, kappa inside [kappa_min, kappa_max]
-  A, B                rows 0-95 against the oracle's Neville lookup, 1e-13 of max
+  A, B                rows 0-95 against the oracle's Neville lookup, 1e-13 of max;
+                      the lattice lookups (regular grids) against the per-entry
+                      kernels over the whole matrices, 1e-13 of max
   P = Q^T B           8 columns against numpy, entrywise 1e-13 of |Q|^T |B|
   eigenpairs          the 4 largest and 4 smallest: |A q - lambda q| <= 100 n eps
                       lambda_max, |q_i . q_j - delta_ij| <= 100 n eps
@@ -44,8 +46,29 @@ def large():
         T = g.build_T(out=True)
         S = g.build_S()
         H = g.build_image()
+        lattice = g.lattice_state()
+        g.clear_grid()  # the per-entry lookups, for the lattice comparison
+        A_pe = g.lookup_A(t.npoly, t.npad, out=True)
+        B_pe = g.lookup_B(t.npoly, t.npad, out=True)
     rng = np.random.default_rng(16384)
-    return dict(t=t, A=A, Q=Q, L=L, cond=cond, B=B, P=P, K=K, U=U, ev=ev, T=T, S=S, H=H, rng=rng)
+    return dict(t=t, A=A, Q=Q, L=L, cond=cond, B=B, P=P, K=K, U=U, ev=ev, T=T, S=S, H=H, rng=rng,
+                lattice=lattice, A_pe=A_pe, B_pe=B_pe)
+
+
+def test_large_lattice_lookups_match_per_entry(large):
+    """The solve's A and B came from the grid lattices (one interpolation per
+    pixel separation); the per-entry kernels give the same matrices to
+    1e-13 of their largest entry, and A is exactly symmetric."""
+    print("[case] large lattice A, B vs per-entry", flush=True)
+    assert large["lattice"] == (True, True)
+    A, B = large["A"], large["B"]
+    dA = np.abs(A - large["A_pe"]).max()
+    dB = np.abs(B - large["B_pe"]).max()
+    print(f"[large] lattice vs per-entry: max|dA| {dA:.3e} (max|A| {np.abs(A).max():.3e}), "
+          f"max|dB| {dB:.3e} (max|B| {np.abs(B).max():.3e})", flush=True)
+    assert dA <= 1e-13 * np.abs(large["A_pe"]).max()
+    assert dB <= 1e-13 * np.abs(large["B_pe"]).max()
+    assert np.array_equal(A, A.T)
 
 
 def test_large_every_pixel_meets_target(large):

@@ -24,6 +24,7 @@ def test_split_P_random(gpu_ctx, scheme, n, m):
     rng = np.random.default_rng(n * 7 + m + scheme)
     t = synth.tiny()
     t.n, t.m = n, m
+    t.grid = None  # the coordinates below are not on the tile's grids
     t.x_i, t.y_i = np.zeros(n), np.zeros(n)
     t.exp_i = np.ones(n, dtype=np.int32)
     t.X_a, t.Y_a = np.zeros(m), np.zeros(m)

@@ -46,6 +46,7 @@ def test_gemm_P_random(gpu_ctx, n, m):
     rng = np.random.default_rng(n + m)
     t = synth.tiny()
     t.n, t.m = n, m
+    t.grid = None  # the coordinates below are not on the tile's grids
     t.x_i = rng.uniform(0, 8, n)
     t.y_i = rng.uniform(0, 8, n)
     t.exp_i = np.ones(n, dtype=np.int32)
@@ -209,6 +210,7 @@ def test_eigen_random_spd(gpu_ctx, n, nb, nbb):
     A = np.asfortranarray(X @ X.T / n)
     t = synth.tiny()
     t.n, t.m = n, 4
+    t.grid = None  # the coordinates below are not on the tile's grids
     t.x_i, t.y_i = np.zeros(n), np.zeros(n)
     t.exp_i = np.ones(n, dtype=np.int32)
     t.X_a, t.Y_a = np.zeros(4), np.zeros(4)
@@ -245,6 +247,7 @@ def test_eigen_maximum_size(gpu_ctx):
     A[np.diag_indices(n)] += d
     t = synth.tiny()
     t.n, t.m = n, 4
+    t.grid = None  # the coordinates below are not on the tile's grids
     t.x_i, t.y_i = np.zeros(n), np.zeros(n)
     t.exp_i = np.ones(n, dtype=np.int32)
     t.X_a, t.Y_a = np.zeros(4), np.zeros(4)
