@@ -71,7 +71,13 @@ class StreamedBatch:
 
     def solve(self, **kw):
         """imcom_gpu_batch_solve on every context at once (keyword arguments
-        as ImcomGPU.batch_solve); returns when all have finished."""
+        as ImcomGPU.batch_solve); returns when all have finished.  The
+        contexts' workspaces are allocated first, on this thread, so that no
+        device allocation runs while another thread captures a solve graph."""
+        from .gpu import IMCOM_PREC_FP64
+        for g in self.ctxs:
+            g.batch_reserve(kw.get("precision_P", IMCOM_PREC_FP64),
+                            kw.get("precision_T", IMCOM_PREC_FP64))
         errors = []
 
         def work(g):

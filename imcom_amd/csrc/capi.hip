@@ -417,7 +417,28 @@ int imcom_gpu_set_B(imcom_gpu_ctx* ctx, const double* B) {
   return IMCOM_OK;
 }
 
+// A persistent reduction tail whose grid barrier ran out of time (its
+// workgroups could not all be resident) leaves an abort flag: the reduction
+// is then incomplete and reported as an error.
+static int check_tail(imcom_gpu_ctx* ctx, int64_t tail_S) {
+  if (tail_S < 2) return IMCOM_OK;
+  unsigned flag = 0;
+  CK(sytrd_tail_status(ctx->stream, tail_S, ctx->ew.stail.as<double>(), &flag),
+     "reduction tail status");
+  NEED(flag != 1, IMCOM_EHIP,
+       "tridiagonal reduction: the persistent tail kernel's grid barrier timed out "
+       "(its workgroups were not all resident); set_reduction_tail(0) avoids it");
+  NEED(flag == 0, IMCOM_EHIP,
+       "tridiagonal reduction: the persistent tail kernel found its barrier count not reset; "
+       "set_reduction_tail(0) avoids it");
+  return IMCOM_OK;
+}
+
 static hipError_t reserve_eigen(imcom_gpu_ctx* ctx, int64_t n, SyTile& t) {
+  if (ctx->tail_rows >= 2) {
+    const int S = (int)std::min<int64_t>(n, std::min(ctx->tail_rows, kTailMaxRows));
+    if (S >= 2) IMCOM_HIP_TRY(ctx->ew.stail.reserve(tail_xch_doubles(S) * 8));
+  }
   IMCOM_HIP_TRY(ctx->Q.reserve((size_t)n * n * 8));
   IMCOM_HIP_TRY(ctx->L.reserve((size_t)n * 8));
   return ctx->ew.reserve(n, ctx->nb, ctx->nbb, ctx->A.as<double>(), ctx->Q.as<double>(), t);
@@ -436,10 +457,12 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
      "upload tile descriptor");
   std::vector<SyTile> host(1, t);
   const SyTile* dt = ctx->ew.stile.as<SyTile>();
+  int64_t tail_S = 0;
   // 1. A = H T H^T (reflectors left in A's lower part)
   {
     StageTimer tm(ctx, PS_SYTRD);
-    CK(sytrd_run(ctx->stream, dt, 1, n, host, ctx->nb, live_contexts(ctx->device), &ctx->prof),
+    CK(sytrd_run(ctx->stream, dt, 1, n, host, ctx->nb, live_contexts(ctx->device), &ctx->prof,
+                 ctx->tail_rows, ctx->ew.stail.as<double>(), &tail_S),
        "tridiagonal reduction");
   }
   // 2. T = Z diag(lam) Z^T, ascending
@@ -459,6 +482,10 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
     CK(ormtr_run(ctx->stream, dt, 1, n, host, ctx->nbb), "back-transform");
   }
   ctx->have_A = false;  // A was overwritten by the reduction
+  {
+    int rc = check_tail(ctx, tail_S);
+    if (rc) return rc;
+  }
   ctx->have_QL = true;
   ctx->have_P = ctx->have_K = ctx->have_T = false;
   CK(download(ctx, Q_out, ctx->Q, (size_t)n * n * 8), "download Q");
@@ -487,10 +514,16 @@ int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out) {
   CK(hipMemcpyAsync(ctx->ew.stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
      "upload tile descriptor");
   std::vector<SyTile> host(1, t);
+  int64_t tail_S = 0;
   CK(sytrd_run(ctx->stream, ctx->ew.stile.as<SyTile>(), 1, n, host, ctx->nb,
-               live_contexts(ctx->device), &ctx->prof),
+               live_contexts(ctx->device), &ctx->prof, ctx->tail_rows,
+               ctx->ew.stail.as<double>(), &tail_S),
      "tridiagonal reduction");
   ctx->have_A = false;
+  {
+    int rc = check_tail(ctx, tail_S);
+    if (rc) return rc;
+  }
   CK(download(ctx, d_out, ctx->ew.sd, (size_t)n * 8), "download d");
   if (n > 1) CK(download(ctx, e_out, ctx->ew.se, (size_t)(n - 1) * 8), "download e");
   CK(hipStreamSynchronize(ctx->stream), "sync");

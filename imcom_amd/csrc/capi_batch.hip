@@ -306,6 +306,29 @@ static hipError_t run_graph(imcom_gpu_ctx* ctx, hipGraphExec_t* exec, int prec_P
   return hipGraphLaunch(*exec, st);
 }
 
+// The split-precision workspace a batched solve with these precisions needs.
+static int batch_reserve_split(imcom_gpu_ctx* ctx, int precision_P, int precision_T) {
+  if (precision_P == IMCOM_PREC_FP64 && precision_T == IMCOM_PREC_FP64) return IMCOM_OK;
+  size_t wb = 0;
+  for (BatchTile* t : ctx->batch->tiles) {
+    const int n = (int)t->in.n, m = (int)t->in.m;
+    wb = std::max(wb, gemm_split_work_bytes(std::max(precision_P, precision_T), n, m, n));
+    if (precision_P) wb = std::max(wb, gemm_split_work_bytes(precision_P, n, m, n));
+    if (precision_T) wb = std::max(wb, gemm_split_work_bytes(precision_T, n, m, n));
+  }
+  if (ctx->split_work.cap < wb) CK(ctx->split_work.reserve(wb), "alloc split workspace");
+  return IMCOM_OK;
+}
+
+int imcom_gpu_batch_reserve(imcom_gpu_ctx* ctx, int precision_P, int precision_T) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(precision_P >= 0 && precision_P <= 2 && precision_T >= 0 && precision_T <= 2, IMCOM_EARG,
+       "batch_reserve: precision must be IMCOM_PREC_FP64, _FP32X3 or _BF16X6");
+  NEED(ctx->batch && !ctx->batch->tiles.empty(), IMCOM_ESTATE, "batch_reserve: no batch set");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  return batch_reserve_split(ctx, precision_P, precision_T);
+}
+
 int imcom_gpu_batch_solve(imcom_gpu_ctx* ctx, int precision_P, int precision_T, int learned,
                           float zsig, int use_graph) {
   if (!ctx) return IMCOM_EARG;
@@ -316,15 +339,9 @@ int imcom_gpu_batch_solve(imcom_gpu_ctx* ctx, int precision_P, int precision_T, 
   NEED(!learned || zsig > 0.0f, IMCOM_EARG, "batch_solve: zsig must be positive");
   CK(hipSetDevice(ctx->device), "hipSetDevice");
   BatchState* B = ctx->batch;
-  if (precision_P != IMCOM_PREC_FP64 || precision_T != IMCOM_PREC_FP64) {
-    size_t wb = 0;
-    for (BatchTile* t : B->tiles) {
-      const int n = (int)t->in.n, m = (int)t->in.m;
-      wb = std::max(wb, gemm_split_work_bytes(std::max(precision_P, precision_T), n, m, n));
-      if (precision_P) wb = std::max(wb, gemm_split_work_bytes(precision_P, n, m, n));
-      if (precision_T) wb = std::max(wb, gemm_split_work_bytes(precision_T, n, m, n));
-    }
-    if (ctx->split_work.cap < wb) CK(ctx->split_work.reserve(wb), "alloc split workspace");
+  {
+    int rc = batch_reserve_split(ctx, precision_P, precision_T);
+    if (rc) return rc;
   }
   const int key = (learned ? 1 + (int)(zsig * 1000.0f) : 0) * 16 + precision_P * 4 + precision_T;
   // re-capture when the controls change or when the split workspace moved
@@ -377,6 +394,14 @@ int imcom_gpu_batch_result(imcom_gpu_ctx* ctx, int tile, double* K_out, double* 
 }
 
 }  // extern "C"
+
+extern "C" int imcom_gpu_set_reduction_tail(imcom_gpu_ctx* ctx, int max_rows) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(max_rows == 0 || (max_rows >= 2 && max_rows <= kTailMaxRows), IMCOM_EARG,
+       "set_reduction_tail: max_rows must be 0 or in 2.." + std::to_string(kTailMaxRows));
+  ctx->tail_rows = max_rows;
+  return IMCOM_OK;
+}
 
 extern "C" int imcom_gpu_set_tuning(imcom_gpu_ctx* ctx, int nb, int nbb) {
   if (!ctx) return IMCOM_EARG;

@@ -53,6 +53,9 @@ using imcom_ctx_detail::kNBB;
 struct EigWork {
   DevBuf sW, sVW, sWV, sy, sycp, swpre, svtmp, sd, se, stau, spart, spart2, ss, sscal, sVb, sSb,
       sTb, sY, sY2, slam, sZ, stile, sdp, sAt, sKs;
+  // exchange space of the persistent reduction tail (sytrd_tail; single
+  // tiles only, reserved by eigen_A / tridiagonalize)
+  DevBuf stail;
   // splitk: workspace for the single-tile back-transform's split-K products
   // (the batched back-transform uses grouped GEMMs instead)
   hipError_t reserve(int64_t n, int nb, int nbb, double* A, double* Z, SyTile& t,
@@ -119,7 +122,7 @@ struct EigWork {
   void release() {
     DevBuf* b[] = {&sW, &sVW, &sWV, &sy, &sycp, &swpre, &svtmp, &sd, &se, &stau, &spart,
                    &spart2, &ss, &sscal, &sVb, &sSb, &sTb, &sY, &sY2, &slam, &sZ, &stile, &sdp,
-                   &sAt, &sKs};
+                   &sAt, &sKs, &stail};
     for (DevBuf* p : b) p->release();
   }
 };
@@ -146,6 +149,9 @@ struct imcom_gpu_ctx {
   int64_t n = 0, m = 0;
   double C_a = 0.0;
   int nb = kNB, nbb = kNBB;  // eigensolver tuning (imcom_gpu_set_tuning)
+  // largest trailing size the persistent kernel reduces (imcom_gpu_set_reduction_tail;
+  // 0: off)
+  int tail_rows = imcom::kTailMaxRows;
   std::vector<int32_t> h_exp;
   // tables
   int psfconst = 1, nexppsf = 1, n1 = 0, n2 = 0;

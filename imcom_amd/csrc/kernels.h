@@ -240,8 +240,24 @@ constexpr int kDotChunks = 16;
 // `sharers`: contexts whose reductions may run at once on the device; each
 // keeps 1/sharers of the matrix-vector product's cached share of the
 // last-level cache.
+// Single tile (ntiles == 1) with tail_rows >= 2 and tail_xch set: the
+// columns from the first panel boundary k0 with n - k0 <= tail_rows on are
+// reduced by the persistent kernel (sytrd_tail); *tail_S receives n - k0
+// (0 when it did not run).
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                     const std::vector<SyTile>& host, int nb, int sharers, Prof* prof);
+                     const std::vector<SyTile>& host, int nb, int sharers, Prof* prof,
+                     int tail_rows = 0, double* tail_xch = nullptr, int64_t* tail_S = nullptr);
+
+// The persistent tail of the reduction (sytrd_tail.hip): trailing sizes up to
+// kTailMaxRows, whose columns, spread over at most 256 workgroups, fit in
+// their LDS.  xch: tail_xch_doubles(S) doubles of device exchange space.
+constexpr int kTailMaxRows = 2048;
+size_t tail_xch_doubles(int S);
+void tail_shape(int S, int* G, int* cpw);
+hipError_t sytrd_tail(hipStream_t st, const SyTile& t, int64_t k0, bool from_A, double* xch);
+// The abort flag the last tail launch left in xch (synchronises the stream):
+// 0 completed, 1 a barrier wait ran out, 2 the barrier count was not reset.
+hipError_t sytrd_tail_status(hipStream_t st, int64_t S, const double* xch, unsigned* flag);
 hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nbb);
 

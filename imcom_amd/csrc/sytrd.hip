@@ -750,12 +750,22 @@ static hipError_t sy_panel_columns(hipStream_t st, const SyTile* d_tiles, int nt
 }
 
 hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
-                     const std::vector<SyTile>& host, int nb, int sharers, Prof* prof) {
+                     const std::vector<SyTile>& host, int nb, int sharers, Prof* prof,
+                     int tail_rows, double* tail_xch, int64_t* tail_S) {
   size_t tok = 0;
   const int kkeep = symv_keep_blocks(ntiles, sharers);
   const int nbt = ceil_div(nmax, SB);
-  hipLaunchKernelGGL(sy_tile_init, dim3(nbt, nbt, ntiles), dim3(256), 0, st, d_tiles);
-  for (int64_t k = 0; k < nmax - 1; k += nb) {
+  // the first panel boundary from which the persistent kernel takes over
+  int64_t k_tail = nmax;
+  if (ntiles == 1 && tail_xch && tail_rows >= 2 && nmax >= 2) {
+    const int rows = std::min(tail_rows, kTailMaxRows);
+    k_tail = 0;
+    while (nmax - k_tail > rows) k_tail += nb;
+  }
+  if (tail_S) *tail_S = 0;
+  if (k_tail > 0)  // the block layout the blocked columns read (a whole tail reads A)
+    hipLaunchKernelGGL(sy_tile_init, dim3(nbt, nbt, ntiles), dim3(256), 0, st, d_tiles);
+  for (int64_t k = 0; k < std::min(nmax - 1, k_tail); k += nb) {
     const int pb = (int)std::min<int64_t>(nb, nmax - 1 - k);
     IMCOM_HIP_TRY(sy_panel_columns(st, d_tiles, ntiles, nmax, k, pb, kkeep, prof));
     // trailing update A22 -= [V W] [W V]^T over rows/cols [k+nb, n)
@@ -774,6 +784,10 @@ hipError_t sytrd_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t 
       }
       if (prof) IMCOM_HIP_TRY(prof->end(PS_SYR2K, tok));
     }
+  }
+  if (k_tail < nmax - 1) {
+    IMCOM_HIP_TRY(sytrd_tail(st, host[0], k_tail, k_tail == 0, tail_xch));
+    if (tail_S) *tail_S = nmax - k_tail;
   }
   hipLaunchKernelGGL(sy_last_diag, dim3(1, 1, ntiles), dim3(1), 0, st, d_tiles);
   return hipGetLastError();

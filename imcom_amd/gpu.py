@@ -38,8 +38,9 @@ EXPORTS = [
     "imcom_gpu_stage_times", "imcom_gpu_mlp_nparams", "imcom_gpu_mlp_set_params",
     "imcom_gpu_mlp_get_params", "imcom_gpu_kappa_features", "imcom_gpu_mlp_infer",
     "imcom_gpu_mlp_grad", "imcom_gpu_mlp_train", "imcom_gpu_bisect_kappa_learned",
-    "imcom_gpu_batch_set", "imcom_gpu_batch_solve", "imcom_gpu_batch_result",
-    "imcom_gpu_set_tuning", "imcom_gpu_fetch_async", "imcom_gpu_fetch_wait",
+    "imcom_gpu_batch_set", "imcom_gpu_batch_solve", "imcom_gpu_batch_reserve",
+    "imcom_gpu_batch_result",
+    "imcom_gpu_set_tuning", "imcom_gpu_set_reduction_tail", "imcom_gpu_fetch_async", "imcom_gpu_fetch_wait",
     "imcom_gpu_host_alloc", "imcom_gpu_host_free", "imcom_gpu_bisect_kappa_learned_noalloc",
 ]
 
@@ -131,8 +132,10 @@ def load_library(path: str = LIB_PATH):
                                                   ctypes.c_float, _dp, _dp, _ip, _ip]),
         "imcom_gpu_batch_set": (_int, [_vp, _int, ctypes.POINTER(TileDesc)]),
         "imcom_gpu_batch_solve": (_int, [_vp, _int, _int, _int, ctypes.c_float, _int]),
+        "imcom_gpu_batch_reserve": (_int, [_vp, _int, _int]),
         "imcom_gpu_batch_result": (_int, [_vp, _int, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
         "imcom_gpu_set_tuning": (_int, [_vp, _int, _int]),
+        "imcom_gpu_set_reduction_tail": (_int, [_vp, _int]),
         "imcom_gpu_assemble_inputs": (_int, [_vp, _int, _int, _int, _dp, _dbl, _dp, _dp, _dp,
                                               ctypes.POINTER(_i64), _dp, _dp, _ip, _dp]),
         "imcom_gpu_rotate_image": (_int, [_vp, _int, _int, _dp, _dbl, _int, _int, _dp]),
@@ -501,6 +504,11 @@ class ImcomGPU:
     def set_tuning(self, nb=32, nbb=128):
         self._ck(self.lib.imcom_gpu_set_tuning(self.h, nb, nbb), "set_tuning")
 
+    def set_reduction_tail(self, max_rows=2048):
+        """Largest trailing size whose columns the persistent kernel reduces
+        (the library's default 2048; 0: every column on the blocked kernels)."""
+        self._ck(self.lib.imcom_gpu_set_reduction_tail(self.h, max_rows), "set_reduction_tail")
+
     # ---- batched tiles ----------------------------------------------------
     def batch_set(self, tiles):
         """Upload many tiles (each a tile.Tile) for one batched solve."""
@@ -529,6 +537,11 @@ class ImcomGPU:
         self._ck(self.lib.imcom_gpu_batch_solve(self.h, precision_P, precision_T,
                                                 1 if learned else 0, zsig, 1 if use_graph else 0),
                  "batch_solve")
+
+    def batch_reserve(self, precision_P=IMCOM_PREC_FP64, precision_T=IMCOM_PREC_FP64):
+        """Allocate batch_solve's workspace for these precisions now."""
+        self._ck(self.lib.imcom_gpu_batch_reserve(self.h, precision_P, precision_T),
+                 "batch_reserve")
 
     def batch_result(self, i, want_T=False):
         t = self.batch_tiles[i]

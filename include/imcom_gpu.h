@@ -274,10 +274,22 @@ typedef struct imcom_gpu_tile {
  * reduction panel width nb (16, 32, 64) and the back-transform block nbb
  * (64, 128 or 256); applies to later eigen_A calls and batch_set. */
 int imcom_gpu_set_tuning(imcom_gpu_ctx* ctx, int nb, int nbb);
+/* Single-tile eigen_A and tridiagonalize reduce the last columns, from the
+ * first panel boundary where at most max_rows remain (2..2048; default 2048),
+ * in one persistent launch that keeps the trailing matrix in on-chip memory
+ * and synchronises its workgroups per column; 0 reduces every column with
+ * the blocked kernels.  Part of the DSYTRD stage of DSYEVR
+ * (imcom_matrix.f90:642); no reference counterpart of its own. */
+int imcom_gpu_set_reduction_tail(imcom_gpu_ctx* ctx, int max_rows);
 
 int imcom_gpu_batch_set(imcom_gpu_ctx* ctx, int ntiles, const imcom_gpu_tile* tiles);
 int imcom_gpu_batch_solve(imcom_gpu_ctx* ctx, int precision_P, int precision_T, int learned,
                           float zsig, int use_graph);
+/* Allocates the workspace imcom_gpu_batch_solve needs for these precisions
+ * (batch_solve allocates it itself otherwise): a caller that runs batched
+ * solves of several contexts from several threads calls it first, so that
+ * no device allocation runs while another thread captures a solve graph. */
+int imcom_gpu_batch_reserve(imcom_gpu_ctx* ctx, int precision_P, int precision_T);
 int imcom_gpu_batch_result(imcom_gpu_ctx* ctx, int tile, double* K_out, double* US_out,
                            double* SU_out, double* H_out, int32_t* evals_out, int32_t* hits_out,
                            double* T_out);
