@@ -24,10 +24,12 @@ Two paths are timed: "plain" (full-range bisection, FP64 GEMMs) and "fast"
 P stays FP64 because U ~ 1e-6 is a difference of O(1) terms).  `value` is
 the fast path's output pixels per second; the plain path's number is in
 config.  After the timed steps every pixel's U is checked against U_max.
-With --batch the large run also measures the 64-tile mosaic (config.batch),
-where the batched fast path is compared with the plain single-tile path; it
-is off by default while an intermittent stall of the three-stream fast path
-(two silent runs of its test) is unexplained.
+The default (large) run also measures the 64-tile mosaic (config.batch),
+where the batched fast path (one batched solve of all 64 tiles) is compared
+with the plain single-tile path.  --batch-streams 3 splits the tiles over
+three concurrent solves from three host threads (faster, but that form
+stalled silently in two test runs with no cause found, so it is not the
+default).
 
 The JSON line carries a roofline entry for the dominant kernel (sy_symv, the
 matrix-vector kernel of the tridiagonal reduction; HBM-bound; HIP events
@@ -128,15 +130,17 @@ def stage_rooflines(n: int, m: int, st: dict, t_bf16_ms: float) -> list:
     return [r for r in rows if r is not None]
 
 
-def batch_block(steps: int, warmup: int, weights, skip_plain: bool) -> dict:
-    """The 64-tile mosaic (synth.batch): the fast path as three concurrent
-    batched solves replayed from HIP graphs (learned bracket, BF16x6 T), the
-    plain path as the single-tile FP64 path tile after tile, each tile's
+def batch_block(steps: int, warmup: int, weights, skip_plain: bool, nstreams: int = 1) -> dict:
+    """The 64-tile mosaic (synth.batch): the fast path as batched solves
+    replayed from HIP graphs (learned bracket, BF16x6 T): one solve of all the
+    tiles on one context and stream (nstreams = 1), or the tiles dealt over
+    nstreams concurrent solves from as many host threads (imcom_amd.batch);
+    the plain path as the single-tile FP64 path tile after tile, each tile's
     inputs resident in its own context before the timed region.  Every
     pixel's U is checked against U_max after the timed steps."""
     import torch
     from imcom_amd import autotune, synth
-    from imcom_amd.batch import DEFAULT_STREAMS, StreamedBatch
+    from imcom_amd.batch import StreamedBatch
     from imcom_amd.gpu import IMCOM_PREC_BF16X6, ImcomGPU
     tiles = synth.batch()
     m_total = sum(t.m for t in tiles)
@@ -153,21 +157,36 @@ def batch_block(steps: int, warmup: int, weights, skip_plain: bool) -> dict:
         torch.cuda.synchronize()
         return (time.perf_counter() - t1) / k, int(np.sum(~(U < Umax))), float(ev.mean())
 
-    # fast: opened before any other context (an idle context beside the three
-    # solves slows them)
-    with StreamedBatch(tiles, DEFAULT_STREAMS, weights=weights) as sb:
+    # fast: opened before any other context (an idle context beside
+    # concurrent solves slows them)
+    if nstreams > 1:
+        sb = StreamedBatch(tiles, nstreams, weights=weights)
+        solve, result, close = sb.solve, sb.result, sb.close
+    else:
+        gb = ImcomGPU(0)
+        autotune.apply(gb, max(t.n for t in tiles))
+        if weights is not None:
+            gb.mlp_set_params(weights)
+        gb.batch_set(tiles)
+        solve, result, close = gb.batch_solve, gb.batch_result, gb.close
+    try:
         def fast():
-            sb.solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
-            res = [sb.result(i) for i in range(len(tiles))]
+            solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
+            res = [result(i) for i in range(len(tiles))]
             return (np.concatenate([r["U"] for r in res]),
                     np.concatenate([r["evals"] for r in res]))
         f_sec, f_bad, f_ev = timed(fast, steps, warmup)
+    finally:
+        close()
     log(f"batch fast: {f_sec * 1e3:.1f} ms/step, {m_total / f_sec:.1f} px/s, pixels with "
         f"U >= U_max: {f_bad}, mean U evaluations {f_ev:.2f}")
+    how = ("one batched solve of all the tiles (one context and stream, HIP-graph replay)"
+           if nstreams == 1 else
+           f"{nstreams} concurrent batched solves (one stream and host thread each, HIP-graph "
+           "replay)")
     out = {"workload": (f"{len(tiles)} tiles, n in [{min(t.n for t in tiles)}, "
                         f"{max(t.n for t in tiles)}], sum m = {m_total}, U_max = 1e-6"),
-           "fast_path": ("three concurrent batched solves (one stream each, HIP-graph replay), "
-                         + ("learned kappa bracket + " if learned else "")
+           "fast_path": (how + ", " + ("learned kappa bracket + " if learned else "")
                          + "BF16x6 split GEMM for T; P and the bisection FP64"),
            "fast_px_per_s": round(m_total / f_sec, 2), "fast_ms_per_step": round(f_sec * 1e3, 2),
            "fast_pixels_U_ge_Umax": f_bad, "fast_mean_U_evaluations": round(f_ev, 2),
@@ -268,12 +287,13 @@ def main():
     ap.add_argument("--workload", default="large", choices=["large", "small", "batch"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true")
-    ap.add_argument("--batch", action="store_true",
-                    help="large workload: add the 64-tile mosaic block (config.batch); off by "
-                         "default while an intermittent stall of the three-stream fast path is "
-                         "unexplained")
     ap.add_argument("--no-batch", action="store_true",
-                    help="large workload: leave out the 64-tile mosaic block (the default)")
+                    help="large workload: leave out the 64-tile mosaic block (config.batch)")
+    ap.add_argument("--batch-streams", type=int, default=1,
+                    help="the mosaic's fast path as this many concurrent batched solves (one "
+                         "host thread each); 1 (default): one batched solve of all the tiles.  "
+                         "The three-solve form is faster but stalled silently in two test runs "
+                         "with no cause found")
     ap.add_argument("--vendor-eig", action="store_true",
                     help="also time rocSOLVER's dsyevd on the same A (single-tile workloads)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -339,13 +359,15 @@ def main():
             return
 
     if args.workload == "batch":
-        # fast: the size-sorted tiles cut into three runs of about equal n^3
-        # work, one batched solve each (lockstep launches replayed from HIP
-        # graphs) on its own stream, solved concurrently (imcom_amd.batch); plain: the single-tile FP64 path tile after tile,
-        # each tile's inputs resident in its own context before the timed region
-        # (the plain path's contexts are released before the streamed solves
-        # are set up: streams beyond the process's hardware queues share them)
-        from imcom_amd.batch import DEFAULT_STREAMS, StreamedBatch
+        # fast: one batched solve of all the tiles (lockstep launches replayed
+        # from a HIP graph), or with --batch-streams N the size-sorted tiles
+        # cut into N runs of about equal n^3 work, one batched solve each on
+        # its own stream, solved concurrently (imcom_amd.batch); plain: the
+        # single-tile FP64 path tile after tile, each tile's inputs resident in
+        # its own context before the timed region (the plain path's contexts
+        # are released before the fast path's are set up: streams beyond the
+        # process's hardware queues share them)
+        from imcom_amd.batch import StreamedBatch
         sbs = []
         singles = []
         if not args.skip_plain:
@@ -371,7 +393,15 @@ def main():
             for gs in singles:
                 gs.close()
             singles.clear()
-            sbs.append(StreamedBatch(tiles, DEFAULT_STREAMS, weights=weights))
+            if args.batch_streams > 1:
+                sbs.append(StreamedBatch(tiles, args.batch_streams, weights=weights))
+            else:
+                gb = ImcomGPU(0)
+                autotune.apply(gb, max(t.n for t in tiles))
+                if weights is not None:
+                    gb.mlp_set_params(weights)
+                gb.batch_set(tiles)
+                sbs.append(gb)
 
         def step(fast):
             if not fast:
@@ -379,8 +409,12 @@ def main():
                 return (np.concatenate([r[0] for r in res]), np.concatenate([r[1] for r in res]),
                         Umax_all)
             sb = sbs[0]
-            sb.solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
-            res = [sb.result(i) for i in range(len(tiles))]
+            if args.batch_streams > 1:
+                sb.solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
+                res = [sb.result(i) for i in range(len(tiles))]
+            else:
+                sb.batch_solve(precision_T=IMCOM_PREC_BF16X6, learned=learned, use_graph=True)
+                res = [sb.batch_result(i) for i in range(len(tiles))]
             U = np.concatenate([r["U"] for r in res])
             ev = np.concatenate([r["evals"] for r in res])
             return U, ev, Umax_all
@@ -579,8 +613,11 @@ def main():
         log(f"cpu baseline: {tot:.0f} s for {m_sampled} px -> {cpu['value']} px/s ({cores} threads)")
 
     cfg = {"workload": describe(args.workload, tiles),
-           "path": (("batched tiles (three concurrent batched solves, one stream each, each "
-                     "replayed from HIP graphs) + " if args.workload == "batch" else "")
+           "path": ((("batched tiles (one batched solve, replayed from a HIP graph) + "
+                      if args.batch_streams <= 1 else
+                      f"batched tiles ({args.batch_streams} concurrent batched solves, one stream "
+                      "each, each replayed from HIP graphs) + ")
+                     if args.workload == "batch" else "")
                     + ("learned kappa bracket + BF16x6 split GEMM for T" if learned
                        else "BF16x6 split GEMM for T (no trained bracket weights found)")
                     + "; P and the bisection FP64"),
@@ -603,10 +640,10 @@ def main():
     if g is not None:
         g.close()
         g = None
-    if args.workload == "large" and args.batch and not args.no_batch:
+    if args.workload == "large" and not args.no_batch:
         # the 64-tile mosaic beside the large tile (fewer steps: a plain
         # step solves the 64 tiles one after another, about 4 s)
-        cfg["batch"] = batch_block(2, 1, weights, args.skip_plain)
+        cfg["batch"] = batch_block(2, 1, weights, args.skip_plain, args.batch_streams)
     out = {
         "metric": METRIC, "value": round(m_total / sec, 2), "unit": "output pixels/sec",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(sec * 1e3, 2),

@@ -1,8 +1,9 @@
 """Parity on the 64-tile mosaic (2-6 exposures of 16-32 px squares, n from
 1024 to 6144, m from 256 to 2304, PSF sigma in [1.0, 1.6], noise sigma
 log-uniform in [1e-4, 1e-2]), solved as three concurrent batched solves
-(imcom_amd.batch.StreamedBatch) replayed from HIP graphs.  These launches are
-large enough for the grouped 128x128 FP64 products.
+(imcom_amd.batch.StreamedBatch) replayed from HIP graphs, and (fast path) as
+one batched solve of all 64 tiles.  These launches are large enough for the
+grouped 128x128 FP64 products.
 
 Both paths the benchmark times are checked:
   plain  FP64 products, full-range bisection
@@ -76,6 +77,27 @@ def _solve_twice(tiles, picked, **kw):
     return first
 
 
+def _solve_twice_one(tiles, picked, **kw):
+    """The same as _solve_twice with all the tiles in one batched solve (one
+    context, one stream, one host thread)."""
+    from imcom_amd import autotune
+    from imcom_amd.gpu import ImcomGPU
+    weights = kw.pop("weights", None)
+    with ImcomGPU(0) as g:
+        autotune.apply(g, max(t.n for t in tiles))
+        if weights is not None:
+            g.mlp_set_params(weights)
+        g.batch_set(tiles)
+        g.batch_solve(use_graph=True, **kw)
+        first = [g.batch_result(i, want_T=i in picked) for i in range(len(tiles))]
+        g.batch_solve(use_graph=True, **kw)
+        for i in range(len(tiles)):
+            again = g.batch_result(i)
+            for k in ("K", "U", "S", "H", "evals"):
+                assert np.array_equal(again[k], first[i][k]), (i, k)
+    return first
+
+
 def _check_maps(t, r, out, idx, K, extra):
     """T, S and H at pixels idx against the oracle's T at kappa K."""
     Tr = O.build_T(r["Q"], r["P"][:, idx], r["L"], K)
@@ -134,4 +156,30 @@ def test_mosaic_fast_path(mosaic):
         idx = np.sort(rng.choice(t.m, min(48, t.m), replace=False))
         rel = _check_maps(t, r, out, idx, out["K"][idx], 2e-6)
         print(f"[mosaic fast] tile {i} n={t.n} m={t.m}: max rel dT (BF16x6, own kappa) {rel:.2e}",
+              flush=True)
+
+
+def test_mosaic_fast_path_one_solve(mosaic):
+    """The fast path with the 64 tiles in one batched solve (the benchmark's
+    default form): the checks of test_mosaic_fast_path."""
+    tiles, picked, refs = mosaic
+    weights = kappa_net.load_weights()
+    assert weights is not None, "trained bracket weights missing"
+    print("[case] mosaic fast path, one batched solve of 64 tiles", flush=True)
+    outs = _solve_twice_one(tiles, picked, precision_T=IMCOM_PREC_BF16X6, learned=True,
+                            weights=weights)
+    bad = sum(int(np.sum(~(o["U"] < t.US_max))) for o, t in zip(outs, tiles))
+    assert bad == 0, f"{bad} pixels with U >= U_max"
+    hits = np.concatenate([o["hits"] for o in outs])
+    ev = np.concatenate([o["evals"] for o in outs])
+    print(f"[mosaic fast, one solve] brackets held in {hits.mean():.3f} of pixels, mean U "
+          f"evaluations {ev.mean():.2f}", flush=True)
+    rng = np.random.default_rng(66)
+    for i in picked:
+        t, r, out = tiles[i], refs[i], outs[i]
+        reached = t.US_max - r["U"] <= t.US_tol
+        assert np.all((t.US_max - out["U"] <= t.US_tol)[reached])
+        idx = np.sort(rng.choice(t.m, min(48, t.m), replace=False))
+        rel = _check_maps(t, r, out, idx, out["K"][idx], 2e-6)
+        print(f"[mosaic fast, one solve] tile {i} n={t.n} m={t.m}: max rel dT {rel:.2e}",
               flush=True)
