@@ -40,7 +40,8 @@ namespace imcom {
 
 namespace {
 
-constexpr int TT = 256;                  // threads per workgroup
+constexpr int TT = 512;                  // threads per workgroup
+constexpr int kWaves = TT / 64;
 constexpr int kRPT = (kTailMaxRows + TT - 1) / TT;  // rows per thread (register arrays)
 constexpr int kMaxCPW = 16;              // columns per workgroup at most
 // ... and at least: the LDS holds kTailLdsDoubles / kTailMaxRows columns of
@@ -109,14 +110,17 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
   return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 
-// Fixed-order workgroup sum (wavefront sums, then the four wavefronts in
+// Fixed-order workgroup sum (wavefront sums, then the wavefronts in
 // order); every thread gets the result.
 __device__ __forceinline__ double tail_sum(double v, double* red) {
   v = wave_sum_dpp(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
-  return (red[0] + red[1]) + (red[2] + red[3]);
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) t += red[w];
+  return t;
 }
 
 // Grid barrier: arrival count sync[0] (monotonic within a launch), abort
@@ -222,9 +226,9 @@ __global__ __launch_bounds__(TT) void sy_tail_kernel(SyTile t, int64_t k0, int S
                                                      unsigned spin_max) {
   constexpr int cpw = CPW;
   extern __shared__ double L[];                 // cpw x S, column-major by slot
-  __shared__ double red[8];
+  __shared__ double red[kWaves];
   __shared__ double bc[2];
-  __shared__ double redp[CPW][TT / 64];         // wavefront partials of the owned p_j
+  __shared__ double redp[CPW][kWaves];          // wavefront partials of the owned p_j
   __shared__ double vjs[CPW], wjs[CPW];         // v_j, w_j at the owned columns
   __shared__ double vw1[2];                     // v and w at row c+1
   __shared__ int ok_sh;
@@ -306,17 +310,24 @@ __global__ __launch_bounds__(TT) void sy_tail_kernel(SyTile t, int64_t k0, int S
       if ((tid & 63) == 0) redp[s][tid >> 6] = ws;
     }
     __syncthreads();
-    double q = 0.0;
-    if (tid < cpw) {
-      const int jl = g + G * tid;
-      if (jl > c && jl < S) {
-        const double p = tau * ((redp[tid][0] + redp[tid][1]) + (redp[tid][2] + redp[tid][3]));
-        st_coh(&pc[jl], p);
-        q = p * vjs[tid];
+    // p_j by lanes 0..cpw-1 of wavefront 0, and its share of p^T v by that
+    // wavefront's sum (the other lanes add zeros)
+    if (tid < 64) {
+      double q = 0.0;
+      if (tid < cpw) {
+        const int jl = g + G * tid;
+        if (jl > c && jl < S) {
+          double tot = 0.0;
+#pragma unroll
+          for (int w = 0; w < kWaves; ++w) tot += redp[tid][w];
+          const double p = tau * tot;
+          st_coh(&pc[jl], p);
+          q = p * vjs[tid];
+        }
       }
+      const double qs = wave_sum_dpp(q);
+      if (tid == 0) st_coh(&qc[g], qs);
     }
-    const double qs = tail_sum(q, red);
-    if (tid == 0) st_coh(&qc[g], qs);
     if ((c + 1) % G == g) {  // the owner of column c+1 publishes it
       const int s1 = (c + 1) / G;
 #pragma unroll
