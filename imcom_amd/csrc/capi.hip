@@ -91,6 +91,7 @@ void imcom_gpu_finalize(imcom_gpu_ctx* ctx) {
   for (DevBuf* b : bufs) b->release();
   ctx->ew.release();
   ctx->tw.release();
+  ctx->tsw.release();
   imcom_batch_free(ctx);
   dc_destroy(ctx->dc);
   ctx->prof.destroy();
@@ -444,6 +445,102 @@ static hipError_t reserve_eigen(imcom_gpu_ctx* ctx, int64_t n, SyTile& t) {
   return ctx->ew.reserve(n, ctx->nb, ctx->nbb, ctx->A.as<double>(), ctx->Q.as<double>(), t);
 }
 
+static bool use_twostage(const imcom_gpu_ctx* ctx, int64_t n) {
+  return ctx->stages == 2 && n >= ctx->twostage_min && n >= 3;
+}
+
+// Two-stage reduction of A (overwritten): d, e into the eigensolver
+// workspace, the bulge-chasing reflectors into Q's buffer, the panels' V^T
+// in A's upper triangle and T in the two-stage workspace.
+static int twostage_reduce(imcom_gpu_ctx* ctx, int64_t n) {
+  TwoStageBufs w;
+  CK(ctx->tsw.reserve(n, w), "alloc two-stage workspace");
+  CK(ctx->ew.sd.reserve((size_t)n * 8), "alloc d");
+  CK(ctx->ew.se.reserve((size_t)n * 8), "alloc e");
+  CK(ctx->Q.reserve((size_t)n * n * 8), "alloc Q");
+  {
+    StageTimer tm(ctx, PS_SYTRD);
+    CK(band_reduce(ctx->stream, ctx->A.as<double>(), n, w), "band reduction");
+    CK(bulge_chase(ctx->stream, ctx->A.as<double>(), n, ctx->Q.as<double>(),
+                   ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(), w.prog),
+       "bulge chasing");
+  }
+  unsigned flags[2] = {0, 0};
+  CK(twostage_status(ctx->stream, w, n, flags), "two-stage status");
+  NEED(flags[0] == 0, IMCOM_EHIP,
+       "band reduction: a panel's grid barrier timed out (its workgroups were not all resident)");
+  NEED(flags[1] == 0, IMCOM_EHIP,
+       "bulge chasing: a wait for the previous sweep timed out (its workgroups were not all "
+       "resident)");
+  return IMCOM_OK;
+}
+
+// Eigendecomposition through the two-stage reduction: Q = Q1 Q2 Z with Z
+// the tridiagonal matrix's eigenvectors, descending order as eigen_A's.
+static int twostage_eigen(imcom_gpu_ctx* ctx, int64_t n) {
+  {
+    int rc = twostage_reduce(ctx, n);
+    if (rc) return rc;
+  }
+  TwoStageBufs w;
+  CK(ctx->tsw.reserve(n, w), "alloc two-stage workspace");
+  CK(ctx->tsw.reserve_q2(n, ctx->stream), "alloc bulge back-transform workspace");
+  CK(ctx->ew.slam.reserve((size_t)n * 8), "alloc lam");
+  CK(ctx->ew.sZ.reserve((size_t)n * n * 8), "alloc Z");
+  CK(ctx->L.reserve((size_t)n * 8), "alloc L");
+  {
+    StageTimer tm(ctx, PS_STEDC);
+    CK(dc_solve(ctx->stream, ctx->dc, n, ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(),
+                ctx->ew.slam.as<double>(), ctx->ew.sZ.as<double>()),
+       "divide-and-conquer");
+  }
+  {
+    StageTimer tm(ctx, PS_ORMTR);
+    const int nS = q2_groups(n);
+    const int maxk = nS > 0 ? ctx->tsw.h_blk_off[1] - ctx->tsw.h_blk_off[0] : 0;
+    CK(apply_q2(ctx->stream, ctx->Q.as<double>(), n, ctx->tsw.blk_off.as<int>(), nS, maxk,
+                ctx->tsw.Tq.as<double>(), ctx->ew.sZ.as<double>()),
+       "bulge back-transform");
+    CK(launch_reverse_cols(ctx->stream, n, ctx->ew.sZ.as<double>(), ctx->Q.as<double>(),
+                           ctx->ew.slam.as<double>(), ctx->L.as<double>()),
+       "reverse");
+    CK(apply_q1(ctx->stream, ctx->A.as<double>(), n, w, ctx->Q.as<double>(), n),
+       "band back-transform");
+  }
+  return IMCOM_OK;
+}
+
+static int eigen_finish(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* condition);
+
+int imcom_gpu_set_reduction_stages(imcom_gpu_ctx* ctx, int stages, int64_t min_n) {
+  if (!ctx) return IMCOM_EARG;
+  NEED((stages == 1 || stages == 2) && min_n >= 3, IMCOM_EARG,
+       "set_reduction_stages: stages must be 1 or 2 and min_n >= 3");
+  ctx->stages = stages;
+  ctx->twostage_min = min_n;
+  return IMCOM_OK;
+}
+
+int imcom_gpu_band_reduce(imcom_gpu_ctx* ctx, double* A_out) {
+  if (!ctx) return IMCOM_EARG;
+  NEED(ctx->have_A, IMCOM_ESTATE, "band_reduce: A not built (lookup_A or set_A first)");
+  NEED(ctx->n >= 3, IMCOM_EARG, "band_reduce: n < 3");
+  CK(hipSetDevice(ctx->device), "hipSetDevice");
+  CK(fetch_guard(ctx, 1u << IMCOM_FETCH_A), "band_reduce: wait for fetches");
+  const int64_t n = ctx->n;
+  TwoStageBufs w;
+  CK(ctx->tsw.reserve(n, w), "alloc two-stage workspace");
+  CK(band_reduce(ctx->stream, ctx->A.as<double>(), n, w), "band reduction");
+  unsigned flags[2] = {0, 0};
+  CK(twostage_status(ctx->stream, w, 2, flags), "two-stage status");
+  NEED(flags[0] == 0, IMCOM_EHIP,
+       "band reduction: a panel's grid barrier timed out (its workgroups were not all resident)");
+  ctx->have_A = false;
+  CK(download(ctx, A_out, ctx->A, (size_t)n * n * 8), "download A");
+  CK(hipStreamSynchronize(ctx->stream), "sync");
+  return IMCOM_OK;
+}
+
 int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* condition) {
   if (!ctx) return IMCOM_EARG;
   NEED(ctx->have_A, IMCOM_ESTATE, "eigen_A: A not built (lookup_A or set_A first)");
@@ -451,6 +548,12 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
   CK(fetch_guard(ctx, (1u << IMCOM_FETCH_A) | (1u << IMCOM_FETCH_Q) | (1u << IMCOM_FETCH_L)),
      "eigen_A: wait for fetches");
   const int64_t n = ctx->n;
+  if (use_twostage(ctx, n)) {
+    int rc = twostage_eigen(ctx, n);
+    ctx->have_A = false;
+    if (rc) return rc;
+    return eigen_finish(ctx, Q_out, L_out, condition);
+  }
   SyTile t;
   CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
   CK(hipMemcpyAsync(ctx->ew.stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),
@@ -486,6 +589,12 @@ int imcom_gpu_eigen_A(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* 
     int rc = check_tail(ctx, tail_S);
     if (rc) return rc;
   }
+  return eigen_finish(ctx, Q_out, L_out, condition);
+}
+
+// Outputs of eigen_A once Q and L are formed.
+static int eigen_finish(imcom_gpu_ctx* ctx, double* Q_out, double* L_out, double* condition) {
+  const int64_t n = ctx->n;
   ctx->have_QL = true;
   ctx->have_P = ctx->have_K = ctx->have_T = false;
   CK(download(ctx, Q_out, ctx->Q, (size_t)n * n * 8), "download Q");
@@ -509,6 +618,15 @@ int imcom_gpu_tridiagonalize(imcom_gpu_ctx* ctx, double* d_out, double* e_out) {
   CK(hipSetDevice(ctx->device), "hipSetDevice");
   CK(fetch_guard(ctx, 1u << IMCOM_FETCH_A), "tridiagonalize: wait for fetches");
   const int64_t n = ctx->n;
+  if (use_twostage(ctx, n)) {
+    int rc = twostage_reduce(ctx, n);
+    ctx->have_A = false;
+    if (rc) return rc;
+    CK(download(ctx, d_out, ctx->ew.sd, (size_t)n * 8), "download d");
+    CK(download(ctx, e_out, ctx->ew.se, (size_t)(n - 1) * 8), "download e");
+    CK(hipStreamSynchronize(ctx->stream), "sync");
+    return IMCOM_OK;
+  }
   SyTile t;
   CK(reserve_eigen(ctx, n, t), "alloc eigensolver workspace");
   CK(hipMemcpyAsync(ctx->ew.stile.p, &t, sizeof(SyTile), hipMemcpyHostToDevice, ctx->stream),

@@ -127,6 +127,62 @@ struct EigWork {
   }
 };
 
+// Workspace of the two-stage reduction (sytrd2.hip).
+struct TwoStageWork {
+  DevBuf U, X, Mt, Mm, T1, xch, kwork, Y1, Y2, sync, prog;
+  // the bulge chase's back-transform: block offsets per sweep group (host
+  // and device, for n_blk) and each block's triangular factor
+  DevBuf blk_off, Tq;
+  std::vector<int> h_blk_off;
+  int64_t n_blk = -1;
+  hipError_t reserve(int64_t n, imcom::TwoStageBufs& w) {
+    const size_t d = 8, b = imcom::kBand;
+    IMCOM_HIP_TRY(U.reserve((size_t)n * 3 * b * d));
+    IMCOM_HIP_TRY(X.reserve((size_t)n * b * d));
+    IMCOM_HIP_TRY(Mt.reserve(b * b * d));
+    IMCOM_HIP_TRY(Mm.reserve(b * b * d));
+    IMCOM_HIP_TRY(T1.reserve((size_t)std::max(1, imcom::band_panels(n)) * b * b * d));
+    IMCOM_HIP_TRY(xch.reserve(imcom::band_xch_doubles(n) * d));
+    IMCOM_HIP_TRY(kwork.reserve(imcom::band_kwork_doubles(n) * d));
+    IMCOM_HIP_TRY(Y1.reserve((size_t)n * b * d));
+    IMCOM_HIP_TRY(Y2.reserve((size_t)n * b * d));
+    IMCOM_HIP_TRY(sync.reserve(2 * 4));
+    IMCOM_HIP_TRY(prog.reserve((size_t)(n + 1) * 4));
+    w.U = U.as<double>();
+    w.X = X.as<double>();
+    w.Mt = Mt.as<double>();
+    w.Mm = Mm.as<double>();
+    w.T1 = T1.as<double>();
+    w.xch = xch.as<double>();
+    w.kwork = kwork.as<double>();
+    w.Y1 = Y1.as<double>();
+    w.Y2 = Y2.as<double>();
+    w.sync = sync.as<unsigned>();
+    w.prog = prog.as<unsigned>();
+    return hipSuccess;
+  }
+  // block offsets and T factors of the bulge back-transform (uploaded on
+  // the stream; the host copy stays alive)
+  hipError_t reserve_q2(int64_t n, hipStream_t st) {
+    if (n_blk != n) {
+      imcom::q2_block_offsets(n, h_blk_off);
+      IMCOM_HIP_TRY(blk_off.reserve(h_blk_off.size() * sizeof(int)));
+      IMCOM_HIP_TRY(hipMemcpyAsync(blk_off.p, h_blk_off.data(), h_blk_off.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, st));
+      IMCOM_HIP_TRY(hipStreamSynchronize(st));
+      n_blk = n;
+    }
+    const size_t nblocks = h_blk_off.empty() ? 0 : (size_t)h_blk_off.back();
+    return Tq.reserve(std::max<size_t>(nblocks, 1) * imcom::kQ2Group * imcom::kQ2Group * 8);
+  }
+  void release() {
+    DevBuf* b[] = {&U, &X, &Mt, &Mm, &T1, &xch, &kwork, &Y1, &Y2, &sync, &prog, &blk_off, &Tq};
+    for (DevBuf* p : b) p->release();
+    h_blk_off.clear();
+    n_blk = -1;
+  }
+};
+
 // Workspace of the device table set-up (tables.hip): the cached DFT
 // matrices (E_f for the PSF spectra, E_t for the padded table transform,
 // with the (N, n, f0) each was built for), the product spectrum, the
@@ -152,6 +208,11 @@ struct imcom_gpu_ctx {
   // largest trailing size the persistent kernel reduces (imcom_gpu_set_reduction_tail;
   // 0: off)
   int tail_rows = imcom::kTailMaxRows;
+  // 1: one-stage reduction; 2: two-stage (dense -> band -> tridiagonal) for
+  // n >= twostage_min (imcom_gpu_set_reduction_stages)
+  int stages = 1;
+  int64_t twostage_min = 4096;
+  TwoStageWork tsw;
   std::vector<int32_t> h_exp;
   // tables
   int psfconst = 1, nexppsf = 1, n1 = 0, n2 = 0;

@@ -258,6 +258,49 @@ hipError_t sytrd_tail(hipStream_t st, const SyTile& t, int64_t k0, bool from_A, 
 // The abort flag the last tail launch left in xch (synchronises the stream):
 // 0 completed, 1 a barrier wait ran out, 2 the barrier count was not reset.
 hipError_t sytrd_tail_status(hipStream_t st, int64_t S, const double* xch, unsigned* flag);
+
+// Two-stage tridiagonal reduction (sytrd2.hip): A (column-major, both
+// triangles) to a band of width kBand by blocked QR panels, the band to
+// tridiagonal form by bulge chasing, and the back-transforms.
+constexpr int kBand = 64;
+constexpr int kQ2Group = 32;  // reflectors per block of the bulge-chasing back-transform
+struct TwoStageBufs {
+  double* U = nullptr;      // n x 3 kBand: [V W V] of the current panel
+  double* X = nullptr;      // n x kBand
+  double* Mt = nullptr;     // kBand^2
+  double* Mm = nullptr;     // kBand^2
+  double* T1 = nullptr;     // band_panels(n) x kBand^2: the panels' triangular factors
+  double* xch = nullptr;    // band_xch_doubles(n): the panel kernel's exchange slots
+  double* kwork = nullptr;  // band_kwork_doubles(n): split-K partial products
+  double* Y1 = nullptr;     // kBand x n
+  double* Y2 = nullptr;     // kBand x n
+  unsigned* sync = nullptr; // 2 words: panel barrier count, abort flag
+  unsigned* prog = nullptr; // n + 1 words: bulge-chasing progress per sweep, abort flag
+};
+size_t band_xch_doubles(int64_t n);
+size_t band_kwork_doubles(int64_t n);
+int band_panels(int64_t n);
+// Stage 1: the band in A's lower triangle (zeros below it), each panel's V^T
+// in the upper triangle rows of its columns, T per panel in w.T1.
+hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs& w);
+// Stage 2: d, e of the tridiagonal matrix; reflector (s, k) in column s of
+// V2 (n x n), rows 64k.. (tau in place of the unit first entry).
+hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double* d, double* e,
+                       unsigned* prog);
+// flags[0]: a panel barrier timed out; flags[1]: a bulge-chasing wait timed out.
+hipError_t twostage_status(hipStream_t st, const TwoStageBufs& w, int64_t n, unsigned* flags);
+// Q2 blocks: kQ2Group consecutive sweeps at one task index, numbered group
+// by group (blk_off: q2_groups(n) + 1 prefix counts).
+int q2_groups(int64_t n);
+void q2_block_offsets(int64_t n, std::vector<int>& blk_off);
+// Z := Q2 Z (n x n, leading dimension n), Q2 the product of the bulge
+// chase's reflectors (V2 as bulge_chase leaves it); Tq holds
+// blk_off[nS] x kQ2Group^2 doubles, maxk = the first group's block count.
+hipError_t apply_q2(hipStream_t st, const double* V2, int64_t n, const int* d_blk_off, int nS,
+                    int maxk, double* Tq, double* Z);
+// Q := Q1 Q (ncols columns of Q, leading dimension n).
+hipError_t apply_q1(hipStream_t st, const double* A, int64_t n, const TwoStageBufs& w, double* Q,
+                    int64_t ncols);
 hipError_t ormtr_run(hipStream_t st, const SyTile* d_tiles, int ntiles, int64_t nmax,
                      const std::vector<SyTile>& host, int nbb);
 

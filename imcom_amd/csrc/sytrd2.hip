@@ -1,0 +1,753 @@
+// Kernel (b), two-stage form: the reduction to tridiagonal form inside
+// DSYEVR (imcom_matrix.f90:642) as dense -> band -> tridiagonal, with the
+// back-transform of the eigenvectors through both stages.
+//
+// The one-stage reduction streams the whole trailing matrix once per column
+// (a matrix-vector product, bound by HBM).  Here the first stage reduces A
+// to a band of width kBand = 64 with blocked Householder QR panels whose
+// trailing updates are FP64 matrix products (A22 := Q^T A22 Q with
+// Q = I - V T V^T); the second stage chases the band down to tridiagonal
+// form with reflectors of length kBand that touch only a 64 x 128 window
+// next to the diagonal at a time.
+//
+// Stage 1, per panel of 64 columns k0.. (r0 = k0 + 64 the first row below
+// the band):
+//   * one persistent launch QR-factors the panel A(r0:n, k0:k0+64): its rows
+//     are split among G workgroups (up to 224 rows each, held in LDS), and
+//     each column c takes one grid barrier: every workgroup publishes the
+//     sums P(c+1:, c)^T P(c+1:, :) over its rows, and the pivot row's owner
+//     publishes row c; after the barrier every workgroup forms the same
+//     reflector (dlarfg conventions) and w = tau v^T P(c:, :) from these
+//     totals and updates its rows.  Workgroup 0 also builds the panel's
+//     triangular factor T (LAPACK dlarft, forward, columnwise) from the same
+//     sums: V(:, 0:c)^T v_c needs only the pivot row and the published sums.
+//   * the panel's R goes to the band (zeros below it), V^T to the upper
+//     triangle rows k0..k0+63 (never read as matrix entries again), and V,
+//     padded with zeros to 64 columns, to U = [V W V];
+//   * X = A22 V, Y = X T, M = T^T (V^T Y), W = Y - 1/2 V M and
+//     A22 -= [V W] [W V]^T (both triangles of A22, so the next product can
+//     read it as a plain matrix).
+//
+// Stage 2 (bulge chasing): sweep s annihilates column s below the
+// subdiagonal with a reflector of rows s+1..s+64, applied on both sides to
+// the 64 x 64 diagonal block; that creates fill in the block below it, whose
+// first column the next reflector (rows s+65..) annihilates, and so on down
+// the band (Lang's sequential algorithm; the rest of each fill triangle is
+// annihilated by the following sweeps).  Task (s, k) touches rows
+// r = s+1+64k .. r+63 and columns r-64 .. r+63 only; the task (s, k) may
+// start once (s, k-1) and (s-1, k+1) are done (or all of sweep s-1 when it
+// has fewer tasks), and any two tasks not ordered by these rules touch
+// disjoint entries.  One persistent launch: workgroup g takes sweeps
+// g, g + G, ... in order, and publishes each finished task in a progress
+// counter per sweep (agent-scope release store after its stores complete;
+// the waiter acquires before it reads).  The band and its fill live in
+// column-major A itself (rows j..j+127 of column j).
+//
+// Back-transform: the eigenvectors Z of the tridiagonal matrix become
+// Q1 Q2 Z.  Q2 = H(0,0) H(0,1) ... (creation order) is applied in blocks of
+// 32 consecutive sweeps at one task index k: such a block's reflectors are
+// staggered by one row (rows s0+1+64k .. s0+95+64k), and applying the blocks
+// in the order (sweep group descending, k ascending) with each block as one
+// compact-WY factor gives the product in a valid order (reflectors of one
+// sweep touch disjoint rows; a later sweep's reflector with a higher task
+// index lies wholly below an earlier sweep's with a lower one).  Q1 is the
+// product of the panels' block reflectors, applied last panel first.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "wave.h"
+
+namespace imcom {
+
+namespace {
+
+constexpr int B = kBand;       // band width
+constexpr int PT = 256;        // threads per workgroup
+constexpr int kPanRows = 224;  // panel rows per workgroup at most
+constexpr int LDP = B + 1;     // LDS row stride of the panel rows
+constexpr unsigned kSpinMax = 1u << 22;
+
+// ---------------------------------------------------------------- stage 1
+
+__global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, int64_t n,
+                                                        int64_t k0, int bc, int nref, int rpw,
+                                                        int G, double* __restrict__ U,
+                                                        int64_t ldu, double* __restrict__ T1,
+                                                        double* __restrict__ xch,
+                                                        unsigned* __restrict__ sync,
+                                                        unsigned base, unsigned spin_max) {
+  extern __shared__ double P[];  // owned rows, row-major with stride LDP
+  __shared__ double part[4][B];
+  __shared__ double sv[B], pr[B], wv[B], yv[B];
+  __shared__ double vl[kPanRows];
+  __shared__ double Tl[B][B + 1];  // the triangular factor (workgroup 0)
+  __shared__ int ok_sh;
+  const int tid = threadIdx.x, g = blockIdx.x;
+  const int jj = tid & 63, q = tid >> 6;
+  const int64_t r0 = k0 + B;
+  const int m = (int)(n - r0);
+  const int i0 = g * rpw;
+  const int nr = min(rpw, m - i0);
+  double* part_x = xch;                         // [nref][G][B], written once
+  double* piv_x = xch + (size_t)nref * G * B;  // [nref][B]
+
+  for (int j = 0; j < B; ++j)
+    for (int i = tid; i < nr; i += PT)
+      P[i * LDP + j] = j < bc ? A[(r0 + i0 + i) + (k0 + j) * n] : 0.0;
+  __syncthreads();
+
+  bool ok = true;
+  for (int c = 0; c < nref; ++c) {
+    // (a) this workgroup's share of P(c+1:, c)^T P(c+1:, j), every j: rows
+    // i = q mod 4 from the first below the pivot, four independent chains
+    {
+      int i = max(c + 1 - i0, 0);
+      i += (q - i) & 3;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      for (; i + 12 < nr; i += 16) {
+        s0 += P[i * LDP + c] * P[i * LDP + jj];
+        s1 += P[(i + 4) * LDP + c] * P[(i + 4) * LDP + jj];
+        s2 += P[(i + 8) * LDP + c] * P[(i + 8) * LDP + jj];
+        s3 += P[(i + 12) * LDP + c] * P[(i + 12) * LDP + jj];
+      }
+      for (; i < nr; i += 4) s0 += P[i * LDP + c] * P[i * LDP + jj];
+      part[q][jj] = (s0 + s1) + (s2 + s3);
+    }
+    __syncthreads();
+    if (G == 1) {  // one workgroup: the totals are its own
+      if (tid < B) {
+        sv[tid] = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+        pr[tid] = P[c * LDP + tid];
+      }
+      __syncthreads();
+    } else {
+      if (tid < B) {
+        const double t = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+        st_coh(&part_x[((size_t)c * G + g) * B + tid], t);
+        if (c >= i0 && c < i0 + nr)
+          st_coh(&piv_x[(size_t)c * B + tid], P[(c - i0) * LDP + tid]);
+      }
+      if (!grid_barrier(sync, base + (unsigned)(c + 1) * G, spin_max, &ok_sh)) {
+        ok = false;
+        break;
+      }
+      // (b) totals in a fixed order (the same in every workgroup); plain
+      // loads, all issued before the sum: the slots were written once, with
+      // write-through stores, before the barrier whose acquire dropped this
+      // XCD's stale lines
+      {
+        const double* src = part_x + (size_t)c * G * B + jj;
+        double t0 = 0.0, t1 = 0.0;
+        int h = q;
+        for (; h + 4 < G; h += 8) {
+          t0 += src[(size_t)h * B];
+          t1 += src[(size_t)(h + 4) * B];
+        }
+        if (h < G) t0 += src[(size_t)h * B];
+        part[q][jj] = t0 + t1;
+      }
+      if (tid < B) pr[tid] = piv_x[(size_t)c * B + tid];
+      __syncthreads();
+      if (tid < B) sv[tid] = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+      __syncthreads();
+    }
+    const double alpha = pr[c];
+    const double xnorm = sqrt(sv[c]);
+    double tau, beta, sc;
+    if (xnorm == 0.0) {
+      tau = 0.0;
+      beta = alpha;
+      sc = 0.0;
+    } else {
+      beta = -copysign(dlapy2_t(alpha, xnorm), alpha);
+      tau = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
+    }
+    if (tid < B) {
+      // w_j = tau v^T P(c:, j); V(:, j)^T v for the columns before c
+      wv[tid] = (tid > c && tid < bc) ? tau * (pr[tid] + sc * sv[tid]) : 0.0;
+      yv[tid] = tid < c ? pr[tid] + sc * sv[tid] : 0.0;
+    }
+    for (int i = tid; i < nr; i += PT) vl[i] = P[i * LDP + c] * sc;
+    __syncthreads();
+    if (g == 0 && tid < B) {  // T(0:c, c) = -tau T(0:c, 0:c) V(:, 0:c)^T v, T(c, c) = tau
+      double t = 0.0;
+      if (tid < c) {
+        for (int l = tid; l < c; ++l) t += Tl[tid][l] * yv[l];
+        t *= -tau;
+      } else if (tid == c) {
+        t = tau;
+      }
+      Tl[tid][c] = t;
+    }
+    // (c) the owned rows: P(i, j) -= v_i w_j below the pivot, the pivot row
+    // less w, column c becomes v (beta on the pivot row)
+    if (jj >= c) {
+      const double wj = wv[jj];
+      int i = max(c + 1 - i0, 0);
+      i += (q - i) & 3;
+      if (jj > c) {
+        for (; i + 4 < nr; i += 8) {
+          P[i * LDP + jj] -= vl[i] * wj;
+          P[(i + 4) * LDP + jj] -= vl[i + 4] * wj;
+        }
+        if (i < nr) P[i * LDP + jj] -= vl[i] * wj;
+      } else {
+        for (; i < nr; i += 4) P[i * LDP + jj] = vl[i];
+      }
+      const int ip = c - i0;  // the pivot row
+      if (ip >= 0 && ip < nr && (ip & 3) == q) P[ip * LDP + jj] = jj == c ? beta : P[ip * LDP + jj] - wj;
+    }
+    __syncthreads();
+  }
+  if (!ok) return;
+
+  // R into the band (zeros below it)
+  for (int j = 0; j < bc; ++j)
+    for (int i = tid; i < nr; i += PT) {
+      const int ig = i0 + i;
+      A[(r0 + ig) + (k0 + j) * n] = ig <= j ? P[i * LDP + j] : 0.0;
+    }
+  // V (unit diagonal, zeros above, zero columns past nref) into U's first
+  // and third blocks of columns
+  for (int j = 0; j < B; ++j)
+    for (int i = tid; i < nr; i += PT) {
+      const int ig = i0 + i;
+      double v = 0.0;
+      if (j < nref) v = ig == j ? 1.0 : (ig > j ? P[i * LDP + j] : 0.0);
+      U[ig + j * ldu] = v;
+      U[ig + (2 * B + j) * ldu] = v;
+    }
+  // V^T into the upper triangle: row k0 + j, column r0 + ig
+  if (jj < bc)
+    for (int i = q; i < nr; i += 4) {
+      const int ig = i0 + i;
+      double v = 0.0;
+      if (jj < nref) v = ig == jj ? 1.0 : (ig > jj ? P[i * LDP + jj] : 0.0);
+      A[(k0 + jj) + (r0 + ig) * n] = v;
+    }
+  if (g == 0)
+    for (int idx = tid; idx < B * B; idx += PT) {
+      const int j = idx / B, i = idx % B;
+      T1[idx] = (i <= j && j < nref) ? Tl[i][j] : 0.0;
+    }
+}
+
+__global__ void band_zero_kernel(unsigned* w, int nw) {
+  for (int i = threadIdx.x; i < nw; i += blockDim.x)
+    __hip_atomic_store(&w[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// workgroups and rows per workgroup for m panel rows
+void panel_shape(int m, int* G, int* rpw) {
+  const int g = (m + kPanRows - 1) / kPanRows;
+  *G = g;
+  *rpw = (m + g - 1) / g;
+}
+
+// ---------------------------------------------------------------- stage 2
+
+__global__ __launch_bounds__(PT) void bulge_kernel(double* __restrict__ A, int64_t n,
+                                                   double* __restrict__ V2,
+                                                   unsigned* __restrict__ prog,
+                                                   unsigned spin_max) {
+  __shared__ double E[B][B + 1];  // rows r.., columns r-64.. (k > 0); column s (k = 0)
+  __shared__ double D[B][B + 1];  // the diagonal block, both triangles
+  __shared__ double vp[B], v[B], y[B], z[B], sc[2];
+  __shared__ int ok_sh;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ri = tid >> 2, qq = tid & 3;  // row (or column) and quarter
+  const int64_t nsw = n - 2;
+  unsigned* abort_flag = prog + n;
+  for (int64_t s = blockIdx.x; s < nsw; s += gridDim.x) {
+    const int ntask = (int)((n - 2 - s) / B) + 1;
+    const int ntask_prev = s > 0 ? (int)((n - 1 - s) / B) + 1 : 0;
+    double taup = 0.0;
+    for (int k = 0; k < ntask; ++k) {
+      if (s > 0) {
+        const unsigned need = (unsigned)min(k + 2, ntask_prev);
+        if (tid == 0) {
+          int okv = 1;
+          unsigned it = 0;
+          while (__hip_atomic_load(&prog[s - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                 need) {
+            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                ++it > spin_max) {
+              __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              okv = 0;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          ok_sh = okv;
+        }
+        __syncthreads();
+        if (!ok_sh) return;
+      }
+      const int64_t r = s + 1 + (int64_t)k * B;
+      const int L = (int)min<int64_t>(B, n - r);
+      {
+        // every load issued before the first LDS write: E (columns
+        // r-64+wv+4u, or column s for k = 0) and D (columns r+wv+4u, rows
+        // on or below the diagonal), row r+lane
+        double ev[16], dv[16];
+        const double* Ar = A + (r + lane);
+        if (k == 0) {
+          ev[0] = (wv == 0 && lane < L) ? Ar[s * n] : 0.0;
+        } else {
+#pragma unroll
+          for (int u = 0; u < 16; ++u)
+            ev[u] = lane < L ? Ar[(r - B + wv + 4 * u) * n] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = wv + 4 * u;
+          dv[u] = (lane >= j && lane < L) ? Ar[(r + j) * n] : 0.0;
+        }
+        if (k == 0) {
+          if (wv == 0) E[lane][0] = ev[0];
+        } else {
+#pragma unroll
+          for (int u = 0; u < 16; ++u) E[lane][wv + 4 * u] = ev[u];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = wv + 4 * u;
+          if (lane >= j) {
+            D[lane][j] = dv[u];
+            D[j][lane] = dv[u];
+          }
+        }
+      }
+      __syncthreads();
+      if (k > 0) {  // E := E (I - taup vp vp^T)
+        double t = 0.0;
+#pragma unroll
+        for (int j = 16 * qq; j < 16 * qq + 16; ++j) t += E[ri][j] * vp[j];
+        t = quad_sum_dpp(t) * taup;
+#pragma unroll
+        for (int j = 16 * qq; j < 16 * qq + 16; ++j) E[ri][j] -= t * vp[j];
+        __syncthreads();
+      }
+      if (wv == 0) {  // the reflector of E(0:L, 0)
+        const double x = lane < L ? E[lane][0] : 0.0;
+        const double s2 = wave_sum_dpp(lane >= 1 ? x * x : 0.0);
+        const double alpha = readlane_d(x, 0);
+        double tau, beta, scal;
+        if (L < 2 || s2 == 0.0) {
+          tau = 0.0;
+          beta = alpha;
+          scal = 0.0;
+        } else {
+          beta = -copysign(dlapy2_t(alpha, sqrt(s2)), alpha);
+          tau = (beta - alpha) / beta;
+          scal = 1.0 / (alpha - beta);
+        }
+        v[lane] = lane == 0 ? 1.0 : (lane < L ? x * scal : 0.0);
+        if (lane == 0) {
+          sc[0] = tau;
+          sc[1] = beta;
+        }
+      }
+      __syncthreads();
+      const double tau = sc[0], beta = sc[1];
+      if (k > 0) {  // E(:, 1:) := (I - tau v v^T) E(:, 1:); column 0 becomes beta e1
+        double t = 0.0;
+#pragma unroll
+        for (int i = 16 * qq; i < 16 * qq + 16; ++i)
+          if (i < L) t += v[i] * E[i][ri];
+        t = quad_sum_dpp(t);
+        if (qq == 0) z[ri] = t;
+        __syncthreads();
+        const double tv = tau * v[ri];
+#pragma unroll
+        for (int j = 16 * qq; j < 16 * qq + 16; ++j)
+          E[ri][j] = j == 0 ? (ri == 0 ? beta : 0.0) : E[ri][j] - tv * z[j];
+      }
+      {  // p = tau D v
+        double t = 0.0;
+#pragma unroll
+        for (int j = 16 * qq; j < 16 * qq + 16; ++j)
+          if (j < L) t += D[ri][j] * v[j];
+        t = quad_sum_dpp(t) * tau;
+        if (qq == 0) y[ri] = ri < L ? t : 0.0;
+      }
+      __syncthreads();
+      if (wv == 0) {  // w = p - 1/2 tau (p^T v) v
+        const double pv = wave_sum_dpp(y[lane] * v[lane]);
+        z[lane] = y[lane] - 0.5 * tau * pv * v[lane];
+      }
+      __syncthreads();
+      if (ri < L) {  // D -= v w^T + w v^T (lower triangle)
+        const double vi = v[ri], wi = z[ri];
+#pragma unroll
+        for (int j = 16 * qq; j < 16 * qq + 16; ++j)
+          if (j <= ri) D[ri][j] -= vi * z[j] + wi * v[j];
+      }
+      __syncthreads();
+      if (k == 0) {
+        if (tid < L) A[(r + tid) + s * n] = tid == 0 ? beta : 0.0;
+      } else {
+        for (int j = wv; j < B; j += 4)
+          if (lane < L) A[(r + lane) + (r - B + j) * n] = E[lane][j];
+      }
+      for (int j = wv; j < L; j += 4)
+        if (lane >= j && lane < L) A[(r + lane) + (r + j) * n] = D[lane][j];
+      if (tid < L) V2[s * n + (int64_t)k * B + tid] = tid == 0 ? tau : v[tid];
+      if (tid < B) vp[tid] = v[tid];
+      taup = tau;
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (tid == 0)
+        __hip_atomic_store(&prog[s], (unsigned)(k + 1), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+__global__ void band_diag_kernel(const double* __restrict__ A, int64_t n, double* __restrict__ d,
+                                 double* __restrict__ e) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  d[j] = A[j + j * n];
+  if (j + 1 < n) e[j] = A[(j + 1) + j * n];
+}
+
+// ------------------------------------------------- bulge back-transform
+
+constexpr int QL = kQ2Group;       // reflectors per block (consecutive sweeps)
+constexpr int QR = 96;             // block rows (B + QL - 1 = 95, padded)
+constexpr int QC = 64;             // columns of Z per workgroup
+constexpr int LZ = QC + 8;         // LDS row stride of the Z window
+constexpr int LV = QL + 1;         // LDS row stride of V
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int q2_ntask(int64_t n, int64_t s) { return (int)((n - 2 - s) / B) + 1; }
+
+// V of block (S, k) into Vs (QR x QL, zero outside the reflectors) and
+// tau into tau_s: column j is reflector (s0 + j, k), rows j .. j + L - 1.
+__device__ __forceinline__ void q2_load_V(const double* __restrict__ V2, int64_t n, int64_t s0,
+                                          int k, double* Vs, double* tau_s) {
+  const int64_t nsw = n - 2;
+  for (int idx = threadIdx.x; idx < QR * QL; idx += blockDim.x) Vs[(idx / QL) * LV + idx % QL] = 0.0;
+  __syncthreads();
+  const int64_t rb = s0 + 1 + (int64_t)k * B;
+  for (int idx = threadIdx.x; idx < QL * B; idx += blockDim.x) {
+    const int j = idx / B, i = idx % B;
+    const int64_t s = s0 + j;
+    double v = 0.0;
+    bool have = s < nsw && k < q2_ntask(n, s);
+    int L = 0;
+    if (have) {
+      L = (int)min<int64_t>(B, n - (rb + j));
+      if (i < L) v = V2[s * n + (int64_t)k * B + i];
+    }
+    if (i == 0) {
+      tau_s[j] = have ? v : 0.0;
+      v = have ? 1.0 : 0.0;
+    }
+    if (have && i < L) Vs[(j + i) * LV + j] = v;
+  }
+  __syncthreads();
+}
+
+// T of every block (LAPACK dlarft, forward, columnwise): one workgroup per
+// block, blocks numbered group by group (blk_off[S] + k).
+__global__ __launch_bounds__(256) void q2_tfactor_kernel(const double* __restrict__ V2, int64_t n,
+                                                         const int* __restrict__ blk_off, int nS,
+                                                         double* __restrict__ Tq) {
+  __shared__ double Vs[QR * LV];
+  __shared__ double tau_s[QL];
+  __shared__ double Gm[QL][QL + 1];
+  __shared__ double Ts[QL][QL + 1];
+  const int S = blockIdx.y, k = blockIdx.x;
+  if (S >= nS || k >= blk_off[S + 1] - blk_off[S]) return;
+  const int64_t s0 = (int64_t)S * QL;
+  q2_load_V(V2, n, s0, k, Vs, tau_s);
+  const int tid = threadIdx.x;
+  // Gram matrix of V's columns (upper triangle)
+  for (int idx = tid; idx < QL * QL; idx += blockDim.x) {
+    const int i = idx / QL, j = idx % QL;
+    double g = 0.0;
+    if (i < j)
+      for (int r = j; r < min(QR, i + B); ++r) g += Vs[r * LV + i] * Vs[r * LV + j];
+    Gm[i][j] = g;
+  }
+  __syncthreads();
+  for (int j = 0; j < QL; ++j) {
+    if (tid < QL) {
+      double t = 0.0;
+      if (tid < j) {
+        for (int l = tid; l < j; ++l) t += Ts[tid][l] * Gm[l][j];
+        t *= -tau_s[j];
+      } else if (tid == j) {
+        t = tau_s[j];
+      }
+      Ts[tid][j] = t;
+    }
+    __syncthreads();
+  }
+  double* out = Tq + ((size_t)blk_off[S] + k) * QL * QL;
+  for (int idx = tid; idx < QL * QL; idx += blockDim.x) out[idx] = Ts[idx % QL][idx / QL];
+}
+
+// Z := Q2 Z for columns c0 .. c0+63 of Z: blocks (S descending, k
+// ascending), each Z(rows, :) -= V (T (V^T Z(rows, :))) on FP64 MFMA, the
+// block's rows held in LDS; the last QL - 1 rows of one block are the first
+// of the next (k + 1), so only B new rows are read per block.
+__global__ __launch_bounds__(256) void q2_apply_kernel(const double* __restrict__ V2, int64_t n,
+                                                       const int* __restrict__ blk_off, int nS,
+                                                       const double* __restrict__ Tq,
+                                                       double* __restrict__ Z) {
+  __shared__ double Zs[QR * LZ];
+  __shared__ double Vs[QR * LV];
+  __shared__ double Ws[QL * LZ];
+  __shared__ double Ts[QL * LV];
+  __shared__ double tau_s[QL];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int64_t c0 = (int64_t)blockIdx.x * QC;
+  const int nc = (int)min<int64_t>(QC, n - c0);
+  for (int S = nS - 1; S >= 0; --S) {
+    const int64_t s0 = (int64_t)S * QL;
+    const int nk = blk_off[S + 1] - blk_off[S];
+    for (int k = 0; k < nk; ++k) {
+      const int64_t rb = s0 + 1 + (int64_t)k * B;
+      const int R = (int)min<int64_t>(QR - 1, n - rb);
+      const int keep = k == 0 ? 0 : QL - 1;  // rows carried from block k - 1
+      // new rows keep .. R-1 of the window (zeros past R)
+      for (int idx = tid; idx < (QR - keep) * QC; idx += blockDim.x) {
+        const int i = keep + idx / QC, c = idx % QC;
+        Zs[i * LZ + c] = (i < R && c < nc) ? Z[(rb + i) + (c0 + c) * n] : 0.0;
+      }
+      q2_load_V(V2, n, s0, k, Vs, tau_s);  // ends with a barrier
+      {
+        const double* Tg = Tq + ((size_t)blk_off[S] + k) * QL * QL;
+        for (int idx = tid; idx < QL * QL; idx += blockDim.x)
+          Ts[(idx % QL) * LV + idx / QL] = Tg[idx];
+      }
+      // W = V^T Z: 2 x 4 tiles of 16 x 16, wavefront wv: tile row wv & 1,
+      // tile columns 2 (wv >> 1) and 2 (wv >> 1) + 1
+      v4d acc[2];
+      const int tr = wv & 1, tc0 = 2 * (wv >> 1);
+      acc[0] = acc[1] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int kk = 0; kk < QR; kk += 4) {
+        const double a = Vs[(kk + lk) * LV + tr * 16 + lr];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const double b = Zs[(kk + lk) * LZ + (tc0 + t) * 16 + lr];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        }
+      }
+      __syncthreads();  // Ts complete
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + (tc0 + t) * 16 + lr] = acc[t][r];
+      __syncthreads();
+      // W := T W
+      acc[0] = acc[1] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int kk = 0; kk < QL; kk += 4) {
+        const double a = Ts[(tr * 16 + lr) * LV + kk + lk];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const double b = Ws[(kk + lk) * LZ + (tc0 + t) * 16 + lr];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + (tc0 + t) * 16 + lr] = acc[t][r];
+      __syncthreads();
+      // Z -= V W: wavefront wv takes column tile wv, all six row tiles
+      {
+        v4d z[QR / 16];
+#pragma unroll
+        for (int t = 0; t < QR / 16; ++t) z[t] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < QL; kk += 4) {
+          const double b = Ws[(kk + lk) * LZ + wv * 16 + lr];
+#pragma unroll
+          for (int t = 0; t < QR / 16; ++t) {
+            const double a = Vs[(t * 16 + lr) * LV + kk + lk];
+            z[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, z[t], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int t = 0; t < QR / 16; ++t)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) Zs[(t * 16 + 4 * r + lk) * LZ + wv * 16 + lr] -= z[t][r];
+      }
+      __syncthreads();
+      // rows that no later block of this group touches go back to Z; the
+      // last QL - 1 rows move to the top of the window
+      const bool last = k + 1 == nk;
+      const int nout = last ? R : min(B, R);
+      for (int idx = tid; idx < nout * QC; idx += blockDim.x) {
+        const int i = idx / QC, c = idx % QC;
+        if (c < nc) Z[(rb + i) + (c0 + c) * n] = Zs[i * LZ + c];
+      }
+      if (!last) {
+        __syncthreads();
+        for (int idx = tid; idx < (QL - 1) * QC; idx += blockDim.x) {
+          const int i = idx / QC, c = idx % QC;
+          Zs[i * LZ + c] = Zs[(B + i) * LZ + c];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+}  // namespace
+
+size_t band_xch_doubles(int64_t n) {
+  int G, rpw;
+  panel_shape((int)std::max<int64_t>(n - B, 1), &G, &rpw);
+  return (size_t)B * G * B + (size_t)B * B;
+}
+
+size_t band_kwork_doubles(int64_t n) {
+  size_t mx = 8;
+  for (int64_t k0 = 0; n - (k0 + B) >= 2; k0 += B) {
+    const int m = (int)(n - k0 - B);
+    if ((m + 63) / 64 < 128) mx = std::max(mx, dgemm_splitk_work(m, B, m));
+    mx = std::max(mx, dgemm_splitk_work(B, B, m));
+  }
+  return mx / 8 + 1;
+}
+
+int band_panels(int64_t n) {
+  int np = 0;
+  for (int64_t k0 = 0; n - (k0 + B) >= 2; k0 += B) ++np;
+  return np;
+}
+
+hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs& w) {
+  hipLaunchKernelGGL(band_zero_kernel, dim3(1), dim3(64), 0, st, w.sync, 2);
+  IMCOM_HIP_TRY(hipGetLastError());
+  static bool attr_set = false;
+  if (!attr_set) {
+    IMCOM_HIP_TRY(hipFuncSetAttribute((const void*)band_panel_kernel,
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      kPanRows * LDP * 8));
+    attr_set = true;
+  }
+  unsigned base = 0;
+  const int64_t ldu = n;
+  int p = 0;
+  for (int64_t k0 = 0;; k0 += B, ++p) {
+    const int64_t r0 = k0 + B;
+    const int m = (int)(n - r0);
+    if (m < 2) break;
+    const int bc = (int)std::min<int64_t>(B, n - k0);
+    const int nref = std::min(bc, m - 1);
+    int G, rpw;
+    panel_shape(m, &G, &rpw);
+    double* T1p = w.T1 + (size_t)p * B * B;
+    hipLaunchKernelGGL(band_panel_kernel, dim3(G), dim3(PT), (size_t)rpw * LDP * 8, st, A, n, k0,
+                       bc, nref, rpw, G, w.U, ldu, T1p, w.xch, w.sync, base, kSpinMax);
+    IMCOM_HIP_TRY(hipGetLastError());
+    base += (unsigned)nref * G;
+    double* A22 = A + r0 + r0 * n;
+    double* V = w.U;
+    double* Wb = w.U + B * ldu;
+    // X = A22 V
+    if ((m + 63) / 64 >= 128)
+      IMCOM_HIP_TRY(dgemm(st, false, false, m, B, m, 1.0, A22, n, V, ldu, 0.0, w.X, m));
+    else
+      IMCOM_HIP_TRY(dgemm_splitk(st, false, false, m, B, m, 1.0, A22, n, V, ldu, w.X, m, w.kwork));
+    // Y = X T (into W's place)
+    IMCOM_HIP_TRY(dgemm(st, false, false, m, B, B, 1.0, w.X, m, T1p, B, 0.0, Wb, ldu));
+    // Mt = V^T Y, M = T^T Mt
+    IMCOM_HIP_TRY(dgemm_splitk(st, true, false, B, B, m, 1.0, V, ldu, Wb, ldu, w.Mt, B, w.kwork));
+    IMCOM_HIP_TRY(dgemm(st, true, false, B, B, B, 1.0, T1p, B, w.Mt, B, 0.0, w.Mm, B));
+    // W = Y - 1/2 V M
+    IMCOM_HIP_TRY(dgemm(st, false, false, m, B, B, -0.5, V, ldu, w.Mm, B, 1.0, Wb, ldu));
+    // A22 -= [V W] [W V]^T
+    IMCOM_HIP_TRY(dgemm(st, false, true, m, m, 2 * B, -1.0, w.U, ldu, Wb, ldu, 1.0, A22, n));
+  }
+  return hipSuccess;
+}
+
+hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double* d, double* e,
+                       unsigned* prog) {
+  if (n >= 3) {
+    int dev = 0, ncu = 0;
+    IMCOM_HIP_TRY(hipGetDevice(&dev));
+    IMCOM_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    hipLaunchKernelGGL(band_zero_kernel, dim3(1), dim3(256), 0, st, prog, (int)n + 1);
+    IMCOM_HIP_TRY(hipGetLastError());
+    // LDS takes 67 KB of a workgroup, so two fit a CU: G <= 2 x CUs are all
+    // resident at once on an otherwise idle GPU
+    const int G = (int)std::min<int64_t>(n - 2, ncu);
+    hipLaunchKernelGGL(bulge_kernel, dim3(G), dim3(PT), 0, st, A, n, V2, prog, kSpinMax);
+    IMCOM_HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(band_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, n,
+                     d, e);
+  return hipGetLastError();
+}
+
+// Abort flags of the panel barriers and of the bulge chase (0: none).
+hipError_t twostage_status(hipStream_t st, const TwoStageBufs& w, int64_t n, unsigned* flags) {
+  IMCOM_HIP_TRY(
+      hipMemcpyAsync(&flags[0], w.sync + 1, 4, hipMemcpyDeviceToHost, st));
+  flags[1] = 0;
+  if (n >= 3)
+    IMCOM_HIP_TRY(hipMemcpyAsync(&flags[1], w.prog + n, 4, hipMemcpyDeviceToHost, st));
+  return hipStreamSynchronize(st);
+}
+
+// Q := Q1 Q with Q1 the product of the panels' block reflectors (V^T in the
+// upper triangle of A, T per panel).
+hipError_t apply_q1(hipStream_t st, const double* A, int64_t n, const TwoStageBufs& w, double* Q,
+                    int64_t ncols) {
+  const int np = band_panels(n);
+  for (int p = np - 1; p >= 0; --p) {
+    const int64_t k0 = (int64_t)p * B, r0 = k0 + B;
+    const int m = (int)(n - r0);
+    const double* Vt = A + k0 + r0 * n;  // B x m, leading dimension n
+    const double* T1p = w.T1 + (size_t)p * B * B;
+    IMCOM_HIP_TRY(dgemm(st, false, false, B, (int)ncols, m, 1.0, Vt, n, Q + r0, n, 0.0, w.Y1, B));
+    IMCOM_HIP_TRY(dgemm(st, false, false, B, (int)ncols, B, 1.0, T1p, B, w.Y1, B, 0.0, w.Y2, B));
+    IMCOM_HIP_TRY(dgemm(st, true, false, m, (int)ncols, B, -1.0, Vt, n, w.Y2, B, 1.0, Q + r0, n));
+  }
+  return hipSuccess;
+}
+
+}  // namespace imcom
+
+namespace imcom {
+
+int q2_groups(int64_t n) { return (int)((std::max<int64_t>(n - 2, 0) + QL - 1) / QL); }
+
+// Blocks of group S: the task count of its first sweep.  blk_off has
+// q2_groups(n) + 1 entries.
+void q2_block_offsets(int64_t n, std::vector<int>& blk_off) {
+  const int nS = q2_groups(n);
+  blk_off.assign(nS + 1, 0);
+  for (int S = 0; S < nS; ++S)
+    blk_off[S + 1] = blk_off[S] + (int)((n - 2 - (int64_t)S * QL) / B) + 1;
+}
+
+hipError_t apply_q2(hipStream_t st, const double* V2, int64_t n, const int* d_blk_off, int nS,
+                    int maxk, double* Tq, double* Z) {
+  if (nS <= 0) return hipSuccess;
+  hipLaunchKernelGGL(q2_tfactor_kernel, dim3((unsigned)maxk, (unsigned)nS), dim3(256), 0, st, V2,
+                     n, d_blk_off, nS, Tq);
+  IMCOM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(q2_apply_kernel, dim3((unsigned)((n + QC - 1) / QC)), dim3(256), 0, st, V2,
+                     n, d_blk_off, nS, Tq, Z);
+  return hipGetLastError();
+}
+
+}  // namespace imcom

@@ -40,7 +40,8 @@ EXPORTS = [
     "imcom_gpu_mlp_grad", "imcom_gpu_mlp_train", "imcom_gpu_bisect_kappa_learned",
     "imcom_gpu_batch_set", "imcom_gpu_batch_solve", "imcom_gpu_batch_reserve",
     "imcom_gpu_batch_result",
-    "imcom_gpu_set_tuning", "imcom_gpu_set_reduction_tail", "imcom_gpu_fetch_async", "imcom_gpu_fetch_wait",
+    "imcom_gpu_set_tuning", "imcom_gpu_set_reduction_tail", "imcom_gpu_set_reduction_stages",
+    "imcom_gpu_band_reduce", "imcom_gpu_fetch_async", "imcom_gpu_fetch_wait",
     "imcom_gpu_host_alloc", "imcom_gpu_host_free", "imcom_gpu_bisect_kappa_learned_noalloc",
 ]
 
@@ -136,6 +137,8 @@ def load_library(path: str = LIB_PATH):
         "imcom_gpu_batch_result": (_int, [_vp, _int, _dp, _dp, _dp, _dp, _ip, _ip, _dp]),
         "imcom_gpu_set_tuning": (_int, [_vp, _int, _int]),
         "imcom_gpu_set_reduction_tail": (_int, [_vp, _int]),
+        "imcom_gpu_set_reduction_stages": (_int, [_vp, _int, _i64]),
+        "imcom_gpu_band_reduce": (_int, [_vp, _dp]),
         "imcom_gpu_assemble_inputs": (_int, [_vp, _int, _int, _int, _dp, _dbl, _dp, _dp, _dp,
                                               ctypes.POINTER(_i64), _dp, _dp, _ip, _dp]),
         "imcom_gpu_rotate_image": (_int, [_vp, _int, _int, _dp, _dbl, _int, _int, _dp]),
@@ -508,6 +511,20 @@ class ImcomGPU:
         """Largest trailing size whose columns the persistent kernel reduces
         (the library's default 2048; 0: every column on the blocked kernels)."""
         self._ck(self.lib.imcom_gpu_set_reduction_tail(self.h, max_rows), "set_reduction_tail")
+
+    def set_reduction_stages(self, stages=1, min_n=4096):
+        """Reduction to tridiagonal form: 1 the one-stage blocked kernels, 2
+        the two-stage reduction (dense -> band -> tridiagonal) for n >= min_n."""
+        self._ck(self.lib.imcom_gpu_set_reduction_stages(self.h, stages, min_n),
+                 "set_reduction_stages")
+
+    def band_reduce(self):
+        """First stage of the two-stage reduction alone (A consumed): the n x n
+        result, band matrix in the lower triangle, the panels' Householder
+        vectors in the upper."""
+        out = np.empty((self.n, self.n), order="F")
+        self._ck(self.lib.imcom_gpu_band_reduce(self.h, _ptr(out)), "band_reduce")
+        return out
 
     # ---- batched tiles ----------------------------------------------------
     def batch_set(self, tiles):

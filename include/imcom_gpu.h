@@ -282,6 +282,22 @@ int imcom_gpu_set_tuning(imcom_gpu_ctx* ctx, int nb, int nbb);
  * (imcom_matrix.f90:642); no reference counterpart of its own. */
 int imcom_gpu_set_reduction_tail(imcom_gpu_ctx* ctx, int max_rows);
 
+/* Selects the reduction to tridiagonal form inside eigen_A / tridiagonalize
+ * (the DSYTRD stage of DSYEVR, imcom_matrix.f90:642): stages = 1, the
+ * one-stage blocked reduction (default); stages = 2, for n >= min_n, the
+ * two-stage reduction (A to a band of width 64 by blocked QR panels with
+ * matrix-product updates, then bulge chasing to tridiagonal form), with the
+ * eigenvectors back-transformed through both stages.  Same outputs and
+ * conventions either way; the eigenvalues agree to rounding. */
+int imcom_gpu_set_reduction_stages(imcom_gpu_ctx* ctx, int stages, int64_t min_n);
+/* Diagnostic: the first stage of the two-stage reduction alone.  A (n >= 3)
+ * is reduced in place and downloaded (n x n, column-major): its lower
+ * triangle holds the symmetric band matrix (bandwidth 64, zeros below the
+ * band), orthogonally similar to the input; the upper triangle holds the
+ * panels' Householder vectors.  A is consumed, as by eigen_A.  No reference
+ * counterpart (a stage inside DSYEVR, imcom_matrix.f90:642). */
+int imcom_gpu_band_reduce(imcom_gpu_ctx* ctx, double* A_out);
+
 int imcom_gpu_batch_set(imcom_gpu_ctx* ctx, int ntiles, const imcom_gpu_tile* tiles);
 int imcom_gpu_batch_solve(imcom_gpu_ctx* ctx, int precision_P, int precision_T, int learned,
                           float zsig, int use_graph);
