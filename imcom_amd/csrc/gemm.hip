@@ -338,7 +338,8 @@ static void splitk_shape(int M, int N, int K, int* tile, int* ksplit) {
   const int maxs = std::max(1, K / 512);
   const int64_t t128 = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
   int s = (int)std::min<int64_t>(maxs, std::max<int64_t>(1, (1024 + t128 - 1) / t128));
-  if (t128 * s >= 256) {
+  // a 128-wide tile over a dimension of at most 64 would compute half zeros
+  if (t128 * s >= 256 && M > 64 && N > 64) {
     *tile = 128;
     *ksplit = s;
     return;

@@ -359,8 +359,7 @@ __global__ __launch_bounds__(PT) void bulge_kernel(double* __restrict__ A, int64
       if (k > 0) {  // E(:, 1:) := (I - tau v v^T) E(:, 1:); column 0 becomes beta e1
         double t = 0.0;
 #pragma unroll
-        for (int i = 16 * qq; i < 16 * qq + 16; ++i)
-          if (i < L) t += v[i] * E[i][ri];
+        for (int i = 16 * qq; i < 16 * qq + 16; ++i) t += v[i] * E[i][ri];  // rows >= L are 0
         t = quad_sum_dpp(t);
         if (qq == 0) z[ri] = t;
         __syncthreads();
@@ -372,8 +371,7 @@ __global__ __launch_bounds__(PT) void bulge_kernel(double* __restrict__ A, int64
       {  // p = tau D v
         double t = 0.0;
 #pragma unroll
-        for (int j = 16 * qq; j < 16 * qq + 16; ++j)
-          if (j < L) t += D[ri][j] * v[j];
+        for (int j = 16 * qq; j < 16 * qq + 16; ++j) t += D[ri][j] * v[j];  // v_j = 0 for j >= L
         t = quad_sum_dpp(t) * tau;
         if (qq == 0) y[ri] = ri < L ? t : 0.0;
       }
@@ -383,11 +381,10 @@ __global__ __launch_bounds__(PT) void bulge_kernel(double* __restrict__ A, int64
         z[lane] = y[lane] - 0.5 * tau * pv * v[lane];
       }
       __syncthreads();
-      if (ri < L) {  // D -= v w^T + w v^T (lower triangle)
+      {  // D -= v w^T + w v^T (v, w vanish past L; the upper triangle is not read again)
         const double vi = v[ri], wi = z[ri];
 #pragma unroll
-        for (int j = 16 * qq; j < 16 * qq + 16; ++j)
-          if (j <= ri) D[ri][j] -= vi * z[j] + wi * v[j];
+        for (int j = 16 * qq; j < 16 * qq + 16; ++j) D[ri][j] -= vi * z[j] + wi * v[j];
       }
       __syncthreads();
       if (k == 0) {
@@ -422,7 +419,7 @@ __global__ void band_diag_kernel(const double* __restrict__ A, int64_t n, double
 
 constexpr int QL = kQ2Group;       // reflectors per block (consecutive sweeps)
 constexpr int QR = 96;             // block rows (B + QL - 1 = 95, padded)
-constexpr int QC = 64;             // columns of Z per workgroup
+constexpr int QC = 32;             // columns of Z per workgroup
 constexpr int LZ = QC + 8;         // LDS row stride of the Z window
 constexpr int LV = QL + 1;         // LDS row stride of V
 
@@ -497,113 +494,189 @@ __global__ __launch_bounds__(256) void q2_tfactor_kernel(const double* __restric
   for (int idx = tid; idx < QL * QL; idx += blockDim.x) out[idx] = Ts[idx % QL][idx / QL];
 }
 
-// Z := Q2 Z for columns c0 .. c0+63 of Z: blocks (S descending, k
+// Z := Q2 Z for columns c0 .. c0+QC-1 of Z: blocks (S descending, k
 // ascending), each Z(rows, :) -= V (T (V^T Z(rows, :))) on FP64 MFMA, the
 // block's rows held in LDS; the last QL - 1 rows of one block are the first
-// of the next (k + 1), so only B new rows are read per block.
-__global__ __launch_bounds__(256) void q2_apply_kernel(const double* __restrict__ V2, int64_t n,
-                                                       const int* __restrict__ blk_off, int nS,
-                                                       const double* __restrict__ Tq,
-                                                       double* __restrict__ Z) {
+// of the next (k + 1), so only B new rows are read per block.  The next
+// block's V, T and new rows are loaded into registers at the start of a
+// block and written to the LDS at its end, so the loads run under the
+// block's products (its new rows lie below the current block's; at a change
+// of group they are read after the current block's stores instead).  The
+// LDS (70 KB) and registers (two waves per SIMD) let two workgroups share a
+// CU.
+__device__ __forceinline__ void q2_regs_load(const double* __restrict__ V2, int64_t n, int S,
+                                             int k, const double* __restrict__ Tq,
+                                             const int* __restrict__ blk_off, bool with_z,
+                                             const double* __restrict__ Z, int64_t c0, int nc,
+                                             double* rv, double* rt, double* rz) {
+  constexpr int NV = QR * QL / 256, NT = QL * QL / 256, NZ = QR * QC / 256;
+  const int tid = threadIdx.x;
+  const int64_t s0 = (int64_t)S * QL, nsw = n - 2;
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int idx = tid + 256 * u;
+    const int j = idx / QR, row = idx % QR, i = row - j;
+    const int64_t s = s0 + j;
+    double v = 0.0;
+    if (s < nsw && k < q2_ntask(n, s) && i >= 0) {
+      const int L = (int)min<int64_t>(B, n - (s + 1 + (int64_t)k * B));
+      if (i < L) v = V2[s * n + (int64_t)k * B + i];
+    }
+    rv[u] = v;
+  }
+  const double* Tg = Tq + ((size_t)blk_off[S] + k) * QL * QL;
+#pragma unroll
+  for (int u = 0; u < NT; ++u) rt[u] = Tg[tid + 256 * u];
+  if (with_z) {
+    const int64_t rb = s0 + 1 + (int64_t)k * B;
+    const int R = (int)min<int64_t>(QR - 1, n - rb);
+    const int keep = k == 0 ? 0 : QL - 1;
+#pragma unroll
+    for (int u = 0; u < NZ; ++u) {
+      const int idx = tid + 256 * u;
+      const int i = keep + idx / QC, c = idx % QC;
+      double v = 0.0;
+      if (i < R && c < nc) v = Z[(rb + i) + (c0 + c) * n];
+      rz[u] = v;
+    }
+  }
+}
+
+__device__ __forceinline__ void q2_regs_store(int k, bool with_z, const double* rv,
+                                              const double* rt, const double* rz, double* Zs,
+                                              double* Vs, double* Ts, double* tau_s) {
+  constexpr int NV = QR * QL / 256, NT = QL * QL / 256, NZ = QR * QC / 256;
+  const int tid = threadIdx.x;
+  if (with_z) {
+    const int keep = k == 0 ? 0 : QL - 1;
+#pragma unroll
+    for (int u = 0; u < NZ; ++u) {
+      const int idx = tid + 256 * u;
+      const int i = keep + idx / QC, c = idx % QC;
+      if (i < QR) Zs[i * LZ + c] = rz[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int idx = tid + 256 * u;
+    const int j = idx / QR, row = idx % QR;
+    double v = rv[u];
+    if (row == j) {
+      tau_s[j] = v;
+      v = v != 0.0 ? 1.0 : 0.0;  // an absent reflector reads 0 (tau 0)
+    }
+    Vs[row * LV + j] = v;
+  }
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const int idx = tid + 256 * u;
+    Ts[(idx % QL) * LV + idx / QL] = rt[u];
+  }
+}
+
+// two waves per SIMD (registers capped at half the file), so that two
+// workgroups share a CU
+__global__ __launch_bounds__(256, 2) void q2_apply_kernel(const double* __restrict__ V2, int64_t n,
+                                                          const int* __restrict__ blk_off, int nS,
+                                                          const double* __restrict__ Tq,
+                                                          double* __restrict__ Z) {
   __shared__ double Zs[QR * LZ];
   __shared__ double Vs[QR * LV];
   __shared__ double Ws[QL * LZ];
   __shared__ double Ts[QL * LV];
   __shared__ double tau_s[QL];
+  constexpr int NV = QR * QL / 256, NT = QL * QL / 256, NZ = QR * QC / 256;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int64_t c0 = (int64_t)blockIdx.x * QC;
   const int nc = (int)min<int64_t>(QC, n - c0);
-  for (int S = nS - 1; S >= 0; --S) {
+  // wavefront tiles: W (32 x 32) tile (tr, tc); Z (96 x 32) column tile zc,
+  // row tiles 3 zr0 .. 3 zr0 + 2
+  const int tr = wv & 1, tc = wv >> 1, zc = wv & 1, zr0 = 3 * (wv >> 1);
+  if (nS <= 0) return;
+  {
+    double rv[NV], rt[NT], rz[NZ];
+    q2_regs_load(V2, n, nS - 1, 0, Tq, blk_off, true, Z, c0, nc, rv, rt, rz);
+    q2_regs_store(0, true, rv, rt, rz, Zs, Vs, Ts, tau_s);
+  }
+  lds_sync();
+  int S = nS - 1, k = 0;
+  while (S >= 0) {
     const int64_t s0 = (int64_t)S * QL;
     const int nk = blk_off[S + 1] - blk_off[S];
-    for (int k = 0; k < nk; ++k) {
-      const int64_t rb = s0 + 1 + (int64_t)k * B;
-      const int R = (int)min<int64_t>(QR - 1, n - rb);
-      const int keep = k == 0 ? 0 : QL - 1;  // rows carried from block k - 1
-      // new rows keep .. R-1 of the window (zeros past R)
-      for (int idx = tid; idx < (QR - keep) * QC; idx += blockDim.x) {
-        const int i = keep + idx / QC, c = idx % QC;
-        Zs[i * LZ + c] = (i < R && c < nc) ? Z[(rb + i) + (c0 + c) * n] : 0.0;
-      }
-      q2_load_V(V2, n, s0, k, Vs, tau_s);  // ends with a barrier
-      {
-        const double* Tg = Tq + ((size_t)blk_off[S] + k) * QL * QL;
-        for (int idx = tid; idx < QL * QL; idx += blockDim.x)
-          Ts[(idx % QL) * LV + idx / QL] = Tg[idx];
-      }
-      // W = V^T Z: 2 x 4 tiles of 16 x 16, wavefront wv: tile row wv & 1,
-      // tile columns 2 (wv >> 1) and 2 (wv >> 1) + 1
-      v4d acc[2];
-      const int tr = wv & 1, tc0 = 2 * (wv >> 1);
-      acc[0] = acc[1] = (v4d){0.0, 0.0, 0.0, 0.0};
-      for (int kk = 0; kk < QR; kk += 4) {
-        const double a = Vs[(kk + lk) * LV + tr * 16 + lr];
+    const int64_t rb = s0 + 1 + (int64_t)k * B;
+    const int R = (int)min<int64_t>(QR - 1, n - rb);
+    const bool last = k + 1 == nk;
+    const int nS2 = last ? S - 1 : S, nk2 = last ? 0 : k + 1;
+    // the next block's loads, in flight during this block's products
+    double rv[NV], rt[NT], rz[NZ];
+    if (nS2 >= 0) q2_regs_load(V2, n, nS2, nk2, Tq, blk_off, !last, Z, c0, nc, rv, rt, rz);
+    // W = V^T Z, two accumulation chains (even and odd steps)
+    v4d acc0 = (v4d){0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          const double b = Zs[(kk + lk) * LZ + (tc0 + t) * 16 + lr];
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-        }
-      }
-      __syncthreads();  // Ts complete
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + (tc0 + t) * 16 + lr] = acc[t][r];
-      __syncthreads();
-      // W := T W
-      acc[0] = acc[1] = (v4d){0.0, 0.0, 0.0, 0.0};
-      for (int kk = 0; kk < QL; kk += 4) {
-        const double a = Ts[(tr * 16 + lr) * LV + kk + lk];
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-          const double b = Ws[(kk + lk) * LZ + (tc0 + t) * 16 + lr];
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + (tc0 + t) * 16 + lr] = acc[t][r];
-      __syncthreads();
-      // Z -= V W: wavefront wv takes column tile wv, all six row tiles
-      {
-        v4d z[QR / 16];
-#pragma unroll
-        for (int t = 0; t < QR / 16; ++t) z[t] = (v4d){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int kk = 0; kk < QL; kk += 4) {
-          const double b = Ws[(kk + lk) * LZ + wv * 16 + lr];
-#pragma unroll
-          for (int t = 0; t < QR / 16; ++t) {
-            const double a = Vs[(t * 16 + lr) * LV + kk + lk];
-            z[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, z[t], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int t = 0; t < QR / 16; ++t)
-#pragma unroll
-          for (int r = 0; r < 4; ++r) Zs[(t * 16 + 4 * r + lk) * LZ + wv * 16 + lr] -= z[t][r];
-      }
-      __syncthreads();
-      // rows that no later block of this group touches go back to Z; the
-      // last QL - 1 rows move to the top of the window
-      const bool last = k + 1 == nk;
-      const int nout = last ? R : min(B, R);
-      for (int idx = tid; idx < nout * QC; idx += blockDim.x) {
-        const int i = idx / QC, c = idx % QC;
-        if (c < nc) Z[(rb + i) + (c0 + c) * n] = Zs[i * LZ + c];
-      }
-      if (!last) {
-        __syncthreads();
-        for (int idx = tid; idx < (QL - 1) * QC; idx += blockDim.x) {
-          const int i = idx / QC, c = idx % QC;
-          Zs[i * LZ + c] = Zs[(B + i) * LZ + c];
-        }
-      }
-      __syncthreads();
+    for (int kk = 0; kk < QR; kk += 8) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[(kk + lk) * LV + tr * 16 + lr],
+                                                  Zs[(kk + lk) * LZ + tc * 16 + lr], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[(kk + 4 + lk) * LV + tr * 16 + lr],
+                                                  Zs[(kk + 4 + lk) * LZ + tc * 16 + lr], acc1, 0, 0,
+                                                  0);
     }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + tc * 16 + lr] = acc0[r] + acc1[r];
+    lds_sync();
+    // W := T W
+    acc0 = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < QL; kk += 4)
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[(tr * 16 + lr) * LV + kk + lk],
+                                                  Ws[(kk + lk) * LZ + tc * 16 + lr], acc0, 0, 0, 0);
+    lds_sync();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + tc * 16 + lr] = acc0[r];
+    lds_sync();
+    // Z -= V W
+    {
+      v4d z[3];
+#pragma unroll
+      for (int t = 0; t < 3; ++t) z[t] = (v4d){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kk = 0; kk < QL; kk += 4) {
+        const double b = Ws[(kk + lk) * LZ + zc * 16 + lr];
+#pragma unroll
+        for (int t = 0; t < 3; ++t)
+          z[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[((zr0 + t) * 16 + lr) * LV + kk + lk], b,
+                                                      z[t], 0, 0, 0);
+      }
+#pragma unroll
+      for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Zs[((zr0 + t) * 16 + 4 * r + lk) * LZ + zc * 16 + lr] -= z[t][r];
+    }
+    lds_sync();
+    // rows that no later block of this group touches go back to Z; the
+    // last QL - 1 rows move to the top of the window
+    const int nout = last ? R : min(B, R);
+    for (int idx = tid; idx < nout * QC; idx += 256) {
+      const int i = idx / QC, c = idx % QC;
+      if (c < nc) Z[(rb + i) + (c0 + c) * n] = Zs[i * LZ + c];
+    }
+    if (nS2 < 0) break;
+    if (!last) {
+      lds_sync();  // the stores above have read rows 0 .. B-1
+      for (int idx = tid; idx < (QL - 1) * QC; idx += 256) {
+        const int i = idx / QC, c = idx % QC;
+        Zs[i * LZ + c] = Zs[(B + i) * LZ + c];  // rows B.. -> 0.. (QL - 1 < B: no overlap)
+      }
+      lds_sync();  // rows B.. are read before the next block's rows QL-1.. replace them
+      q2_regs_store(nk2, true, rv, rt, rz, Zs, Vs, Ts, tau_s);
+    } else {
+      __syncthreads();  // the next group's rows include rows these stores wrote
+      q2_regs_load(V2, n, nS2, 0, Tq, blk_off, true, Z, c0, nc, rv, rt, rz);
+      q2_regs_store(0, true, rv, rt, rz, Zs, Vs, Ts, tau_s);
+    }
+    lds_sync();
+    S = nS2;
+    k = nk2;
   }
 }
 
@@ -619,8 +692,9 @@ size_t band_kwork_doubles(int64_t n) {
   size_t mx = 8;
   for (int64_t k0 = 0; n - (k0 + B) >= 2; k0 += B) {
     const int m = (int)(n - k0 - B);
-    if ((m + 63) / 64 < 128) mx = std::max(mx, dgemm_splitk_work(m, B, m));
+    mx = std::max(mx, dgemm_splitk_work(m, B, m));
     mx = std::max(mx, dgemm_splitk_work(B, B, m));
+    mx = std::max(mx, dgemm_splitk_work(B, (int)n, m));  // the band back-transform
   }
   return mx / 8 + 1;
 }
@@ -660,11 +734,8 @@ hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs&
     double* A22 = A + r0 + r0 * n;
     double* V = w.U;
     double* Wb = w.U + B * ldu;
-    // X = A22 V
-    if ((m + 63) / 64 >= 128)
-      IMCOM_HIP_TRY(dgemm(st, false, false, m, B, m, 1.0, A22, n, V, ldu, 0.0, w.X, m));
-    else
-      IMCOM_HIP_TRY(dgemm_splitk(st, false, false, m, B, m, 1.0, A22, n, V, ldu, w.X, m, w.kwork));
+    // X = A22 V (split K: m / 64 tiles alone leave the CUs short of work)
+    IMCOM_HIP_TRY(dgemm_splitk(st, false, false, m, B, m, 1.0, A22, n, V, ldu, w.X, m, w.kwork));
     // Y = X T (into W's place)
     IMCOM_HIP_TRY(dgemm(st, false, false, m, B, B, 1.0, w.X, m, T1p, B, 0.0, Wb, ldu));
     // Mt = V^T Y, M = T^T Mt
@@ -717,7 +788,8 @@ hipError_t apply_q1(hipStream_t st, const double* A, int64_t n, const TwoStageBu
     const int m = (int)(n - r0);
     const double* Vt = A + k0 + r0 * n;  // B x m, leading dimension n
     const double* T1p = w.T1 + (size_t)p * B * B;
-    IMCOM_HIP_TRY(dgemm(st, false, false, B, (int)ncols, m, 1.0, Vt, n, Q + r0, n, 0.0, w.Y1, B));
+    IMCOM_HIP_TRY(
+        dgemm_splitk(st, false, false, B, (int)ncols, m, 1.0, Vt, n, Q + r0, n, w.Y1, B, w.kwork));
     IMCOM_HIP_TRY(dgemm(st, false, false, B, (int)ncols, B, 1.0, T1p, B, w.Y1, B, 0.0, w.Y2, B));
     IMCOM_HIP_TRY(dgemm(st, true, false, m, (int)ncols, B, -1.0, Vt, n, w.Y2, B, 1.0, Q + r0, n));
   }

@@ -39,6 +39,19 @@ __device__ __forceinline__ double readlane_d(double x, int lane) {
   const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+// Workgroup barrier for data exchanged through the LDS only: waits for the
+// wavefront's LDS operations (lgkmcnt 0; vmcnt and expcnt left at their
+// maxima) and meets the other wavefronts, without the vmcnt(0) wait that
+// __syncthreads' workgroup-scope fences imply, so global loads issued before
+// it stay in flight.  The empty asm statements keep the compiler from moving
+// memory accesses across it.
+__device__ __forceinline__ void lds_sync() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // Sum over the 64 lanes of a wavefront with DPP moves (a few cycles each,
 // against a round trip through the LDS unit for each step of a shuffle
 // butterfly): pairs, quads, half-rows and rows of 16 by the quad_perm and
