@@ -462,7 +462,7 @@ static int twostage_reduce(imcom_gpu_ctx* ctx, int64_t n) {
     StageTimer tm(ctx, PS_SYTRD);
     CK(band_reduce(ctx->stream, ctx->A.as<double>(), n, w), "band reduction");
     CK(bulge_chase(ctx->stream, ctx->A.as<double>(), n, ctx->Q.as<double>(),
-                   ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(), w.prog),
+                   ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(), w.msg, w.prog),
        "bulge chasing");
   }
   unsigned flags[2] = {0, 0};

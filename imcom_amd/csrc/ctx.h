@@ -129,7 +129,7 @@ struct EigWork {
 
 // Workspace of the two-stage reduction (sytrd2.hip).
 struct TwoStageWork {
-  DevBuf U, X, Mt, Mm, T1, xch, kwork, Y1, Y2, sync, prog;
+  DevBuf U, X, Mt, Mm, T1, xch, kwork, Y1, Y2, sync, prog, msg;
   // the bulge chase's back-transform: block offsets per sweep group (host
   // and device, for n_blk) and each block's triangular factor
   DevBuf blk_off, Tq;
@@ -147,7 +147,8 @@ struct TwoStageWork {
     IMCOM_HIP_TRY(Y1.reserve((size_t)n * b * d));
     IMCOM_HIP_TRY(Y2.reserve((size_t)n * b * d));
     IMCOM_HIP_TRY(sync.reserve(2 * 4));
-    IMCOM_HIP_TRY(prog.reserve((size_t)(n + 1) * 4));
+    IMCOM_HIP_TRY(prog.reserve(imcom::bulge_flag_words(n) * 4));
+    IMCOM_HIP_TRY(msg.reserve(imcom::bulge_msg_doubles(n) * d));
     w.U = U.as<double>();
     w.X = X.as<double>();
     w.Mt = Mt.as<double>();
@@ -159,6 +160,7 @@ struct TwoStageWork {
     w.Y2 = Y2.as<double>();
     w.sync = sync.as<unsigned>();
     w.prog = prog.as<unsigned>();
+    w.msg = msg.as<double>();
     return hipSuccess;
   }
   // block offsets and T factors of the bulge back-transform (uploaded on
@@ -176,7 +178,8 @@ struct TwoStageWork {
     return Tq.reserve(std::max<size_t>(nblocks, 1) * imcom::kQ2Group * imcom::kQ2Group * 8);
   }
   void release() {
-    DevBuf* b[] = {&U, &X, &Mt, &Mm, &T1, &xch, &kwork, &Y1, &Y2, &sync, &prog, &blk_off, &Tq};
+    DevBuf* b[] = {&U, &X, &Mt, &Mm, &T1, &xch, &kwork, &Y1, &Y2, &sync, &prog, &msg, &blk_off,
+                   &Tq};
     for (DevBuf* p : b) p->release();
     h_blk_off.clear();
     n_blk = -1;

@@ -275,7 +275,8 @@ struct TwoStageBufs {
   double* Y1 = nullptr;     // kBand x n
   double* Y2 = nullptr;     // kBand x n
   unsigned* sync = nullptr; // 2 words: panel barrier count, abort flag
-  unsigned* prog = nullptr; // n + 1 words: bulge-chasing progress per sweep, abort flag
+  unsigned* prog = nullptr; // bulge_flag_words(n): bulge-chasing message counters, abort flag
+  double* msg = nullptr;    // bulge_msg_doubles(n): bulge-chasing message rings
 };
 size_t band_xch_doubles(int64_t n);
 size_t band_kwork_doubles(int64_t n);
@@ -283,10 +284,12 @@ int band_panels(int64_t n);
 // Stage 1: the band in A's lower triangle (zeros below it), each panel's V^T
 // in the upper triangle rows of its columns, T per panel in w.T1.
 hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs& w);
-// Stage 2: d, e of the tridiagonal matrix; reflector (s, k) in column s of
-// V2 (n x n), rows 64k.. (tau in place of the unit first entry).
+// Stage 2 (n >= 3): d, e of the tridiagonal matrix; reflector (s, k) in
+// column s of V2 (n x n), rows 64k.. (tau in place of the unit first entry).
+size_t bulge_msg_doubles(int64_t n);
+size_t bulge_flag_words(int64_t n);
 hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double* d, double* e,
-                       unsigned* prog);
+                       double* msg, unsigned* flags);
 // flags[0]: a panel barrier timed out; flags[1]: a bulge-chasing wait timed out.
 hipError_t twostage_status(hipStream_t st, const TwoStageBufs& w, int64_t n, unsigned* flags);
 // Q2 blocks: kQ2Group consecutive sweeps at one task index, numbered group

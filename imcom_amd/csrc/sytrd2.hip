@@ -251,168 +251,238 @@ void panel_shape(int m, int* G, int* rpw) {
 
 // ---------------------------------------------------------------- stage 2
 
-__global__ __launch_bounds__(PT) void bulge_kernel(double* __restrict__ A, int64_t n,
-                                                   double* __restrict__ V2,
-                                                   unsigned* __restrict__ prog,
-                                                   unsigned spin_max) {
-  __shared__ double E[B][B + 1];  // rows r.., columns r-64.. (k > 0); column s (k = 0)
-  __shared__ double D[B][B + 1];  // the diagonal block, both triangles
-  __shared__ double vp[B], v[B], y[B], z[B], sc[2];
-  __shared__ int ok_sh;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int ri = tid >> 2, qq = tid & 3;  // row (or column) and quarter
-  const int64_t nsw = n - 2;
-  unsigned* abort_flag = prog + n;
-  for (int64_t s = blockIdx.x; s < nsw; s += gridDim.x) {
-    const int ntask = (int)((n - 2 - s) / B) + 1;
-    const int ntask_prev = s > 0 ? (int)((n - 1 - s) / B) + 1 : 0;
-    double taup = 0.0;
-    for (int k = 0; k < ntask; ++k) {
-      if (s > 0) {
-        const unsigned need = (unsigned)min(k + 2, ntask_prev);
-        if (tid == 0) {
-          int okv = 1;
-          unsigned it = 0;
-          while (__hip_atomic_load(&prog[s - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                 need) {
-            if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-                ++it > spin_max) {
-              __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              okv = 0;
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-          }
-          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-          ok_sh = okv;
-        }
-        __syncthreads();
-        if (!ok_sh) return;
-      }
-      const int64_t r = s + 1 + (int64_t)k * B;
-      const int L = (int)min<int64_t>(B, n - r);
-      {
-        // every load issued before the first LDS write: E (columns
-        // r-64+wv+4u, or column s for k = 0) and D (columns r+wv+4u, rows
-        // on or below the diagonal), row r+lane
-        double ev[16], dv[16];
-        const double* Ar = A + (r + lane);
-        if (k == 0) {
-          ev[0] = (wv == 0 && lane < L) ? Ar[s * n] : 0.0;
-        } else {
-#pragma unroll
-          for (int u = 0; u < 16; ++u)
-            ev[u] = lane < L ? Ar[(r - B + wv + 4 * u) * n] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int j = wv + 4 * u;
-          dv[u] = (lane >= j && lane < L) ? Ar[(r + j) * n] : 0.0;
-        }
-        if (k == 0) {
-          if (wv == 0) E[lane][0] = ev[0];
-        } else {
-#pragma unroll
-          for (int u = 0; u < 16; ++u) E[lane][wv + 4 * u] = ev[u];
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-          const int j = wv + 4 * u;
-          if (lane >= j) {
-            D[lane][j] = dv[u];
-            D[j][lane] = dv[u];
-          }
-        }
-      }
-      __syncthreads();
-      if (k > 0) {  // E := E (I - taup vp vp^T)
-        double t = 0.0;
-#pragma unroll
-        for (int j = 16 * qq; j < 16 * qq + 16; ++j) t += E[ri][j] * vp[j];
-        t = quad_sum_dpp(t) * taup;
-#pragma unroll
-        for (int j = 16 * qq; j < 16 * qq + 16; ++j) E[ri][j] -= t * vp[j];
-        __syncthreads();
-      }
-      if (wv == 0) {  // the reflector of E(0:L, 0)
-        const double x = lane < L ? E[lane][0] : 0.0;
-        const double s2 = wave_sum_dpp(lane >= 1 ? x * x : 0.0);
-        const double alpha = readlane_d(x, 0);
-        double tau, beta, scal;
-        if (L < 2 || s2 == 0.0) {
-          tau = 0.0;
-          beta = alpha;
-          scal = 0.0;
-        } else {
-          beta = -copysign(dlapy2_t(alpha, sqrt(s2)), alpha);
-          tau = (beta - alpha) / beta;
-          scal = 1.0 / (alpha - beta);
-        }
-        v[lane] = lane == 0 ? 1.0 : (lane < L ? x * scal : 0.0);
-        if (lane == 0) {
-          sc[0] = tau;
-          sc[1] = beta;
-        }
-      }
-      __syncthreads();
-      const double tau = sc[0], beta = sc[1];
-      if (k > 0) {  // E(:, 1:) := (I - tau v v^T) E(:, 1:); column 0 becomes beta e1
-        double t = 0.0;
-#pragma unroll
-        for (int i = 16 * qq; i < 16 * qq + 16; ++i) t += v[i] * E[i][ri];  // rows >= L are 0
-        t = quad_sum_dpp(t);
-        if (qq == 0) z[ri] = t;
-        __syncthreads();
-        const double tv = tau * v[ri];
-#pragma unroll
-        for (int j = 16 * qq; j < 16 * qq + 16; ++j)
-          E[ri][j] = j == 0 ? (ri == 0 ? beta : 0.0) : E[ri][j] - tv * z[j];
-      }
-      {  // p = tau D v
-        double t = 0.0;
-#pragma unroll
-        for (int j = 16 * qq; j < 16 * qq + 16; ++j) t += D[ri][j] * v[j];  // v_j = 0 for j >= L
-        t = quad_sum_dpp(t) * tau;
-        if (qq == 0) y[ri] = ri < L ? t : 0.0;
-      }
-      __syncthreads();
-      if (wv == 0) {  // w = p - 1/2 tau (p^T v) v
-        const double pv = wave_sum_dpp(y[lane] * v[lane]);
-        z[lane] = y[lane] - 0.5 * tau * pv * v[lane];
-      }
-      __syncthreads();
-      {  // D -= v w^T + w v^T (v, w vanish past L; the upper triangle is not read again)
-        const double vi = v[ri], wi = z[ri];
-#pragma unroll
-        for (int j = 16 * qq; j < 16 * qq + 16; ++j) D[ri][j] -= vi * z[j] + wi * v[j];
-      }
-      __syncthreads();
-      if (k == 0) {
-        if (tid < L) A[(r + tid) + s * n] = tid == 0 ? beta : 0.0;
-      } else {
-        for (int j = wv; j < B; j += 4)
-          if (lane < L) A[(r + lane) + (r - B + j) * n] = E[lane][j];
-      }
-      for (int j = wv; j < L; j += 4)
-        if (lane >= j && lane < L) A[(r + lane) + (r + j) * n] = D[lane][j];
-      if (tid < L) V2[s * n + (int64_t)k * B + tid] = tid == 0 ? tau : v[tid];
-      if (tid < B) vp[tid] = v[tid];
-      taup = tau;
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-      if (tid == 0)
-        __hip_atomic_store(&prog[s], (unsigned)(k + 1), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+// Workgroup k owns task index k of every sweep and keeps its window (rows
+// r..r+63, columns r-64..r+63, r = s+1+64k) in LDS as a circular array,
+// both triangles of the diagonal block: sliding from sweep s to s+1 drops
+// row r (sent to workgroup k-1, whose window it enters as the bottom row)
+// and takes row r+64 from workgroup k+1 (its top row after task (s, k+1));
+// the entries of the new row left of column r are zero (annihilated
+// columns).  Task (s, k) needs only the reflector of (s, k-1), sent by
+// workgroup k-1.  Messages go through rings of 4 slots per workgroup (a
+// producer is at most one sweep ahead of its consumers); each is published
+// by an agent-scope release store of a counter after its stores complete
+// and read after an acquire.  Workgroup 0's top rows are the tridiagonal
+// matrix: d and e come out as the window slides.
+constexpr int WLD = 65;  // LDS stride of a window column slot (64 row slots + 1)
+constexpr int BT = 512;  // threads per bulge-chasing workgroup: row ri = tid / 8, eighth tid % 8
+
+// Sum over each group of eight consecutive lanes (every lane of the group
+// holds it; partner sums commutative, so the order is fixed).
+__device__ __forceinline__ double oct_sum_dpp(double v) {
+  v += dpp_d<0xb1>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_d<0x4e>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_d<0x141>(v);  // row_half_mirror: the other quad of the eight
+  return v;
 }
 
-__global__ void band_diag_kernel(const double* __restrict__ A, int64_t n, double* __restrict__ d,
-                                 double* __restrict__ e) {
-  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  d[j] = A[j + j * n];
-  if (j + 1 < n) e[j] = A[(j + 1) + j * n];
+__global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__ A, int64_t n,
+                                                      double* __restrict__ V2,
+                                                      double* __restrict__ d,
+                                                      double* __restrict__ e,
+                                                      double* __restrict__ msg,
+                                                      unsigned* __restrict__ flags,
+                                                      unsigned spin_max) {
+  __shared__ double Wl[2 * B * WLD];
+  __shared__ double vp[B], v[B], y[B], z[B], w[B], sc[2];
+  __shared__ int ok_sh;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int ri = tid >> 3, q8 = tid & 7;
+  const int k = blockIdx.x, K = gridDim.x;
+  const int64_t s_last = min<int64_t>(n - 3, n - 2 - (int64_t)B * k);
+  unsigned* vflag = flags;
+  unsigned* rflag = flags + K;
+  unsigned* abort_flag = flags + 2 * K;
+  double* vmsg = msg;                             // [K][4][B + 1]: v, tau
+  double* rmsg = msg + (size_t)K * 4 * (B + 1);   // [K][4][B + 1]: row r, columns r-64..r
+  auto W = [&](int64_t i, int64_t j) -> double& {
+    return Wl[(int)(j & (2 * B - 1)) * WLD + (int)(i & (B - 1))];
+  };
+  // the window of sweep 0 (entries outside the matrix are zero; the upper
+  // triangle of the diagonal block mirrored from the lower)
+  {
+    const int64_t r = 1 + (int64_t)B * k;
+    for (int idx = tid; idx < 2 * B * B; idx += BT) {
+      const int jj = idx >> 6, ii = idx & 63;
+      const int64_t i = r + ii, j = r - B + jj;
+      double val = 0.0;
+      if (i < n && j >= 0 && j < n) val = j <= i ? A[i + j * n] : A[j + i * n];
+      W(i, j) = val;
+    }
+    if (k == 0 && tid == 0) d[0] = A[0];
+  }
+  __syncthreads();
+  for (int64_t s = 0; s <= s_last; ++s) {
+    const int64_t r = s + 1 + (int64_t)B * k;
+    const int L = (int)min<int64_t>(B, n - r);
+    const int64_t rn = r + B - 1;  // the window's bottom row
+    if (s > 0) {  // slide: row r-1 and column r-65 become row rn and column rn
+      if (tid < 2 * B) W(rn, r - B + tid) = 0.0;
+      if (tid < B) W(r + tid, rn) = 0.0;
+      __syncthreads();
+    }
+    const bool need_row = s > 0 && rn < n;
+    if (need_row || k > 0) {
+      if (tid == 0) {
+        int okv = 1;
+        unsigned it = 0;
+        while (okv && ((need_row && __hip_atomic_load(&rflag[k + 1], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT) < (unsigned)s) ||
+                       (k > 0 && __hip_atomic_load(&vflag[k - 1], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT) < (unsigned)(s + 1)))) {
+          if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+              ++it > spin_max) {
+            __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            okv = 0;
+          } else {
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        ok_sh = okv;
+      }
+      __syncthreads();
+      if (!ok_sh) return;
+      if (need_row && tid <= B) {  // row rn, columns rn-64 .. rn (mirrored into column rn)
+        const double val = rmsg[((size_t)(k + 1) * 4 + ((s - 1) & 3)) * (B + 1) + tid];
+        const int64_t c = rn - B + tid;
+        W(rn, c) = val;
+        if (c >= r && c < rn) W(c, rn) = val;
+      }
+      if (k > 0 && tid >= 2 * B && tid <= 3 * B) {
+        const double val = vmsg[((size_t)(k - 1) * 4 + (s & 3)) * (B + 1) + (tid - 2 * B)];
+        if (tid < 3 * B)
+          vp[tid - 2 * B] = val;
+        else
+          sc[0] = val;  // tau of (s, k-1)
+      }
+      __syncthreads();
+    }
+    const bool send_v = k + 1 < K;
+    double* vslot = vmsg + ((size_t)k * 4 + (s & 3)) * (B + 1);
+    if (k == 0) {  // the reflector of column s = r-1; published at once
+      if (wv == 0) {
+        const double x = lane < L ? W(r + lane, r - 1) : 0.0;
+        const double s2 = wave_sum_dpp(lane >= 1 ? x * x : 0.0);
+        const double alpha = readlane_d(x, 0);
+        double t = 0.0, bt = alpha, scal = 0.0;
+        if (L >= 2 && s2 != 0.0) {
+          bt = -copysign(dlapy2_t(alpha, sqrt(s2)), alpha);
+          t = (bt - alpha) / bt;
+          scal = 1.0 / (alpha - bt);
+        }
+        const double vl = lane == 0 ? 1.0 : (lane < L ? x * scal : 0.0);
+        v[lane] = vl;
+        if (lane < L) W(r + lane, r - 1) = lane == 0 ? bt : 0.0;
+        if (lane == 0) {
+          sc[0] = t;
+          sc[1] = bt;
+        }
+        if (send_v) {
+          vslot[lane] = vl;
+          if (lane == 0) vslot[B] = t;
+          __builtin_amdgcn_s_waitcnt(0);
+          if (lane == 0)
+            __hip_atomic_store(&vflag[k], (unsigned)(s + 1), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      __syncthreads();
+    } else {
+      {  // E := E (I - taup vp vp^T)
+        const double taup = sc[0];
+        double t = 0.0;
+#pragma unroll
+        for (int j = 8 * q8; j < 8 * q8 + 8; ++j) t += W(r + ri, r - B + j) * vp[j];
+        t = oct_sum_dpp(t) * taup;
+#pragma unroll
+        for (int j = 8 * q8; j < 8 * q8 + 8; ++j) W(r + ri, r - B + j) -= t * vp[j];
+      }
+      __syncthreads();
+      if (wv == 0) {  // the reflector of E(:, 0) (rows past L are zero); published at once
+        const double x = W(r + lane, r - B);
+        const double s2 = wave_sum_dpp(lane >= 1 ? x * x : 0.0);
+        const double alpha = readlane_d(x, 0);
+        double t = 0.0, bt = alpha, scal = 0.0;
+        if (L >= 2 && s2 != 0.0) {
+          bt = -copysign(dlapy2_t(alpha, sqrt(s2)), alpha);
+          t = (bt - alpha) / bt;
+          scal = 1.0 / (alpha - bt);
+        }
+        const double vl = lane == 0 ? 1.0 : (lane < L ? x * scal : 0.0);
+        v[lane] = vl;
+        if (lane == 0) {
+          sc[0] = t;
+          sc[1] = bt;
+        }
+        if (send_v) {
+          vslot[lane] = vl;
+          if (lane == 0) vslot[B] = t;
+          __builtin_amdgcn_s_waitcnt(0);
+          if (lane == 0)
+            __hip_atomic_store(&vflag[k], (unsigned)(s + 1), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      __syncthreads();
+    }
+    const double tau = sc[0], beta = sc[1];
+    // column sums of E against v (k > 0) and p = tau D v
+    if (k > 0) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 8 * q8; i < 8 * q8 + 8; ++i) t += v[i] * W(r + i, r - B + ri);
+      t = oct_sum_dpp(t);
+      if (q8 == 0) z[ri] = t;
+    }
+    {
+      double t = 0.0;
+#pragma unroll
+      for (int j = 8 * q8; j < 8 * q8 + 8; ++j) t += W(r + ri, r + j) * v[j];
+      t = oct_sum_dpp(t) * tau;
+      if (q8 == 0) y[ri] = t;
+    }
+    __syncthreads();
+    if (k > 0) {  // E(:, 1:) := (I - tau v v^T) E(:, 1:); column 0 becomes beta e1
+      const double tv = tau * v[ri];
+#pragma unroll
+      for (int j = 8 * q8; j < 8 * q8 + 8; ++j) {
+        double& x = W(r + ri, r - B + j);
+        x = j == 0 ? (ri == 0 ? beta : 0.0) : x - tv * z[j];
+      }
+    }
+    if (wv == 0) {  // w = p - 1/2 tau (p^T v) v
+      const double pv = wave_sum_dpp(y[lane] * v[lane]);
+      w[lane] = y[lane] - 0.5 * tau * pv * v[lane];
+    }
+    __syncthreads();
+    // row r is final once D(0, 0) is: send it to workgroup k-1 before the
+    // rest of the D update
+    if (k > 0 && wv == 0) {
+      double* rslot = rmsg + ((size_t)k * 4 + (s & 3)) * (B + 1);
+      rslot[lane] = W(r, r - B + lane);
+      if (lane == 0) rslot[B] = W(r, r) - (v[0] * w[0] + w[0] * v[0]);
+      __builtin_amdgcn_s_waitcnt(0);
+      if (lane == 0)
+        __hip_atomic_store(&rflag[k], (unsigned)(s + 1), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    {  // D -= v w^T + w v^T (both triangles; v, w vanish past L)
+      const double vi = v[ri], wi = w[ri];
+#pragma unroll
+      for (int j = 8 * q8; j < 8 * q8 + 8; ++j) W(r + ri, r + j) -= vi * w[j] + wi * v[j];
+    }
+    __syncthreads();
+    // outputs: the reflector and (workgroup 0) d, e
+    if (tid < L) V2[s * n + (int64_t)k * B + tid] = tid == 0 ? tau : v[tid];
+    if (k == 0 && tid == 0) {
+      e[s] = W(r, r - 1);
+      d[s + 1] = W(r, r);
+    }
+  }
+  if (k == 0 && tid == 0) {
+    e[n - 2] = W(n - 1, n - 2);
+    d[n - 1] = W(n - 1, n - 1);
+  }
 }
 
 // ------------------------------------------------- bulge back-transform
@@ -749,22 +819,31 @@ hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs&
   return hipSuccess;
 }
 
+size_t bulge_msg_doubles(int64_t n) {
+  const int64_t K = n >= 3 ? (n - 2) / B + 1 : 1;
+  return (size_t)2 * K * 4 * (B + 1);
+}
+
+size_t bulge_flag_words(int64_t n) {
+  const int64_t K = n >= 3 ? (n - 2) / B + 1 : 1;
+  return (size_t)2 * K + 1;
+}
+
 hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double* d, double* e,
-                       unsigned* prog) {
-  if (n >= 3) {
-    int dev = 0, ncu = 0;
-    IMCOM_HIP_TRY(hipGetDevice(&dev));
-    IMCOM_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    hipLaunchKernelGGL(band_zero_kernel, dim3(1), dim3(256), 0, st, prog, (int)n + 1);
-    IMCOM_HIP_TRY(hipGetLastError());
-    // LDS takes 67 KB of a workgroup, so two fit a CU: G <= 2 x CUs are all
-    // resident at once on an otherwise idle GPU
-    const int G = (int)std::min<int64_t>(n - 2, ncu);
-    hipLaunchKernelGGL(bulge_kernel, dim3(G), dim3(PT), 0, st, A, n, V2, prog, kSpinMax);
-    IMCOM_HIP_TRY(hipGetLastError());
-  }
-  hipLaunchKernelGGL(band_diag_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, A, n,
-                     d, e);
+                       double* msg, unsigned* flags) {
+  if (n < 3) return hipErrorInvalidValue;
+  const int K = (int)((n - 2) / B + 1);
+  int dev = 0, ncu = 0;
+  IMCOM_HIP_TRY(hipGetDevice(&dev));
+  IMCOM_HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  // 70 KB of LDS and eight wavefronts: two workgroups per CU, so K <= 2 x CUs
+  // workgroups are all resident on an otherwise idle GPU
+  if (K > 2 * ncu) return hipErrorInvalidValue;
+  const int nw = (int)bulge_flag_words(n);
+  hipLaunchKernelGGL(band_zero_kernel, dim3(1), dim3(256), 0, st, flags, nw);
+  IMCOM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(bulge_kernel, dim3(K), dim3(BT), 0, st, A, n, V2, d, e, msg, flags,
+                     kSpinMax);
   return hipGetLastError();
 }
 
@@ -774,7 +853,8 @@ hipError_t twostage_status(hipStream_t st, const TwoStageBufs& w, int64_t n, uns
       hipMemcpyAsync(&flags[0], w.sync + 1, 4, hipMemcpyDeviceToHost, st));
   flags[1] = 0;
   if (n >= 3)
-    IMCOM_HIP_TRY(hipMemcpyAsync(&flags[1], w.prog + n, 4, hipMemcpyDeviceToHost, st));
+    IMCOM_HIP_TRY(hipMemcpyAsync(&flags[1], w.prog + (bulge_flag_words(n) - 1), 4,
+                                 hipMemcpyDeviceToHost, st));
   return hipStreamSynchronize(st);
 }
 
