@@ -25,6 +25,10 @@ IMCOM_PREC_FP32X3 = 1
 IMCOM_PREC_BF16X6 = 2
 
 # every symbol declared in include/imcom_gpu.h
+# the library's default reduction (imcom_gpu_set_reduction_stages)
+REDUCTION_STAGES = 2
+TWOSTAGE_MIN = 14336
+
 EXPORTS = [
     "imcom_gpu_init", "imcom_gpu_finalize", "imcom_gpu_last_error", "imcom_gpu_synchronize",
     "imcom_gpu_set_tile", "imcom_gpu_set_grid", "imcom_gpu_lattice_state",
@@ -512,9 +516,10 @@ class ImcomGPU:
         (the library's default 2048; 0: every column on the blocked kernels)."""
         self._ck(self.lib.imcom_gpu_set_reduction_tail(self.h, max_rows), "set_reduction_tail")
 
-    def set_reduction_stages(self, stages=1, min_n=4096):
+    def set_reduction_stages(self, stages=REDUCTION_STAGES, min_n=TWOSTAGE_MIN):
         """Reduction to tridiagonal form: 1 the one-stage blocked kernels, 2
-        the two-stage reduction (dense -> band -> tridiagonal) for n >= min_n."""
+        the two-stage reduction (dense -> band -> tridiagonal) for n >= min_n
+        (the library's default: 2 from n = 14336 up, where it is faster)."""
         self._ck(self.lib.imcom_gpu_set_reduction_stages(self.h, stages, min_n),
                  "set_reduction_stages")
 

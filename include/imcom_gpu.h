@@ -284,11 +284,13 @@ int imcom_gpu_set_reduction_tail(imcom_gpu_ctx* ctx, int max_rows);
 
 /* Selects the reduction to tridiagonal form inside eigen_A / tridiagonalize
  * (the DSYTRD stage of DSYEVR, imcom_matrix.f90:642): stages = 1, the
- * one-stage blocked reduction (default); stages = 2, for n >= min_n, the
+ * one-stage blocked reduction; stages = 2, for n >= min_n, the
  * two-stage reduction (A to a band of width 64 by blocked QR panels with
  * matrix-product updates, then bulge chasing to tridiagonal form), with the
  * eigenvectors back-transformed through both stages.  Same outputs and
- * conventions either way; the eigenvalues agree to rounding. */
+ * conventions either way; the eigenvalues agree to rounding.  Default:
+ * stages = 2 with min_n = 14336 (the two-stage reduction where it was
+ * measured faster, the one-stage one below). */
 int imcom_gpu_set_reduction_stages(imcom_gpu_ctx* ctx, int stages, int64_t min_n);
 /* Diagnostic: the first stage of the two-stage reduction alone.  A (n >= 3)
  * is reduced in place and downloaded (n x n, column-major): its lower

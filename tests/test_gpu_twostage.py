@@ -1,5 +1,6 @@
 """The two-stage reduction to tridiagonal form (sytrd2.hip) through the C
-ABI, selected with imcom_gpu_set_reduction_stages(2, min_n): A to a band of
+ABI, selected with imcom_gpu_set_reduction_stages(2, min_n) (the default from
+n = 14336 up; the tests select it for every size): A to a band of
 width 64 by QR panels, the band to tridiagonal form by bulge chasing, and the
 eigenvectors back-transformed through both stages (the DSYTRD / DORMTR stages
 of DSYEVR, imcom_matrix.f90:642).
@@ -23,8 +24,10 @@ import numpy as np
 import pytest
 
 from imcom_amd import synth
+from imcom_amd.gpu import REDUCTION_STAGES, TWOSTAGE_MIN
 
 pytestmark = pytest.mark.gpu
+DEFAULT = (REDUCTION_STAGES, TWOSTAGE_MIN)
 EPS = 2.0 ** -53
 
 
@@ -60,7 +63,7 @@ def _tridiag_eigs(d, e):
 def twostage(gpu_ctx):
     gpu_ctx.set_reduction_stages(2, 3)
     yield gpu_ctx
-    gpu_ctx.set_reduction_stages(1, 4096)
+    gpu_ctx.set_reduction_stages(*DEFAULT)
 
 
 @pytest.mark.parametrize("n,kind", [(66, "spd"), (67, "spd"), (130, "graded"), (1000, "spd"),
@@ -110,7 +113,7 @@ def test_twostage_eigen(gpu_ctx, n, kind):
     nrm = np.abs(A).max()
     gpu_ctx.set_tile(_tile(n))
     try:
-        gpu_ctx.set_reduction_stages(1, 4096)
+        gpu_ctx.set_reduction_stages(1, TWOSTAGE_MIN)
         gpu_ctx.set_A(A)
         Q1, L1, _ = gpu_ctx.eigen_A()
         gpu_ctx.set_reduction_stages(2, 3)
@@ -119,7 +122,7 @@ def test_twostage_eigen(gpu_ctx, n, kind):
         gpu_ctx.set_A(A)
         Q3, L3, _ = gpu_ctx.eigen_A()
     finally:
-        gpu_ctx.set_reduction_stages(1, 4096)
+        gpu_ctx.set_reduction_stages(*DEFAULT)
     res = np.abs(A @ Q2 - Q2 * L2).max() / nrm
     orth = np.abs(Q2.T @ Q2 - np.eye(n)).max()
     ref = np.sort(np.linalg.eigvalsh(A))[::-1]
@@ -144,19 +147,20 @@ def test_twostage_eigen(gpu_ctx, n, kind):
 
 
 def test_twostage_below_threshold_is_one_stage(gpu_ctx):
-    """Sizes below min_n take the one-stage path: bitwise its results."""
+    """Sizes below min_n take the one-stage path: bitwise its results (the
+    default threshold leaves every tile below n = 14336 on it)."""
     n = 300
     print(f"[case] two-stage threshold n={n}", flush=True)
     rng = np.random.default_rng(5)
     A = _matrix(n, "spd", rng)
     gpu_ctx.set_tile(_tile(n))
     try:
-        gpu_ctx.set_reduction_stages(1, 4096)
+        gpu_ctx.set_reduction_stages(1, TWOSTAGE_MIN)
         gpu_ctx.set_A(A)
         Q1, L1, _ = gpu_ctx.eigen_A()
         gpu_ctx.set_reduction_stages(2, n + 1)
         gpu_ctx.set_A(A)
         Q2, L2, _ = gpu_ctx.eigen_A()
     finally:
-        gpu_ctx.set_reduction_stages(1, 4096)
+        gpu_ctx.set_reduction_stages(*DEFAULT)
     assert np.array_equal(Q1, Q2) and np.array_equal(L1, L2)
