@@ -31,9 +31,13 @@ three concurrent solves from three host threads (faster, but that form
 stalled silently in two test runs with no cause found, so it is not the
 default).
 
-The JSON line carries a roofline entry for the dominant kernel (sy_symv, the
-matrix-vector kernel of the tridiagonal reduction; HBM-bound; HIP events
-around each launch on the context's stream in one extra profiled step), with
+The JSON line carries a roofline entry for the dominant kernel, timed by
+HIP events that its dispatch records on the context's stream in one extra
+profiled step: with the two-stage reduction (the default from n = 14336 up,
+so for the large tile) q2_apply_kernel, the bulge back-transform (FP64
+MFMA-bound, 4 n ((n-1) n / 2 - 1) algorithmic flops); with the one-stage
+reduction sy_symv, its matrix-vector kernel (HBM-bound), or for the small
+tile the persistent tail kernel that reduces every column; with
 the per-stage rooflines of that step beside it, and a CPU baseline: the
 oracle's C/LAPACK restatement on this host's cores, on the whole tile for the
 small workload; for the large one, lookup_A and DSYEVR on the whole tile and
@@ -73,6 +77,16 @@ def symv_alg_bytes(n: int, launches: int | None = None) -> float:
     k = n - 1 if launches is None else launches
     s = np.arange(n - 1, n - 1 - k, -1, dtype=np.float64)
     return float(np.sum(s * (s + 1) / 2.0) * 8.0)
+
+
+def q2_alg_flops(n: int) -> float:
+    """Algorithmic flops of the two-stage reduction's bulge back-transform
+    (one launch of q2_apply_kernel): the bulge chase leaves one reflector of
+    length L for every task, and the tasks of sweep s cover rows s+1..n-1 in
+    pieces of at most 64, so the lengths of sweep s add up to n-1-s; each
+    entry is applied to the n columns of Z with 4 flops (a dot product and an
+    update): 4 n sum_{s=0}^{n-3} (n-1-s) = 4 n ((n-1) n / 2 - 1), about 2 n^3."""
+    return 4.0 * n * ((n - 1) * n / 2.0 - 1.0)
 
 
 def describe(workload: str, tiles) -> str:
@@ -303,6 +317,12 @@ def main():
                     help="single-tile eigendecompositions reduce their last columns (up to this "
                          "trailing size, at most 2048, the default) in the persistent kernel; "
                          "0: every column on the blocked kernels")
+    ap.add_argument("--stages", type=int, default=0, choices=[0, 1, 2],
+                    help="reduction to tridiagonal form: 1 one-stage, 2 two-stage (for every n), "
+                         "0 the library's default (two-stage from n = 14336 up)")
+    ap.add_argument("--eigen-only", action="store_true",
+                    help="build A and run one eigendecomposition (no timing events): the run "
+                         "q2_apply_kernel's FETCH_SIZE / WRITE_SIZE counters are collected on")
     ap.add_argument("--reduction-only", action="store_true",
                     help="single-tile workloads: build A and run its tridiagonal reduction once "
                          "(no timing events), print its device time and exit; the run that "
@@ -327,7 +347,7 @@ def main():
 
     import torch  # device plumbing only: barrier/synchronize around the timed region
     from imcom_amd import autotune, kappa_net, synth
-    from imcom_amd.gpu import IMCOM_PREC_BF16X6, IMCOM_PREC_FP64, ImcomGPU
+    from imcom_amd.gpu import IMCOM_PREC_BF16X6, IMCOM_PREC_FP64, TWOSTAGE_MIN, ImcomGPU
 
     t0 = time.time()
     if args.workload == "batch":
@@ -345,8 +365,21 @@ def main():
         g = ImcomGPU(0)
         autotune.apply(g, max(t.n for t in tiles))
         g.set_reduction_tail(args.tail_rows)
+        if args.stages:
+            g.set_reduction_stages(args.stages, 3)
         if weights is not None:
             g.mlp_set_params(weights)
+        if args.eigen_only:
+            g.set_tile(tile)
+            g.lookup_A(tile.npoly, tile.npad)
+            torch.cuda.synchronize()
+            tr = time.perf_counter()
+            g.eigen_A(out=False)
+            g.synchronize()
+            print(json.dumps({"eigen": args.workload, "n": tile.n,
+                              "seconds": round(time.perf_counter() - tr, 4)}), flush=True)
+            g.close()
+            return
         if args.reduction_only:
             g.set_tile(tile)
             g.lookup_A(tile.npoly, tile.npad)
@@ -533,6 +566,25 @@ def main():
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "alg_bytes_per_launch": round(per_launch),
                         "avg_launch_ms": round(avg_ms, 5), "launches": symv_n}
+        elif st["q2"][1] > 0:
+            # the two-stage reduction: its dominant kernel is the bulge
+            # back-transform's one launch, bound by the FP64 matrix cores
+            q2_ms, q2_n = st["q2"]
+            flops = q2_alg_flops(tile.n)
+            achieved = flops / (q2_ms * 1e-3) / 1e12
+            traffic = None
+            tfile = os.path.join(ROOT, "profiles", f"q2_apply_traffic_{args.workload}.json")
+            if os.path.exists(tfile):
+                with open(tfile) as f:
+                    traffic = json.load(f).get("hbm_bytes_per_launch")
+            roofline = {"bound": "mfma", "kernel": "q2_apply_kernel", "achieved": round(achieved, 2),
+                        "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                        "frac": round(achieved / FP64_PEAK_TFS, 4), "traffic": traffic,
+                        "alg_flops_per_launch": round(flops), "avg_launch_ms": round(q2_ms / q2_n, 3),
+                        "launches": q2_n,
+                        "work": "4 n ((n-1) n / 2 - 1) flops: the bulge chase's reflectors "
+                                "applied to the n columns of the eigenvectors",
+                        "bulge_chase_ms": round(st["bulge"][0], 2)}
         else:
             # the persistent tail reduced every column: its one launch reads
             # the trailing matrix once (8 n^2 B) and is bound by its per-column
@@ -635,7 +687,10 @@ def main():
     if pcie is not None:
         cfg["pcie_inclusive_px_per_s"] = round(m_total / pcie, 2)
     if args.workload != "batch":
-        cfg["reduction_tail_rows"] = args.tail_rows
+        two = args.stages == 2 or (args.stages == 0 and tile.n >= TWOSTAGE_MIN)
+        cfg["reduction"] = ("two-stage (dense -> band 64 -> tridiagonal)" if two
+                            else "one-stage blocked")
+        cfg["reduction_tail_rows"] = None if two else args.tail_rows
     if tables is not None:
         cfg["table_setup"] = tables
     if vendor is not None:

@@ -462,7 +462,7 @@ static int twostage_reduce(imcom_gpu_ctx* ctx, int64_t n) {
     StageTimer tm(ctx, PS_SYTRD);
     CK(band_reduce(ctx->stream, ctx->A.as<double>(), n, w), "band reduction");
     CK(bulge_chase(ctx->stream, ctx->A.as<double>(), n, ctx->Q.as<double>(),
-                   ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(), w.msg, w.prog),
+                   ctx->ew.sd.as<double>(), ctx->ew.se.as<double>(), w.msg, w.prog, &ctx->prof),
        "bulge chasing");
   }
   unsigned flags[2] = {0, 0};
@@ -499,7 +499,7 @@ static int twostage_eigen(imcom_gpu_ctx* ctx, int64_t n) {
     const int nS = q2_groups(n);
     const int maxk = nS > 0 ? ctx->tsw.h_blk_off[1] - ctx->tsw.h_blk_off[0] : 0;
     CK(apply_q2(ctx->stream, ctx->Q.as<double>(), n, ctx->tsw.blk_off.as<int>(), nS, maxk,
-                ctx->tsw.Tq.as<double>(), ctx->ew.sZ.as<double>()),
+                ctx->tsw.Tq.as<double>(), ctx->ew.sZ.as<double>(), &ctx->prof),
        "bulge back-transform");
     CK(launch_reverse_cols(ctx->stream, n, ctx->ew.sZ.as<double>(), ctx->Q.as<double>(),
                            ctx->ew.slam.as<double>(), ctx->L.as<double>()),

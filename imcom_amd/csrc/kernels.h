@@ -26,6 +26,8 @@ enum ProfSlot {
   PS_SYMV,      // sy_symv launches only (level 2)
   PS_SYR2K,     // trailing-update GEMMs only (level 2)
   PS_TABLES,    // device lookup-table set-up (build_Alookup / build_Blookup)
+  PS_BULGE,     // the bulge-chasing launch of the two-stage reduction (level 2)
+  PS_Q2,        // the bulge back-transform's apply launch (level 2)
   PS_NSLOTS
 };
 
@@ -289,7 +291,7 @@ hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs&
 size_t bulge_msg_doubles(int64_t n);
 size_t bulge_flag_words(int64_t n);
 hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double* d, double* e,
-                       double* msg, unsigned* flags);
+                       double* msg, unsigned* flags, Prof* prof = nullptr);
 // flags[0]: a panel barrier timed out; flags[1]: a bulge-chasing wait timed out.
 hipError_t twostage_status(hipStream_t st, const TwoStageBufs& w, int64_t n, unsigned* flags);
 // Q2 blocks: kQ2Group consecutive sweeps at one task index, numbered group
@@ -300,7 +302,7 @@ void q2_block_offsets(int64_t n, std::vector<int>& blk_off);
 // chase's reflectors (V2 as bulge_chase leaves it); Tq holds
 // blk_off[nS] x kQ2Group^2 doubles, maxk = the first group's block count.
 hipError_t apply_q2(hipStream_t st, const double* V2, int64_t n, const int* d_blk_off, int nS,
-                    int maxk, double* Tq, double* Z);
+                    int maxk, double* Tq, double* Z, Prof* prof = nullptr);
 // Q := Q1 Q (ncols columns of Q, leading dimension n).
 hipError_t apply_q1(hipStream_t st, const double* A, int64_t n, const TwoStageBufs& w, double* Q,
                     int64_t ncols);

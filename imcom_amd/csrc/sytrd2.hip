@@ -52,6 +52,7 @@
 // sweep touch disjoint rows; a later sweep's reflector with a higher task
 // index lies wholly below an earlier sweep's with a lower one).  Q1 is the
 // product of the panels' block reflectors, applied last panel first.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -830,7 +831,7 @@ size_t bulge_flag_words(int64_t n) {
 }
 
 hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double* d, double* e,
-                       double* msg, unsigned* flags) {
+                       double* msg, unsigned* flags, Prof* prof) {
   if (n < 3) return hipErrorInvalidValue;
   const int K = (int)((n - 2) / B + 1);
   int dev = 0, ncu = 0;
@@ -842,8 +843,10 @@ hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double*
   const int nw = (int)bulge_flag_words(n);
   hipLaunchKernelGGL(band_zero_kernel, dim3(1), dim3(256), 0, st, flags, nw);
   IMCOM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(bulge_kernel, dim3(K), dim3(BT), 0, st, A, n, V2, d, e, msg, flags,
-                     kSpinMax);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (prof) IMCOM_HIP_TRY(prof->events(PS_BULGE, 2, &e0, &e1));
+  hipExtLaunchKernelGGL(bulge_kernel, dim3(K), dim3(BT), 0, st, e0, e1, 0, A, n, V2, d, e, msg,
+                        flags, kSpinMax);
   return hipGetLastError();
 }
 
@@ -892,13 +895,15 @@ void q2_block_offsets(int64_t n, std::vector<int>& blk_off) {
 }
 
 hipError_t apply_q2(hipStream_t st, const double* V2, int64_t n, const int* d_blk_off, int nS,
-                    int maxk, double* Tq, double* Z) {
+                    int maxk, double* Tq, double* Z, Prof* prof) {
   if (nS <= 0) return hipSuccess;
   hipLaunchKernelGGL(q2_tfactor_kernel, dim3((unsigned)maxk, (unsigned)nS), dim3(256), 0, st, V2,
                      n, d_blk_off, nS, Tq);
   IMCOM_HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(q2_apply_kernel, dim3((unsigned)((n + QC - 1) / QC)), dim3(256), 0, st, V2,
-                     n, d_blk_off, nS, Tq, Z);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (prof) IMCOM_HIP_TRY(prof->events(PS_Q2, 2, &e0, &e1));
+  hipExtLaunchKernelGGL(q2_apply_kernel, dim3((unsigned)((n + QC - 1) / QC)), dim3(256), 0, st,
+                        e0, e1, 0, V2, n, d_blk_off, nS, Tq, Z);
   return hipGetLastError();
 }
 

@@ -70,7 +70,7 @@ class TileDesc(ctypes.Structure):
                 ("psfxscale", ctypes.c_double), ("psfyscale", ctypes.c_double)]
 
 STAGE_SLOTS = ["lookup_A", "sytrd", "stedc", "ormtr", "lookup_B", "build_P", "bisect",
-               "build_T", "maps", "symv", "syr2k", "tables"]
+               "build_T", "maps", "symv", "syr2k", "tables", "bulge", "q2"]
 
 
 class ImcomError(RuntimeError):

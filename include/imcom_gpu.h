@@ -314,13 +314,15 @@ int imcom_gpu_batch_result(imcom_gpu_ctx* ctx, int tile, double* K_out, double* 
 
 /* Device-time accounting (no reference counterpart; the reference prints
  * progress lines only).  level 0: off; 1: per-stage HIP-event times; 2: also
- * every launch of the tridiagonal reduction's matrix-vector kernel and its
- * trailing-update GEMMs.  Setting the level resets the counters.
+ * every launch of the one-stage reduction's matrix-vector kernel and its
+ * trailing-update GEMMs, and the two-stage reduction's bulge-chasing and
+ * bulge back-transform launches.  Setting the level resets the counters.
  * imcom_gpu_stage_times fills ms_out[i] / counts_out[i] for the slots
  *   0 lookup_A, 1 tridiagonal reduction, 2 divide-and-conquer,
  *   3 back-transform, 4 lookup_B, 5 build_P, 6 bisect_kappa, 7 build_T,
  *   8 maps (S, U, H), 9 matrix-vector launches, 10 trailing updates,
- *   11 device lookup-table set-up. */
+ *   11 device lookup-table set-up, 12 bulge chasing, 13 bulge back-transform
+ *   (nslots at most 14). */
 int imcom_gpu_set_profiling(imcom_gpu_ctx* ctx, int level);
 
 int imcom_gpu_stage_times(imcom_gpu_ctx* ctx, double* ms_out, int64_t* counts_out, int nslots);
