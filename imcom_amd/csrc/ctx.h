@@ -129,7 +129,7 @@ struct EigWork {
 
 // Workspace of the two-stage reduction (sytrd2.hip).
 struct TwoStageWork {
-  DevBuf U, X, Mt, Mm, T1, xch, kwork, Y1, Y2, sync, prog, msg;
+  DevBuf U, X, Mt, Mm, T1, xch, kwork, Y1, Y2, q1w, sync, prog, msg;
   // the bulge chase's back-transform: block offsets per sweep group (host
   // and device, for n_blk) and each block's triangular factor
   DevBuf blk_off, Tq;
@@ -144,8 +144,9 @@ struct TwoStageWork {
     IMCOM_HIP_TRY(T1.reserve((size_t)std::max(1, imcom::band_panels(n)) * b * b * d));
     IMCOM_HIP_TRY(xch.reserve(imcom::band_xch_doubles(n) * d));
     IMCOM_HIP_TRY(kwork.reserve(imcom::band_kwork_doubles(n) * d));
-    IMCOM_HIP_TRY(Y1.reserve((size_t)n * b * d));
-    IMCOM_HIP_TRY(Y2.reserve((size_t)n * b * d));
+    IMCOM_HIP_TRY(Y1.reserve((size_t)n * 4 * b * d));
+    IMCOM_HIP_TRY(Y2.reserve((size_t)n * 4 * b * d));
+    IMCOM_HIP_TRY(q1w.reserve(imcom::q1_group_work_doubles(n) * d));
     IMCOM_HIP_TRY(sync.reserve(2 * 4));
     IMCOM_HIP_TRY(prog.reserve(imcom::bulge_flag_words(n) * 4));
     IMCOM_HIP_TRY(msg.reserve(imcom::bulge_msg_doubles(n) * d));
@@ -158,6 +159,7 @@ struct TwoStageWork {
     w.kwork = kwork.as<double>();
     w.Y1 = Y1.as<double>();
     w.Y2 = Y2.as<double>();
+    w.q1work = q1w.as<double>();
     w.sync = sync.as<unsigned>();
     w.prog = prog.as<unsigned>();
     w.msg = msg.as<double>();
@@ -178,8 +180,8 @@ struct TwoStageWork {
     return Tq.reserve(std::max<size_t>(nblocks, 1) * imcom::kQ2Group * imcom::kQ2Group * 8);
   }
   void release() {
-    DevBuf* b[] = {&U, &X, &Mt, &Mm, &T1, &xch, &kwork, &Y1, &Y2, &sync, &prog, &msg, &blk_off,
-                   &Tq};
+    DevBuf* b[] = {&U, &X, &Mt, &Mm, &T1, &xch, &kwork, &Y1, &Y2, &q1w, &sync, &prog, &msg,
+                   &blk_off, &Tq};
     for (DevBuf* p : b) p->release();
     h_blk_off.clear();
     n_blk = -1;

@@ -274,8 +274,9 @@ struct TwoStageBufs {
   double* T1 = nullptr;     // band_panels(n) x kBand^2: the panels' triangular factors
   double* xch = nullptr;    // band_xch_doubles(n): the panel kernel's exchange slots
   double* kwork = nullptr;  // band_kwork_doubles(n): split-K partial products
-  double* Y1 = nullptr;     // kBand x n
-  double* Y2 = nullptr;     // kBand x n
+  double* Y1 = nullptr;     // 4 kBand x n
+  double* Y2 = nullptr;     // 4 kBand x n
+  double* q1work = nullptr; // q1_group_work_doubles(n): a panel group's V^T, T, G
   unsigned* sync = nullptr; // 2 words: panel barrier count, abort flag
   unsigned* prog = nullptr; // bulge_flag_words(n): bulge-chasing message counters, abort flag
   double* msg = nullptr;    // bulge_msg_doubles(n): bulge-chasing message rings
@@ -303,6 +304,7 @@ void q2_block_offsets(int64_t n, std::vector<int>& blk_off);
 // blk_off[nS] x kQ2Group^2 doubles, maxk = the first group's block count.
 hipError_t apply_q2(hipStream_t st, const double* V2, int64_t n, const int* d_blk_off, int nS,
                     int maxk, double* Tq, double* Z, Prof* prof = nullptr);
+size_t q1_group_work_doubles(int64_t n);
 // Q := Q1 Q (ncols columns of Q, leading dimension n).
 hipError_t apply_q1(hipStream_t st, const double* A, int64_t n, const TwoStageBufs& w, double* Q,
                     int64_t ncols);

@@ -765,7 +765,14 @@ size_t band_kwork_doubles(int64_t n) {
     const int m = (int)(n - k0 - B);
     mx = std::max(mx, dgemm_splitk_work(m, B, m));
     mx = std::max(mx, dgemm_splitk_work(B, B, m));
-    mx = std::max(mx, dgemm_splitk_work(B, (int)n, m));  // the band back-transform
+  }
+  // the band back-transform, in groups of 4 panels (fewer in the last)
+  const int np = band_panels(n);
+  for (int p0 = 0; p0 < np; p0 += 4) {
+    const int nbg = std::min(4, np - p0) * B;
+    const int m = (int)(n - (int64_t)p0 * B - B);
+    mx = std::max(mx, dgemm_splitk_work(nbg, nbg, m));
+    mx = std::max(mx, dgemm_splitk_work(nbg, (int)n, m));
   }
   return mx / 8 + 1;
 }
@@ -861,20 +868,86 @@ hipError_t twostage_status(hipStream_t st, const TwoStageBufs& w, int64_t n, uns
   return hipStreamSynchronize(st);
 }
 
+namespace {
+
+// Q1 in groups of kQ1Group panels: V^T of a group (nb_g x m, zeros above
+// each panel's first row) gathered from A's upper triangle, and the group's
+// T built from the panels' T by the block recurrence of compact WY
+// (T(0:i, i) = -T(0:i, 0:i) G(0:i, i) T_i, G = V^T V).
+constexpr int kQ1Group = 4;
+constexpr int QG = kQ1Group * B;  // reflectors per group at most
+
+__global__ void q1_gather_kernel(const double* __restrict__ A, int64_t n, int64_t p0, int nbg,
+                                 int64_t r0, int64_t m, double* __restrict__ Vt) {
+  const int64_t total = (int64_t)nbg * m;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(idx % nbg);
+    const int64_t x = idx / nbg;
+    const int off = (c / B) * B;  // the panel's first row within the group
+    Vt[idx] = x >= off ? A[(p0 * B + c) + (r0 + x) * n] : 0.0;
+  }
+}
+
+__global__ void q1_tinit_kernel(const double* __restrict__ T1, int64_t p0, int nbg,
+                                double* __restrict__ Tg) {
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nbg * nbg;
+       idx += gridDim.x * blockDim.x) {
+    const int i = idx % nbg, j = idx / nbg;
+    double v = 0.0;
+    if (i / B == j / B) v = T1[(size_t)(p0 + i / B) * B * B + (i % B) + (j % B) * B];
+    Tg[i + (size_t)j * nbg] = v;
+  }
+}
+
+}  // namespace
+
+size_t q1_group_work_doubles(int64_t n) {
+  // Vt (QG x n), T and G (QG x QG), X (QG x B)
+  return (size_t)QG * n + 2 * (size_t)QG * QG + (size_t)QG * B;
+}
+
 // Q := Q1 Q with Q1 the product of the panels' block reflectors (V^T in the
-// upper triangle of A, T per panel).
+// upper triangle of A, T per panel), last group of panels first.
 hipError_t apply_q1(hipStream_t st, const double* A, int64_t n, const TwoStageBufs& w, double* Q,
                     int64_t ncols) {
   const int np = band_panels(n);
-  for (int p = np - 1; p >= 0; --p) {
-    const int64_t k0 = (int64_t)p * B, r0 = k0 + B;
+  const int ng = (np + kQ1Group - 1) / kQ1Group;
+  double* Vt = w.q1work;
+  double* Tg = Vt + (size_t)QG * n;
+  double* G = Tg + (size_t)QG * QG;
+  double* X = G + (size_t)QG * QG;
+  for (int g = ng - 1; g >= 0; --g) {
+    const int p0 = g * kQ1Group, npan = std::min(kQ1Group, np - p0);
+    const int nbg = npan * B;
+    const int64_t r0 = (int64_t)p0 * B + B;
     const int m = (int)(n - r0);
-    const double* Vt = A + k0 + r0 * n;  // B x m, leading dimension n
-    const double* T1p = w.T1 + (size_t)p * B * B;
+    const int64_t total = (int64_t)nbg * m;
+    hipLaunchKernelGGL(q1_gather_kernel, dim3((unsigned)std::min<int64_t>((total + 255) / 256, 4096)),
+                       dim3(256), 0, st, A, n, (int64_t)p0, nbg, r0, (int64_t)m, Vt);
+    IMCOM_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(q1_tinit_kernel, dim3((nbg * nbg + 255) / 256), dim3(256), 0, st, w.T1,
+                       (int64_t)p0, nbg, Tg);
+    IMCOM_HIP_TRY(hipGetLastError());
+    if (npan > 1) {
+      // G = V^T V (the blocks above the diagonal are used)
+      IMCOM_HIP_TRY(dgemm_splitk(st, false, true, nbg, nbg, m, 1.0, Vt, nbg, Vt, nbg, G, nbg,
+                                 w.kwork));
+      for (int i = 1; i < npan; ++i) {
+        const int c0 = i * B;
+        IMCOM_HIP_TRY(dgemm(st, false, false, c0, B, c0, 1.0, Tg, nbg, G + (size_t)c0 * nbg, nbg,
+                            0.0, X, c0));
+        IMCOM_HIP_TRY(dgemm(st, false, false, c0, B, B, -1.0, X, c0,
+                            Tg + c0 + (size_t)c0 * nbg, nbg, 0.0, Tg + (size_t)c0 * nbg, nbg));
+      }
+    }
     IMCOM_HIP_TRY(
-        dgemm_splitk(st, false, false, B, (int)ncols, m, 1.0, Vt, n, Q + r0, n, w.Y1, B, w.kwork));
-    IMCOM_HIP_TRY(dgemm(st, false, false, B, (int)ncols, B, 1.0, T1p, B, w.Y1, B, 0.0, w.Y2, B));
-    IMCOM_HIP_TRY(dgemm(st, true, false, m, (int)ncols, B, -1.0, Vt, n, w.Y2, B, 1.0, Q + r0, n));
+        dgemm_splitk(st, false, false, nbg, (int)ncols, m, 1.0, Vt, nbg, Q + r0, n, w.Y1, nbg,
+                     w.kwork));
+    IMCOM_HIP_TRY(
+        dgemm(st, false, false, nbg, (int)ncols, nbg, 1.0, Tg, nbg, w.Y1, nbg, 0.0, w.Y2, nbg));
+    IMCOM_HIP_TRY(
+        dgemm(st, true, false, m, (int)ncols, nbg, -1.0, Vt, nbg, w.Y2, nbg, 1.0, Q + r0, n));
   }
   return hipSuccess;
 }
