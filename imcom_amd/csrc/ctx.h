@@ -177,7 +177,7 @@ struct TwoStageWork {
       n_blk = n;
     }
     const size_t nblocks = h_blk_off.empty() ? 0 : (size_t)h_blk_off.back();
-    return Tq.reserve(std::max<size_t>(nblocks, 1) * imcom::kQ2Group * imcom::kQ2Group * 8);
+    return Tq.reserve(std::max<size_t>(nblocks, 1) * imcom::kQ2Image * 8);
   }
   void release() {
     DevBuf* b[] = {&U, &X, &Mt, &Mm, &T1, &xch, &kwork, &Y1, &Y2, &q1w, &sync, &prog, &msg,

@@ -266,6 +266,9 @@ hipError_t sytrd_tail_status(hipStream_t st, int64_t S, const double* xch, unsig
 // tridiagonal form by bulge chasing, and the back-transforms.
 constexpr int kBand = 64;
 constexpr int kQ2Group = 32;  // reflectors per block of the bulge-chasing back-transform
+// doubles per block image of that back-transform: V (96 rows) and -T (32
+// rows), rows of kQ2Group + 1, padded to a multiple of 256
+constexpr int kQ2Image = 4352;
 struct TwoStageBufs {
   double* U = nullptr;      // n x 3 kBand: [V W V] of the current panel
   double* X = nullptr;      // n x kBand
@@ -300,10 +303,12 @@ hipError_t twostage_status(hipStream_t st, const TwoStageBufs& w, int64_t n, uns
 int q2_groups(int64_t n);
 void q2_block_offsets(int64_t n, std::vector<int>& blk_off);
 // Z := Q2 Z (n x n, leading dimension n), Q2 the product of the bulge
-// chase's reflectors (V2 as bulge_chase leaves it); Tq holds
-// blk_off[nS] x kQ2Group^2 doubles, maxk = the first group's block count.
-hipError_t apply_q2(hipStream_t st, const double* V2, int64_t n, const int* d_blk_off, int nS,
-                    int maxk, double* Tq, double* Z, Prof* prof = nullptr);
+// chase's reflectors (V2 as bulge_chase leaves it, n x n; overwritten: once
+// the reflectors are in the block images its storage holds Z^T for the
+// apply); img holds blk_off[nS] x kQ2Image doubles, maxk = the first
+// group's block count.
+hipError_t apply_q2(hipStream_t st, double* V2, int64_t n, const int* d_blk_off, int nS,
+                    int maxk, double* img, double* Z, Prof* prof = nullptr);
 size_t q1_group_work_doubles(int64_t n);
 // Q := Q1 Q (ncols columns of Q, leading dimension n).
 hipError_t apply_q1(hipStream_t st, const double* A, int64_t n, const TwoStageBufs& w, double* Q,

@@ -490,9 +490,14 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
 
 constexpr int QL = kQ2Group;       // reflectors per block (consecutive sweeps)
 constexpr int QR = 96;             // block rows (B + QL - 1 = 95, padded)
-constexpr int QC = 32;             // columns of Z per workgroup
-constexpr int LZ = QC + 8;         // LDS row stride of the Z window
 constexpr int LV = QL + 1;         // LDS row stride of V
+// A block's image, the apply kernel's LDS copy of it: V (QR x QL, unit
+// diagonal, zero outside the reflectors) in rows of stride LV, then -T
+// (QL x QL upper triangular) in rows of stride LV, zero padding.
+constexpr int QIMG = kQ2Image;
+constexpr int QVT = QR * LV;       // offset of -T in an image
+constexpr int QPL = QIMG / 256;    // image entries per thread of the apply kernel
+static_assert(QIMG % 256 == 0 && QVT + QL * LV <= QIMG, "Q2 block image layout");
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -529,7 +534,7 @@ __device__ __forceinline__ void q2_load_V(const double* __restrict__ V2, int64_t
 // block, blocks numbered group by group (blk_off[S] + k).
 __global__ __launch_bounds__(256) void q2_tfactor_kernel(const double* __restrict__ V2, int64_t n,
                                                          const int* __restrict__ blk_off, int nS,
-                                                         double* __restrict__ Tq) {
+                                                         double* __restrict__ img) {
   __shared__ double Vs[QR * LV];
   __shared__ double tau_s[QL];
   __shared__ double Gm[QL][QL + 1];
@@ -561,193 +566,221 @@ __global__ __launch_bounds__(256) void q2_tfactor_kernel(const double* __restric
     }
     __syncthreads();
   }
-  double* out = Tq + ((size_t)blk_off[S] + k) * QL * QL;
-  for (int idx = tid; idx < QL * QL; idx += blockDim.x) out[idx] = Ts[idx % QL][idx / QL];
-}
-
-// Z := Q2 Z for columns c0 .. c0+QC-1 of Z: blocks (S descending, k
-// ascending), each Z(rows, :) -= V (T (V^T Z(rows, :))) on FP64 MFMA, the
-// block's rows held in LDS; the last QL - 1 rows of one block are the first
-// of the next (k + 1), so only B new rows are read per block.  The next
-// block's V, T and new rows are loaded into registers at the start of a
-// block and written to the LDS at its end, so the loads run under the
-// block's products (its new rows lie below the current block's; at a change
-// of group they are read after the current block's stores instead).  The
-// LDS (70 KB) and registers (two waves per SIMD) let two workgroups share a
-// CU.
-__device__ __forceinline__ void q2_regs_load(const double* __restrict__ V2, int64_t n, int S,
-                                             int k, const double* __restrict__ Tq,
-                                             const int* __restrict__ blk_off, bool with_z,
-                                             const double* __restrict__ Z, int64_t c0, int nc,
-                                             double* rv, double* rt, double* rz) {
-  constexpr int NV = QR * QL / 256, NT = QL * QL / 256, NZ = QR * QC / 256;
-  const int tid = threadIdx.x;
-  const int64_t s0 = (int64_t)S * QL, nsw = n - 2;
-#pragma unroll
-  for (int u = 0; u < NV; ++u) {
-    const int idx = tid + 256 * u;
-    const int j = idx / QR, row = idx % QR, i = row - j;
-    const int64_t s = s0 + j;
+  // the block's image: V as the apply kernel's LDS holds it, then -T
+  double* out = img + ((size_t)blk_off[S] + k) * QIMG;
+  for (int idx = tid; idx < QIMG; idx += blockDim.x) {
     double v = 0.0;
-    if (s < nsw && k < q2_ntask(n, s) && i >= 0) {
-      const int L = (int)min<int64_t>(B, n - (s + 1 + (int64_t)k * B));
-      if (i < L) v = V2[s * n + (int64_t)k * B + i];
+    if (idx < QVT) {
+      if (idx % LV < QL) v = Vs[idx];
+    } else if (idx < QVT + QL * LV) {
+      const int r = (idx - QVT) / LV, c = (idx - QVT) % LV;
+      if (c < QL) v = -Ts[r][c];
     }
-    rv[u] = v;
-  }
-  const double* Tg = Tq + ((size_t)blk_off[S] + k) * QL * QL;
-#pragma unroll
-  for (int u = 0; u < NT; ++u) rt[u] = Tg[tid + 256 * u];
-  if (with_z) {
-    const int64_t rb = s0 + 1 + (int64_t)k * B;
-    const int R = (int)min<int64_t>(QR - 1, n - rb);
-    const int keep = k == 0 ? 0 : QL - 1;
-#pragma unroll
-    for (int u = 0; u < NZ; ++u) {
-      const int idx = tid + 256 * u;
-      const int i = keep + idx / QC, c = idx % QC;
-      double v = 0.0;
-      if (i < R && c < nc) v = Z[(rb + i) + (c0 + c) * n];
-      rz[u] = v;
-    }
+    out[idx] = v;
   }
 }
 
-__device__ __forceinline__ void q2_regs_store(int k, bool with_z, const double* rv,
-                                              const double* rt, const double* rz, double* Zs,
-                                              double* Vs, double* Ts, double* tau_s) {
-  constexpr int NV = QR * QL / 256, NT = QL * QL / 256, NZ = QR * QC / 256;
-  const int tid = threadIdx.x;
-  if (with_z) {
-    const int keep = k == 0 ? 0 : QL - 1;
-#pragma unroll
-    for (int u = 0; u < NZ; ++u) {
-      const int idx = tid + 256 * u;
-      const int i = keep + idx / QC, c = idx % QC;
-      if (i < QR) Zs[i * LZ + c] = rz[u];
-    }
-  }
-#pragma unroll
-  for (int u = 0; u < NV; ++u) {
-    const int idx = tid + 256 * u;
-    const int j = idx / QR, row = idx % QR;
-    double v = rv[u];
-    if (row == j) {
-      tau_s[j] = v;
-      v = v != 0.0 ? 1.0 : 0.0;  // an absent reflector reads 0 (tau 0)
-    }
-    Vs[row * LV + j] = v;
-  }
-#pragma unroll
-  for (int u = 0; u < NT; ++u) {
-    const int idx = tid + 256 * u;
-    Ts[(idx % QL) * LV + idx / QL] = rt[u];
-  }
+// Z := Q2 Z on Z stored by rows (Zt = Z^T column-major: Z(i, c) at
+// Zt[c + i n]), 64 columns of Z per workgroup: blocks (S descending, k
+// ascending), each Z(rows, :) += V (-T (V^T Z(rows, :))) on FP64 MFMA.
+// Wavefront w owns the 16-column stripe 16w.. and keeps its 96-row window
+// of Z in registers in the 16x16x4 MFMA's operand layout: lane l holds rows
+// 4s + l/16 (s = 0..23) of column l%16, which is both the B operand of K
+// step s and the accumulator layout of row tile s/4.  So W = V^T Z reads Z
+// from registers, -T W and Z + V W' read W and W' from the accumulators
+// that produced them, and the update of Z accumulates into Z itself; only
+// the block images (V and -T, 34 KB, double-buffered) go through the LDS,
+// with one barrier per block, and no part of Z moves between wavefronts.
+// The last QL - 1 rows of a block's window are the first of the next
+// (k + 1): the window shifts by B rows in each lane's registers and only the
+// new rows are read.  The next block's image and rows are loaded at the
+// start of a block, every load unconditional, and consumed at its end.
+// With Z stored by rows, one load or store of a wavefront covers four rows
+// of 16 consecutive entries: four whole 128-byte lines (stored by columns
+// it would touch 16 lines, and the vector cache's line rate would bound the
+// kernel).
+// Z is read and written through a buffer resource based at the block's
+// first row and the workgroup's first column, whose range ends with the
+// matrix: rows past the matrix read 0.  A load or store that must not
+// happen (columns past n - 1, rows not stored) is given an offset from
+// kQ2Drop on (past the range, whose size is capped at it), so every store
+// is issued without a branch: a store under a lane mask is not counted as
+// issued by the compiler's wait bookkeeping, which then waits for all
+// memory operations, stores included, before using any later load.
+constexpr int QS = 16;       // columns per wavefront
+constexpr int QCW = 4 * QS;  // columns per workgroup
+constexpr unsigned kQ2Drop = 0x80000000u;  // window offsets stay below it for n < 2 million
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double q2_ldz(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void q2_stz(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, (int)off, 0, 0);
+}
+// Buffer resource for Zt from row rb, column cw on to the end of the
+// matrix; base and size from scalar registers (a resource the compiler
+// cannot show uniform is used one lane value at a time).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t q2_zrsrc(double* Zt, int64_t n, int64_t rb,
+                                                          int64_t cw) {
+  const uint64_t bytes = ((uint64_t)(n - rb) * (uint64_t)n - (uint64_t)cw) * 8u;
+  const uint64_t p = (uint64_t)(Zt + rb * n + cw);
+  const uint64_t pu = ((uint64_t)__builtin_amdgcn_readfirstlane((unsigned)(p >> 32)) << 32) |
+                      (unsigned)__builtin_amdgcn_readfirstlane((unsigned)p);
+  const unsigned nb = bytes < kQ2Drop ? (unsigned)bytes : kQ2Drop;
+  return __builtin_amdgcn_make_buffer_rsrc((void*)pu, 0, (int)__builtin_amdgcn_readfirstlane(nb),
+                                           0x00020000);
 }
 
-// two waves per SIMD (registers capped at half the file), so that two
-// workgroups share a CU
-__global__ __launch_bounds__(256, 2) void q2_apply_kernel(const double* __restrict__ V2, int64_t n,
+__global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restrict__ img, int64_t n,
                                                           const int* __restrict__ blk_off, int nS,
-                                                          const double* __restrict__ Tq,
-                                                          double* __restrict__ Z) {
-  __shared__ double Zs[QR * LZ];
-  __shared__ double Vs[QR * LV];
-  __shared__ double Ws[QL * LZ];
-  __shared__ double Ts[QL * LV];
-  __shared__ double tau_s[QL];
-  constexpr int NV = QR * QL / 256, NT = QL * QL / 256, NZ = QR * QC / 256;
+                                                          double* __restrict__ Zt) {
+  __shared__ double Ls[2][QIMG];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
-  const int64_t c0 = (int64_t)blockIdx.x * QC;
-  const int nc = (int)min<int64_t>(QC, n - c0);
-  // wavefront tiles: W (32 x 32) tile (tr, tc); Z (96 x 32) column tile zc,
-  // row tiles 3 zr0 .. 3 zr0 + 2
-  const int tr = wv & 1, tc = wv >> 1, zc = wv & 1, zr0 = 3 * (wv >> 1);
+  const int64_t cw = (int64_t)blockIdx.x * QCW;
+  const bool colok = cw + QS * wv + lr < n;
   if (nS <= 0) return;
+  // byte offsets: the lane's column and window row lk from the resource's
+  // base (kQ2Drop on for a column past the matrix), four rows per step s
+  const unsigned zl = colok ? (unsigned)(((int64_t)lk * n + QS * wv + lr) * 8) : kQ2Drop;
+  const unsigned row4 = (unsigned)(32 * n);
+  v4d zt[QR / 16];  // window row 4s + lk in zt[s / 4][s % 4]
+  // the window of block (S, 0)
+  auto load_window = [&](int S_) {
+    const __amdgpu_buffer_rsrc_t zr = q2_zrsrc(Zt, n, (int64_t)S_ * QL + 1, cw);
+#pragma unroll
+    for (int s = 0; s < QR / 4; ++s) zt[s / 4][s % 4] = q2_ldz(zr, zl + s * row4);
+  };
+  const double* ib = img + tid;
+  int off = blk_off[nS - 1];
   {
-    double rv[NV], rt[NT], rz[NZ];
-    q2_regs_load(V2, n, nS - 1, 0, Tq, blk_off, true, Z, c0, nc, rv, rt, rz);
-    q2_regs_store(0, true, rv, rt, rz, Zs, Vs, Ts, tau_s);
+    double ri[QPL];
+    const double* p = ib + (size_t)off * QIMG;
+#pragma unroll
+    for (int u = 0; u < QPL; ++u) ri[u] = p[256 * u];
+    load_window(nS - 1);
+#pragma unroll
+    for (int u = 0; u < QPL; ++u) Ls[0][tid + 256 * u] = ri[u];
   }
-  lds_sync();
-  int S = nS - 1, k = 0;
-  while (S >= 0) {
-    const int64_t s0 = (int64_t)S * QL;
-    const int nk = blk_off[S + 1] - blk_off[S];
-    const int64_t rb = s0 + 1 + (int64_t)k * B;
+  __syncthreads();
+  int S = nS - 1, k = 0, buf = 0;
+  while (true) {
+    const int64_t rb = (int64_t)S * QL + 1 + (int64_t)k * B;
     const int R = (int)min<int64_t>(QR - 1, n - rb);
+    const int nk = q2_ntask(n, (int64_t)S * QL);
     const bool last = k + 1 == nk;
     const int nS2 = last ? S - 1 : S, nk2 = last ? 0 : k + 1;
-    // the next block's loads, in flight during this block's products
-    double rv[NV], rt[NT], rz[NZ];
-    if (nS2 >= 0) q2_regs_load(V2, n, nS2, nk2, Tq, blk_off, !last, Z, c0, nc, rv, rt, rz);
-    // W = V^T Z, two accumulation chains (even and odd steps)
-    v4d acc0 = (v4d){0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
-#pragma unroll
-    for (int kk = 0; kk < QR; kk += 8) {
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[(kk + lk) * LV + tr * 16 + lr],
-                                                  Zs[(kk + lk) * LZ + tc * 16 + lr], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[(kk + 4 + lk) * LV + tr * 16 + lr],
-                                                  Zs[(kk + 4 + lk) * LZ + tc * 16 + lr], acc1, 0, 0,
-                                                  0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + tc * 16 + lr] = acc0[r] + acc1[r];
-    lds_sync();
-    // W := T W
-    acc0 = (v4d){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int kk = 0; kk < QL; kk += 4)
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Ts[(tr * 16 + lr) * LV + kk + lk],
-                                                  Ws[(kk + lk) * LZ + tc * 16 + lr], acc0, 0, 0, 0);
-    lds_sync();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Ws[(tr * 16 + 4 * r + lk) * LZ + tc * 16 + lr] = acc0[r];
-    lds_sync();
-    // Z -= V W
+    const int off2 = last ? off - (S > 0 ? q2_ntask(n, (int64_t)(S - 1) * QL) : 0) : off;
+    const __amdgpu_buffer_rsrc_t zr = q2_zrsrc(Zt, n, rb, cw);
+    // prefetch: the next block's image (this block's again after the last)
+    // and the next window's new rows 31..95, B rows further down (4s + lk
+    // for s = 7..23; rows 28..30 of s = 7 are carried; read but not used
+    // after the last block of a group)
+    double ri[QPL], zn[QR / 4 - 7];
     {
-      v4d z[3];
+      const double* p = ib + (size_t)(nS2 >= 0 ? off2 + nk2 : off + k) * QIMG;
 #pragma unroll
-      for (int t = 0; t < 3; ++t) z[t] = (v4d){0.0, 0.0, 0.0, 0.0};
+      for (int u = 0; u < QPL; ++u) ri[u] = p[256 * u];
 #pragma unroll
-      for (int kk = 0; kk < QL; kk += 4) {
-        const double b = Ws[(kk + lk) * LZ + zc * 16 + lr];
+      for (int s = 7; s < QR / 4; ++s) zn[s - 7] = q2_ldz(zr, zl + (s + B / 4) * row4);
+    }
+    const double* Lv = Ls[buf];
+    const double* Lt = Ls[buf] + QVT;
+    // W = V^T Z: row tile 0 over window rows 0..79, tile 1 over rows 16..95
+    // (V's columns 16 c .. 16 c + 15 have entries in rows 16 c .. 16 c + 78)
+    v4d w0 = (v4d){0.0, 0.0, 0.0, 0.0}, w1 = w0;
 #pragma unroll
-        for (int t = 0; t < 3; ++t)
-          z[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[((zr0 + t) * 16 + lr) * LV + kk + lk], b,
-                                                      z[t], 0, 0, 0);
+    for (int s = 0; s < QR / 4; ++s) {
+      const double bz = zt[s / 4][s % 4];
+      if (s < 20)
+        w0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lv[(4 * s + lk) * LV + lr], bz, w0, 0, 0, 0);
+      if (s >= 4)
+        w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lv[(4 * s + lk) * LV + 16 + lr], bz, w1, 0, 0, 0);
+    }
+    // W' = -T W (upper triangular)
+    v4d u0 = (v4d){0.0, 0.0, 0.0, 0.0}, u1 = u0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[lr * LV + 4 * r + lk], w0[r], u0, 0, 0, 0);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[lr * LV + 16 + 4 * r + lk], w1[r], u0, 0, 0, 0);
+      u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[(16 + lr) * LV + 16 + 4 * r + lk], w1[r], u1, 0,
+                                                0, 0);
+    }
+    // Z += V W' (V's rows 0-15 have no entries past column 15, rows 80-95
+    // none before column 17)
+#pragma unroll
+    for (int t = 0; t < QR / 16; ++t)
+#pragma unroll
+      for (int q = 0; q < QL / 4; ++q) {
+        if ((t == 0 && q >= 4) || (t == QR / 16 - 1 && q < 4)) continue;
+        const double bw = q < 4 ? u0[q] : u1[q - 4];
+        zt[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lv[(16 * t + lr) * LV + 4 * q + lk], bw, zt[t],
+                                                     0, 0, 0);
       }
-#pragma unroll
-      for (int t = 0; t < 3; ++t)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Zs[((zr0 + t) * 16 + 4 * r + lk) * LZ + zc * 16 + lr] -= z[t][r];
-    }
-    lds_sync();
-    // rows that no later block of this group touches go back to Z; the
-    // last QL - 1 rows move to the top of the window
-    const int nout = last ? R : min(B, R);
-    for (int idx = tid; idx < nout * QC; idx += 256) {
-      const int i = idx / QC, c = idx % QC;
-      if (c < nc) Z[(rb + i) + (c0 + c) * n] = Zs[i * LZ + c];
-    }
-    if (nS2 < 0) break;
+    // rows no later block of this group touches go back to Z; within a
+    // group the window shifts first (consuming the prefetched rows and
+    // image) and the stores go last
+    double* Ln = Ls[buf ^ 1];
     if (!last) {
-      lds_sync();  // the stores above have read rows 0 .. B-1
-      for (int idx = tid; idx < (QL - 1) * QC; idx += 256) {
-        const int i = idx / QC, c = idx % QC;
-        Zs[i * LZ + c] = Zs[(B + i) * LZ + c];  // rows B.. -> 0.. (QL - 1 < B: no overlap)
-      }
-      lds_sync();  // rows B.. are read before the next block's rows QL-1.. replace them
-      q2_regs_store(nk2, true, rv, rt, rz, Zs, Vs, Ts, tau_s);
+      const v4d zs[4] = {zt[0], zt[1], zt[2], zt[3]};  // rows 0..63
+      zt[0] = zt[4];
+      zt[1] = zt[5];
+      if (lk == 3) zt[1][3] = zn[0];
+#pragma unroll
+      for (int s = 8; s < QR / 4; ++s) zt[s / 4][s % 4] = zn[s - 7];
+#pragma unroll
+      for (int u = 0; u < QPL; ++u) Ln[tid + 256 * u] = ri[u];
+      // the new window in registers before the stores are issued (a load
+      // still in flight behind them would be waited for with them)
+#pragma unroll
+      for (int t = 0; t < QR / 16; ++t) asm volatile("" : "+v"(zt[t]));
+      asm volatile("" ::: "memory");
+      const int nout = min(B, R) - lk;
+#pragma unroll
+      for (int s = 0; s < B / 4; ++s)
+        q2_stz(zr, (4 * s < nout ? zl : kQ2Drop) + s * row4, zs[s / 4][s % 4]);
     } else {
-      __syncthreads();  // the next group's rows include rows these stores wrote
-      q2_regs_load(V2, n, nS2, 0, Tq, blk_off, true, Z, c0, nc, rv, rt, rz);
-      q2_regs_store(0, true, rv, rt, rz, Zs, Vs, Ts, tau_s);
+      const int nout = R - lk;
+#pragma unroll
+      for (int s = 0; s < QR / 4; ++s)
+        q2_stz(zr, (4 * s < nout ? zl : kQ2Drop) + s * row4, zt[s / 4][s % 4]);
+      if (nS2 < 0) break;
+      // the next group's window holds rows this wavefront just stored
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_waitcnt(0);
+      load_window(nS2);
+#pragma unroll
+      for (int u = 0; u < QPL; ++u) Ln[tid + 256 * u] = ri[u];
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_waitcnt(0);  // no load left in flight across the loop's back edge
     }
-    lds_sync();
+    lds_sync();  // double buffering: the other image was last read before the previous barrier
+    buf ^= 1;
     S = nS2;
     k = nk2;
+    off = off2;
+  }
+}
+
+// dst = src^T, n x n column-major, 64 x 64 tiles through the LDS
+__global__ __launch_bounds__(256) void transpose_kernel(int64_t n, const double* __restrict__ src,
+                                                        double* __restrict__ dst) {
+  __shared__ double t[64][65];
+  const int64_t i0 = (int64_t)blockIdx.x * 64, j0 = (int64_t)blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int jj = ty + 4 * r;
+    if (i0 + tx < n && j0 + jj < n) t[jj][tx] = src[(i0 + tx) + (j0 + jj) * n];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int ii = ty + 4 * r;
+    if (j0 + tx < n && i0 + ii < n) dst[(j0 + tx) + (i0 + ii) * n] = t[tx][ii];
   }
 }
 
@@ -967,16 +1000,23 @@ void q2_block_offsets(int64_t n, std::vector<int>& blk_off) {
     blk_off[S + 1] = blk_off[S] + (int)((n - 2 - (int64_t)S * QL) / B) + 1;
 }
 
-hipError_t apply_q2(hipStream_t st, const double* V2, int64_t n, const int* d_blk_off, int nS,
-                    int maxk, double* Tq, double* Z, Prof* prof) {
+hipError_t apply_q2(hipStream_t st, double* V2, int64_t n, const int* d_blk_off, int nS,
+                    int maxk, double* img, double* Z, Prof* prof) {
   if (nS <= 0) return hipSuccess;
+  if (n >= (int64_t)1 << 21) return hipErrorInvalidValue;  // window offsets below kQ2Drop
   hipLaunchKernelGGL(q2_tfactor_kernel, dim3((unsigned)maxk, (unsigned)nS), dim3(256), 0, st, V2,
-                     n, d_blk_off, nS, Tq);
+                     n, d_blk_off, nS, img);
+  IMCOM_HIP_TRY(hipGetLastError());
+  // V2 is consumed: its storage holds Z by rows for the apply
+  const dim3 tg((unsigned)((n + 63) / 64), (unsigned)((n + 63) / 64));
+  hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, n, Z, V2);
   IMCOM_HIP_TRY(hipGetLastError());
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (prof) IMCOM_HIP_TRY(prof->events(PS_Q2, 2, &e0, &e1));
-  hipExtLaunchKernelGGL(q2_apply_kernel, dim3((unsigned)((n + QC - 1) / QC)), dim3(256), 0, st,
-                        e0, e1, 0, V2, n, d_blk_off, nS, Tq, Z);
+  hipExtLaunchKernelGGL(q2_apply_kernel, dim3((unsigned)((n + QCW - 1) / QCW)), dim3(256), 0, st,
+                        e0, e1, 0, img, n, d_blk_off, nS, V2);
+  IMCOM_HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(transpose_kernel, tg, dim3(256), 0, st, n, V2, Z);
   return hipGetLastError();
 }
 
