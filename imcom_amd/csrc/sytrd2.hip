@@ -496,8 +496,9 @@ constexpr int LV = QL + 1;         // LDS row stride of V
 // (QL x QL upper triangular) in rows of stride LV, zero padding.
 constexpr int QIMG = kQ2Image;
 constexpr int QVT = QR * LV;       // offset of -T in an image
-constexpr int QPL = QIMG / 256;    // image entries per thread of the apply kernel
-static_assert(QIMG % 256 == 0 && QVT + QL * LV <= QIMG, "Q2 block image layout");
+constexpr int QPL = QIMG / 512;    // image loads of two doubles per thread of the apply kernel
+static_assert(QIMG % 512 == 0 && QVT + QL * LV <= QIMG, "Q2 block image layout");
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -652,17 +653,21 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
 #pragma unroll
     for (int s = 0; s < QR / 4; ++s) zt[s / 4][s % 4] = q2_ldz(zr, zl + s * row4);
   };
-  const double* ib = img + tid;
+  // thread tid moves image entries 2 tid + 512 u, +1 (16 bytes)
+  const v2d* ib = reinterpret_cast<const v2d*>(img) + tid;
   int off = blk_off[nS - 1];
   {
-    double ri[QPL];
-    const double* p = ib + (size_t)off * QIMG;
+    v2d ri[QPL];
+    const v2d* p = ib + (size_t)off * (QIMG / 2);
 #pragma unroll
     for (int u = 0; u < QPL; ++u) ri[u] = p[256 * u];
     load_window(nS - 1);
+    v2d* L0 = reinterpret_cast<v2d*>(Ls[0]) + tid;
 #pragma unroll
-    for (int u = 0; u < QPL; ++u) Ls[0][tid + 256 * u] = ri[u];
+    for (int u = 0; u < QPL; ++u) L0[256 * u] = ri[u];
   }
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);  // no load left in flight into the loop
   __syncthreads();
   int S = nS - 1, k = 0, buf = 0;
   while (true) {
@@ -677,9 +682,10 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
     // and the next window's new rows 31..95, B rows further down (4s + lk
     // for s = 7..23; rows 28..30 of s = 7 are carried; read but not used
     // after the last block of a group)
-    double ri[QPL], zn[QR / 4 - 7];
+    v2d ri[QPL];
+    double zn[QR / 4 - 7];
     {
-      const double* p = ib + (size_t)(nS2 >= 0 ? off2 + nk2 : off + k) * QIMG;
+      const v2d* p = ib + (size_t)(nS2 >= 0 ? off2 + nk2 : off + k) * (QIMG / 2);
 #pragma unroll
       for (int u = 0; u < QPL; ++u) ri[u] = p[256 * u];
 #pragma unroll
@@ -710,9 +716,13 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
                                                 0, 0);
     }
     // Z += V W' (V's rows 0-15 have no entries past column 15, rows 80-95
-    // none before column 17)
+    // none before column 17), tile by tile; rows 0..63 are final after
+    // their tile (no later block of the group touches them) and go back to
+    // Z at once, under the next tiles' products (rows past the block's
+    // rows, nout on, are not stored)
+    const int nout = (last ? R : min(B, R)) - lk;
 #pragma unroll
-    for (int t = 0; t < QR / 16; ++t)
+    for (int t = 0; t < QR / 16; ++t) {
 #pragma unroll
       for (int q = 0; q < QL / 4; ++q) {
         if ((t == 0 && q >= 4) || (t == QR / 16 - 1 && q < 4)) continue;
@@ -720,32 +730,31 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
         zt[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lv[(16 * t + lr) * LV + 4 * q + lk], bw, zt[t],
                                                      0, 0, 0);
       }
-    // rows no later block of this group touches go back to Z; within a
-    // group the window shifts first (consuming the prefetched rows and
-    // image) and the stores go last
-    double* Ln = Ls[buf ^ 1];
+      if (t < B / 16) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          q2_stz(zr, (4 * (4 * t + r) < nout ? zl : kQ2Drop) + (4 * t + r) * row4, zt[t][r]);
+      }
+    }
+    // within a group the window shifts (rows 64.. to the top, the
+    // prefetched rows below them); after a group's last block its other
+    // rows go back to Z and the next group's window is read
+    v2d* Ln = reinterpret_cast<v2d*>(Ls[buf ^ 1]) + tid;
     if (!last) {
-      const v4d zs[4] = {zt[0], zt[1], zt[2], zt[3]};  // rows 0..63
       zt[0] = zt[4];
       zt[1] = zt[5];
       if (lk == 3) zt[1][3] = zn[0];
 #pragma unroll
       for (int s = 8; s < QR / 4; ++s) zt[s / 4][s % 4] = zn[s - 7];
 #pragma unroll
-      for (int u = 0; u < QPL; ++u) Ln[tid + 256 * u] = ri[u];
-      // the new window in registers before the stores are issued (a load
-      // still in flight behind them would be waited for with them)
+      for (int u = 0; u < QPL; ++u) Ln[256 * u] = ri[u];
+      // the new window in registers within this block (left to the next,
+      // its moves would wait there for this block's stores too)
 #pragma unroll
       for (int t = 0; t < QR / 16; ++t) asm volatile("" : "+v"(zt[t]));
-      asm volatile("" ::: "memory");
-      const int nout = min(B, R) - lk;
-#pragma unroll
-      for (int s = 0; s < B / 4; ++s)
-        q2_stz(zr, (4 * s < nout ? zl : kQ2Drop) + s * row4, zs[s / 4][s % 4]);
     } else {
-      const int nout = R - lk;
 #pragma unroll
-      for (int s = 0; s < QR / 4; ++s)
+      for (int s = B / 4; s < QR / 4; ++s)
         q2_stz(zr, (4 * s < nout ? zl : kQ2Drop) + s * row4, zt[s / 4][s % 4]);
       if (nS2 < 0) break;
       // the next group's window holds rows this wavefront just stored
@@ -753,7 +762,7 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
       __builtin_amdgcn_s_waitcnt(0);
       load_window(nS2);
 #pragma unroll
-      for (int u = 0; u < QPL; ++u) Ln[tid + 256 * u] = ri[u];
+      for (int u = 0; u < QPL; ++u) Ln[256 * u] = ri[u];
       asm volatile("" ::: "memory");
       __builtin_amdgcn_s_waitcnt(0);  // no load left in flight across the loop's back edge
     }
