@@ -704,6 +704,14 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
       if (s >= 4)
         w1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lv[(4 * s + lk) * LV + 16 + lr], bz, w1, 0, 0, 0);
     }
+    // the prefetch loads spread over the products of W, one after each
+    // (issued together they queue in the vector memory unit while the
+    // wavefront waits to issue them)
+#pragma unroll
+    for (int i = 0; i < QPL + QR / 4 - 7; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one vector memory read
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
+    }
     // W' = -T W (upper triangular)
     v4d u0 = (v4d){0.0, 0.0, 0.0, 0.0}, u1 = u0;
 #pragma unroll
