@@ -33,7 +33,7 @@ default).
 
 The JSON line carries a roofline entry for the dominant kernel, timed by
 HIP events that its dispatch records on the context's stream in one extra
-profiled step: with the two-stage reduction (the default from n = 14336 up,
+profiled step: with the two-stage reduction (the default from n = 7168 up,
 so for the large tile) q2_apply_kernel, the bulge back-transform (FP64
 MFMA-bound, 4 n ((n-1) n / 2 - 1) algorithmic flops); with the one-stage
 reduction sy_symv, its matrix-vector kernel (HBM-bound), or for the small
@@ -319,7 +319,7 @@ def main():
                          "0: every column on the blocked kernels")
     ap.add_argument("--stages", type=int, default=0, choices=[0, 1, 2],
                     help="reduction to tridiagonal form: 1 one-stage, 2 two-stage (for every n), "
-                         "0 the library's default (two-stage from n = 14336 up)")
+                         "0 the library's default (two-stage from n = 7168 up)")
     ap.add_argument("--eigen-only", action="store_true",
                     help="build A and run one eigendecomposition (no timing events): the run "
                          "q2_apply_kernel's FETCH_SIZE / WRITE_SIZE counters are collected on")

@@ -215,10 +215,10 @@ struct imcom_gpu_ctx {
   int tail_rows = imcom::kTailMaxRows;
   // 1: one-stage reduction; 2: two-stage (dense -> band -> tridiagonal) for
   // n >= twostage_min (imcom_gpu_set_reduction_stages).  By default the
-  // two-stage reduction serves n >= 14336, where it was measured faster
+  // two-stage reduction serves n >= 7168, where it was measured faster
   // (n = 16384: 1.52 against 1.63 s; n = 12288: 0.83 against 0.80 s).
   int stages = 2;
-  int64_t twostage_min = 14336;
+  int64_t twostage_min = 7168;
   TwoStageWork tsw;
   std::vector<int32_t> h_exp;
   // tables

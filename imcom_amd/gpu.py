@@ -27,7 +27,7 @@ IMCOM_PREC_BF16X6 = 2
 # every symbol declared in include/imcom_gpu.h
 # the library's default reduction (imcom_gpu_set_reduction_stages)
 REDUCTION_STAGES = 2
-TWOSTAGE_MIN = 14336
+TWOSTAGE_MIN = 7168
 
 EXPORTS = [
     "imcom_gpu_init", "imcom_gpu_finalize", "imcom_gpu_last_error", "imcom_gpu_synchronize",
@@ -519,7 +519,7 @@ class ImcomGPU:
     def set_reduction_stages(self, stages=REDUCTION_STAGES, min_n=TWOSTAGE_MIN):
         """Reduction to tridiagonal form: 1 the one-stage blocked kernels, 2
         the two-stage reduction (dense -> band -> tridiagonal) for n >= min_n
-        (the library's default: 2 from n = 14336 up, where it is faster)."""
+        (the library's default: 2 from n = 7168 up, where it is faster)."""
         self._ck(self.lib.imcom_gpu_set_reduction_stages(self.h, stages, min_n),
                  "set_reduction_stages")
 

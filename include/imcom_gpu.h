@@ -289,7 +289,7 @@ int imcom_gpu_set_reduction_tail(imcom_gpu_ctx* ctx, int max_rows);
  * matrix-product updates, then bulge chasing to tridiagonal form), with the
  * eigenvectors back-transformed through both stages.  Same outputs and
  * conventions either way; the eigenvalues agree to rounding.  Default:
- * stages = 2 with min_n = 14336 (the two-stage reduction where it was
+ * stages = 2 with min_n = 7168 (the two-stage reduction where it was
  * measured faster, the one-stage one below). */
 int imcom_gpu_set_reduction_stages(imcom_gpu_ctx* ctx, int stages, int64_t min_n);
 /* Diagnostic: the first stage of the two-stage reduction alone.  A (n >= 3)

@@ -1,6 +1,6 @@
 """The two-stage reduction to tridiagonal form (sytrd2.hip) through the C
 ABI, selected with imcom_gpu_set_reduction_stages(2, min_n) (the default from
-n = 14336 up; the tests select it for every size): A to a band of
+n = 7168 up; the tests select it for every size): A to a band of
 width 64 by QR panels, the band to tridiagonal form by bulge chasing, and the
 eigenvectors back-transformed through both stages (the DSYTRD / DORMTR stages
 of DSYEVR, imcom_matrix.f90:642).
@@ -156,7 +156,7 @@ def test_twostage_eigen(gpu_ctx, n, kind):
 
 def test_twostage_below_threshold_is_one_stage(gpu_ctx):
     """Sizes below min_n take the one-stage path: bitwise its results (the
-    default threshold leaves every tile below n = 14336 on it)."""
+    default threshold leaves every tile below n = 7168 on it)."""
     n = 300
     print(f"[case] two-stage threshold n={n}", flush=True)
     rng = np.random.default_rng(5)
