@@ -37,11 +37,9 @@
 // r = s+1+64k .. r+63 and columns r-64 .. r+63 only; the task (s, k) may
 // start once (s, k-1) and (s-1, k+1) are done (or all of sweep s-1 when it
 // has fewer tasks), and any two tasks not ordered by these rules touch
-// disjoint entries.  One persistent launch: workgroup g takes sweeps
-// g, g + G, ... in order, and publishes each finished task in a progress
-// counter per sweep (agent-scope release store after its stores complete;
-// the waiter acquires before it reads).  The band and its fill live in
-// column-major A itself (rows j..j+127 of column j).
+// disjoint entries.  One persistent launch with one workgroup per task
+// index; the band is read from column-major A once, and each workgroup's
+// part of it stays in its LDS (below).
 //
 // Back-transform: the eigenvectors Z of the tridiagonal matrix become
 // Q1 Q2 Z.  Q2 = H(0,0) H(0,1) ... (creation order) is applied in blocks of
@@ -260,10 +258,10 @@ void panel_shape(int m, int* G, int* rpw) {
 // the entries of the new row left of column r are zero (annihilated
 // columns).  Task (s, k) needs only the reflector of (s, k-1), sent by
 // workgroup k-1.  Messages go through rings of 4 slots per workgroup (a
-// producer is at most one sweep ahead of its consumers); each is published
-// by an agent-scope release store of a counter after its stores complete
-// and read after an acquire.  Workgroup 0's top rows are the tridiagonal
-// matrix: d and e come out as the window slides.
+// producer is at most one sweep ahead of its consumers); a message's
+// entries are written and read with agent-scope atomics and its counter is
+// set once those stores have completed.  Workgroup 0's top rows are the
+// tridiagonal matrix: d and e come out as the window slides.
 constexpr int WLD = 65;  // LDS stride of a window column slot (64 row slots + 1)
 constexpr int BT = 512;  // threads per bulge-chasing workgroup: row ri = tid / 8, eighth tid % 8
 
@@ -274,6 +272,17 @@ __device__ __forceinline__ double oct_sum_dpp(double v) {
   v += dpp_d<0x4e>(v);   // quad_perm [2, 3, 0, 1]
   v += dpp_d<0x141>(v);  // row_half_mirror: the other quad of the eight
   return v;
+}
+
+// A message's flag: set by lane 0 once the wavefront's coherent stores of
+// the message have completed (relaxed: the message's entries are written
+// and read with agent-scope atomics, which are coherent across the XCDs
+// without the L2 write-back and invalidation of a release/acquire pair).
+__device__ __forceinline__ void publish(unsigned* flag, unsigned value, int lane) {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0);
+  asm volatile("" ::: "memory");
+  if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__ A, int64_t n,
@@ -338,19 +347,18 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
             __builtin_amdgcn_s_sleep(1);
           }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         ok_sh = okv;
       }
       __syncthreads();
       if (!ok_sh) return;
       if (need_row && tid <= B) {  // row rn, columns rn-64 .. rn (mirrored into column rn)
-        const double val = rmsg[((size_t)(k + 1) * 4 + ((s - 1) & 3)) * (B + 1) + tid];
+        const double val = ld_coh(&rmsg[((size_t)(k + 1) * 4 + ((s - 1) & 3)) * (B + 1) + tid]);
         const int64_t c = rn - B + tid;
         W(rn, c) = val;
         if (c >= r && c < rn) W(c, rn) = val;
       }
       if (k > 0 && tid >= 2 * B && tid <= 3 * B) {
-        const double val = vmsg[((size_t)(k - 1) * 4 + (s & 3)) * (B + 1) + (tid - 2 * B)];
+        const double val = ld_coh(&vmsg[((size_t)(k - 1) * 4 + (s & 3)) * (B + 1) + (tid - 2 * B)]);
         if (tid < 3 * B)
           vp[tid - 2 * B] = val;
         else
@@ -379,12 +387,9 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
           sc[1] = bt;
         }
         if (send_v) {
-          vslot[lane] = vl;
-          if (lane == 0) vslot[B] = t;
-          __builtin_amdgcn_s_waitcnt(0);
-          if (lane == 0)
-            __hip_atomic_store(&vflag[k], (unsigned)(s + 1), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+          st_coh(&vslot[lane], vl);
+          if (lane == 0) st_coh(&vslot[B], t);
+          publish(&vflag[k], (unsigned)(s + 1), lane);
         }
       }
       __syncthreads();
@@ -416,12 +421,9 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
           sc[1] = bt;
         }
         if (send_v) {
-          vslot[lane] = vl;
-          if (lane == 0) vslot[B] = t;
-          __builtin_amdgcn_s_waitcnt(0);
-          if (lane == 0)
-            __hip_atomic_store(&vflag[k], (unsigned)(s + 1), __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+          st_coh(&vslot[lane], vl);
+          if (lane == 0) st_coh(&vslot[B], t);
+          publish(&vflag[k], (unsigned)(s + 1), lane);
         }
       }
       __syncthreads();
@@ -460,12 +462,9 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
     // rest of the D update
     if (k > 0 && wv == 0) {
       double* rslot = rmsg + ((size_t)k * 4 + (s & 3)) * (B + 1);
-      rslot[lane] = W(r, r - B + lane);
-      if (lane == 0) rslot[B] = W(r, r) - (v[0] * w[0] + w[0] * v[0]);
-      __builtin_amdgcn_s_waitcnt(0);
-      if (lane == 0)
-        __hip_atomic_store(&rflag[k], (unsigned)(s + 1), __ATOMIC_RELEASE,
-                           __HIP_MEMORY_SCOPE_AGENT);
+      st_coh(&rslot[lane], W(r, r - B + lane));
+      if (lane == 0) st_coh(&rslot[B], W(r, r) - (v[0] * w[0] + w[0] * v[0]));
+      publish(&rflag[k], (unsigned)(s + 1), lane);
     }
     {  // D -= v w^T + w v^T (both triangles; v, w vanish past L)
       const double vi = v[ri], wi = w[ri];
