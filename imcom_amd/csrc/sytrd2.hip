@@ -130,26 +130,24 @@ __global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, 
         if (c >= i0 && c < i0 + nr)
           st_coh(&piv_x[(size_t)c * B + tid], P[(c - i0) * LDP + tid]);
       }
-      if (!grid_barrier(sync, base + (unsigned)(c + 1) * G, spin_max, &ok_sh)) {
+      if (!grid_barrier_coh(sync, base + (unsigned)(c + 1) * G, spin_max, &ok_sh)) {
         ok = false;
         break;
       }
-      // (b) totals in a fixed order (the same in every workgroup); plain
-      // loads, all issued before the sum: the slots were written once, with
-      // write-through stores, before the barrier whose acquire dropped this
-      // XCD's stale lines
+      // (b) totals in a fixed order (the same in every workgroup), read
+      // with agent-scope atomics like they were written
       {
         const double* src = part_x + (size_t)c * G * B + jj;
         double t0 = 0.0, t1 = 0.0;
         int h = q;
         for (; h + 4 < G; h += 8) {
-          t0 += src[(size_t)h * B];
-          t1 += src[(size_t)(h + 4) * B];
+          t0 += ld_coh(&src[(size_t)h * B]);
+          t1 += ld_coh(&src[(size_t)(h + 4) * B]);
         }
-        if (h < G) t0 += src[(size_t)h * B];
+        if (h < G) t0 += ld_coh(&src[(size_t)h * B]);
         part[q][jj] = t0 + t1;
       }
-      if (tid < B) pr[tid] = piv_x[(size_t)c * B + tid];
+      if (tid < B) pr[tid] = ld_coh(&piv_x[(size_t)c * B + tid]);
       __syncthreads();
       if (tid < B) sv[tid] = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
       __syncthreads();
