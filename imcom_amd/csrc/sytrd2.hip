@@ -136,15 +136,22 @@ __global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, 
       }
       // (b) totals in a fixed order (the same in every workgroup), read
       // with agent-scope atomics like they were written
+      // (h = q, q + 4, ...), sixteen loads in flight at a time (their
+      // addresses clamped to the last slot, the values past it dropped, so
+      // that no load waits behind a branch)
       {
         const double* src = part_x + (size_t)c * G * B + jj;
         double t0 = 0.0, t1 = 0.0;
-        int h = q;
-        for (; h + 4 < G; h += 8) {
-          t0 += ld_coh(&src[(size_t)h * B]);
-          t1 += ld_coh(&src[(size_t)(h + 4) * B]);
+        for (int h0 = q; h0 < G; h0 += 64) {
+          double x[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) x[i] = ld_coh(&src[(size_t)min(h0 + 4 * i, G - 1) * B]);
+#pragma unroll
+          for (int i = 0; i < 16; i += 2) {
+            t0 += h0 + 4 * i < G ? x[i] : 0.0;
+            t1 += h0 + 4 * i + 4 < G ? x[i + 1] : 0.0;
+          }
         }
-        if (h < G) t0 += ld_coh(&src[(size_t)h * B]);
         part[q][jj] = t0 + t1;
       }
       if (tid < B) pr[tid] = ld_coh(&piv_x[(size_t)c * B + tid]);
@@ -187,12 +194,17 @@ __global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, 
       const double wj = wv[jj];
       int i = max(c + 1 - i0, 0);
       i += (q - i) & 3;
-      if (jj > c) {
-        for (; i + 4 < nr; i += 8) {
-          P[i * LDP + jj] -= vl[i] * wj;
-          P[(i + 4) * LDP + jj] -= vl[i + 4] * wj;
+      if (jj > c) {  // four rows at a time, their reads issued together
+        for (; i + 12 < nr; i += 16) {
+          const double p0 = P[i * LDP + jj], p1 = P[(i + 4) * LDP + jj];
+          const double p2 = P[(i + 8) * LDP + jj], p3 = P[(i + 12) * LDP + jj];
+          const double v0 = vl[i], v1 = vl[i + 4], v2 = vl[i + 8], v3 = vl[i + 12];
+          P[i * LDP + jj] = p0 - v0 * wj;
+          P[(i + 4) * LDP + jj] = p1 - v1 * wj;
+          P[(i + 8) * LDP + jj] = p2 - v2 * wj;
+          P[(i + 12) * LDP + jj] = p3 - v3 * wj;
         }
-        if (i < nr) P[i * LDP + jj] -= vl[i] * wj;
+        for (; i < nr; i += 4) P[i * LDP + jj] -= vl[i] * wj;
       } else {
         for (; i < nr; i += 4) P[i * LDP + jj] = vl[i];
       }
