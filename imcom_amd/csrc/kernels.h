@@ -266,10 +266,10 @@ hipError_t sytrd_tail_status(hipStream_t st, int64_t S, const double* xch, unsig
 // tridiagonal form by bulge chasing, and the back-transforms.
 constexpr int kBand = 64;
 constexpr int kQ2Group = 32;  // reflectors per block of the bulge-chasing back-transform
-// doubles per block image of that back-transform: V (96 rows) and -T (32
-// rows), rows of kQ2Group + 1, padded to a multiple of 512 (two doubles per
+// doubles per block image of that back-transform: V and -V T (96 rows
+// each), rows of kQ2Group + 1, padded to a multiple of 512 (two doubles per
 // thread per load of a 256-thread workgroup)
-constexpr int kQ2Image = 4608;
+constexpr int kQ2Image = 6656;
 struct TwoStageBufs {
   double* U = nullptr;      // n x 3 kBand: [V W V] of the current panel
   double* X = nullptr;      // n x kBand

@@ -501,12 +501,13 @@ constexpr int QL = kQ2Group;       // reflectors per block (consecutive sweeps)
 constexpr int QR = 96;             // block rows (B + QL - 1 = 95, padded)
 constexpr int LV = QL + 1;         // LDS row stride of V
 // A block's image, the apply kernel's LDS copy of it: V (QR x QL, unit
-// diagonal, zero outside the reflectors) in rows of stride LV, then -T
-// (QL x QL upper triangular) in rows of stride LV, zero padding.
+// diagonal, zero outside the reflectors) in rows of stride LV, then
+// Y = -V T (QR x QL: (I - V T V^T) Z = Z + Y (V^T Z)) in rows of stride
+// LV, zero padding.
 constexpr int QIMG = kQ2Image;
-constexpr int QVT = QR * LV;       // offset of -T in an image
+constexpr int QVT = QR * LV;       // offset of Y in an image
 constexpr int QPL = QIMG / 512;    // image loads of two doubles per thread of the apply kernel
-static_assert(QIMG % 512 == 0 && QVT + QL * LV <= QIMG, "Q2 block image layout");
+static_assert(QIMG % 512 == 0 && QVT + QR * LV <= QIMG, "Q2 block image layout");
 typedef double v2d __attribute__((ext_vector_type(2)));
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -576,15 +577,20 @@ __global__ __launch_bounds__(256) void q2_tfactor_kernel(const double* __restric
     }
     __syncthreads();
   }
-  // the block's image: V as the apply kernel's LDS holds it, then -T
+  // the block's image: V as the apply kernel's LDS holds it, then
+  // Y = -V T (T upper triangular: Y(r, j) sums over l <= j)
   double* out = img + ((size_t)blk_off[S] + k) * QIMG;
   for (int idx = tid; idx < QIMG; idx += blockDim.x) {
     double v = 0.0;
     if (idx < QVT) {
       if (idx % LV < QL) v = Vs[idx];
-    } else if (idx < QVT + QL * LV) {
+    } else if (idx < QVT + QR * LV) {
       const int r = (idx - QVT) / LV, c = (idx - QVT) % LV;
-      if (c < QL) v = -Ts[r][c];
+      if (c < QL) {
+        double y = 0.0;
+        for (int l = 0; l <= c; ++l) y += Vs[r * LV + l] * Ts[l][c];
+        v = -y;
+      }
     }
     out[idx] = v;
   }
@@ -592,14 +598,14 @@ __global__ __launch_bounds__(256) void q2_tfactor_kernel(const double* __restric
 
 // Z := Q2 Z on Z stored by rows (Zt = Z^T column-major: Z(i, c) at
 // Zt[c + i n]), 64 columns of Z per workgroup: blocks (S descending, k
-// ascending), each Z(rows, :) += V (-T (V^T Z(rows, :))) on FP64 MFMA.
-// Wavefront w owns the 16-column stripe 16w.. and keeps its 96-row window
-// of Z in registers in the 16x16x4 MFMA's operand layout: lane l holds rows
-// 4s + l/16 (s = 0..23) of column l%16, which is both the B operand of K
-// step s and the accumulator layout of row tile s/4.  So W = V^T Z reads Z
-// from registers, -T W and Z + V W' read W and W' from the accumulators
-// that produced them, and the update of Z accumulates into Z itself; only
-// the block images (V and -T, 34 KB, double-buffered) go through the LDS,
+// ascending), each Z(rows, :) += Y (V^T Z(rows, :)) with Y = -V T on FP64
+// MFMA.  Wavefront w owns the 16-column stripe 16w.. and keeps its 96-row
+// window of Z in registers in the 16x16x4 MFMA's operand layout: lane l
+// holds rows 4s + l/16 (s = 0..23) of column l%16, which is both the B
+// operand of K step s and the accumulator layout of row tile s/4.  So
+// W = V^T Z reads Z from registers, Z + Y W reads W from the accumulators
+// that produced it, and the update of Z accumulates into Z itself; only
+// the block images (V and Y, 52 KB, double-buffered) go through the LDS,
 // with one barrier per block, and no part of Z moves between wavefronts.
 // The last QL - 1 rows of a block's window are the first of the next
 // (k + 1): the window shifts by B rows in each lane's registers and only the
@@ -701,7 +707,7 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
       for (int s = 7; s < QR / 4; ++s) zn[s - 7] = q2_ldz(zr, zl + (s + B / 4) * row4);
     }
     const double* Lv = Ls[buf];
-    const double* Lt = Ls[buf] + QVT;
+    const double* Ly = Ls[buf] + QVT;
     // W = V^T Z: row tile 0 over window rows 0..79, tile 1 over rows 16..95
     // (V's columns 16 c .. 16 c + 15 have entries in rows 16 c .. 16 c + 78)
     v4d w0 = (v4d){0.0, 0.0, 0.0, 0.0}, w1 = w0;
@@ -721,30 +727,18 @@ __global__ __launch_bounds__(256, 1) void q2_apply_kernel(const double* __restri
       __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // one vector memory read
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
     }
-    // W' = -T W (upper triangular)
-    v4d u0 = (v4d){0.0, 0.0, 0.0, 0.0}, u1 = u0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-      u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[lr * LV + 4 * r + lk], w0[r], u0, 0, 0, 0);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      u0 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[lr * LV + 16 + 4 * r + lk], w1[r], u0, 0, 0, 0);
-      u1 = __builtin_amdgcn_mfma_f64_16x16x4f64(Lt[(16 + lr) * LV + 16 + 4 * r + lk], w1[r], u1, 0,
-                                                0, 0);
-    }
-    // Z += V W' (V's rows 0-15 have no entries past column 15, rows 80-95
-    // none before column 17), tile by tile; rows 0..63 are final after
-    // their tile (no later block of the group touches them) and go back to
-    // Z at once, under the next tiles' products (rows past the block's
-    // rows, nout on, are not stored)
+    // Z += Y W (Y's rows 80-95 have no entries before column 17), tile by
+    // tile; rows 0..63 are final after their tile (no later block of the
+    // group touches them) and go back to Z at once, under the next tiles'
+    // products (rows past the block's rows, nout on, are not stored)
     const int nout = (last ? R : min(B, R)) - lk;
 #pragma unroll
     for (int t = 0; t < QR / 16; ++t) {
 #pragma unroll
       for (int q = 0; q < QL / 4; ++q) {
-        if ((t == 0 && q >= 4) || (t == QR / 16 - 1 && q < 4)) continue;
-        const double bw = q < 4 ? u0[q] : u1[q - 4];
-        zt[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Lv[(16 * t + lr) * LV + 4 * q + lk], bw, zt[t],
+        if (t == QR / 16 - 1 && q < 4) continue;
+        const double bw = q < 4 ? w0[q] : w1[q - 4];
+        zt[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ly[(16 * t + lr) * LV + 4 * q + lk], bw, zt[t],
                                                      0, 0, 0);
       }
       if (t < B / 16) {
