@@ -334,24 +334,33 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int ks
 
 // K chunks of at least 512, about 1024 blocks in all; 128x128 tiles unless
 // that leaves fewer than 256 blocks
-static void splitk_shape(int M, int N, int K, int* tile, int* ksplit) {
+static void splitk_shape(int M, int N, int K, int* bm, int* bn, int* ksplit) {
   const int maxs = std::max(1, K / 512);
   const int64_t t128 = (int64_t)((M + 127) / 128) * ((N + 127) / 128);
   int s = (int)std::min<int64_t>(maxs, std::max<int64_t>(1, (1024 + t128 - 1) / t128));
   // a 128-wide tile over a dimension of at most 64 would compute half zeros
   if (t128 * s >= 256 && M > 64 && N > 64) {
-    *tile = 128;
+    *bm = *bn = 128;
     *ksplit = s;
     return;
   }
+  // one column of 64: 128 x 64 tiles (a wavefront's 4 x 2 MFMA tiles take
+  // eight products per six operand reads, against four per four on 64 x 64)
+  if (N <= 64 && M > 64) {
+    const int64_t t = (M + 127) / 128;
+    *bm = 128;
+    *bn = 64;
+    *ksplit = (int)std::min<int64_t>(maxs, std::max<int64_t>(1, (1024 + t - 1) / t));
+    return;
+  }
   const int64_t t64 = (int64_t)((M + 63) / 64) * ((N + 63) / 64);
-  *tile = 64;
+  *bm = *bn = 64;
   *ksplit = (int)std::min<int64_t>(maxs, std::max<int64_t>(1, (1024 + t64 - 1) / t64));
 }
 
 size_t dgemm_splitk_work(int M, int N, int K) {
-  int tile, ks;
-  splitk_shape(M, N, K, &tile, &ks);
+  int bm, bn, ks;
+  splitk_shape(M, N, K, &bm, &bn, &ks);
   return (size_t)ks * M * N * sizeof(double);
 }
 
@@ -359,16 +368,19 @@ hipError_t dgemm_splitk(hipStream_t st, bool ta, bool tb, int M, int N, int K, d
                         const double* A, int64_t lda, const double* B, int64_t ldb, double* C,
                         int64_t ldc, double* work) {
   if (M <= 0 || N <= 0) return hipSuccess;
-  int tile, ks;
-  splitk_shape(M, N, K, &tile, &ks);
+  int bm, bn, ks;
+  splitk_shape(M, N, K, &bm, &bn, &ks);
   const int kchunk = (K + ks - 1) / ks;
   ks = (K + kchunk - 1) / kchunk;  // no empty chunk
   GemmDesc g{M, N, K, 1.0, 0.0, A, lda, B, ldb, work, M};
   g.kchunk = kchunk;
   g.c_split_stride = (int64_t)M * N;
-  if (tile == 128) {
+  if (bm == 128 && bn == 128) {
     dim3 grid((unsigned)(((M + 127) / 128) * ((N + 127) / 128)), (unsigned)ks);
     IMCOM_HIP_TRY((launch_sz<128, 128, false>(st, ta, tb, grid, g, nullptr, nullptr, 0)));
+  } else if (bm == 128) {
+    dim3 grid((unsigned)(((M + 127) / 128) * ((N + 63) / 64)), (unsigned)ks);
+    IMCOM_HIP_TRY((launch_sz<128, 64, false>(st, ta, tb, grid, g, nullptr, nullptr, 0)));
   } else {
     dim3 grid((unsigned)(((M + 63) / 64) * ((N + 63) / 64)), (unsigned)ks);
     IMCOM_HIP_TRY((launch_sz<64, 64, false>(st, ta, tb, grid, g, nullptr, nullptr, 0)));
