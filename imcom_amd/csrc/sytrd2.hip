@@ -246,6 +246,11 @@ __global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, 
     }
 }
 
+__global__ void msg_zero_kernel(unsigned* w, size_t nw) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nw) w[i] = 0u;
+}
+
 __global__ void band_zero_kernel(unsigned* w, int nw) {
   for (int i = threadIdx.x; i < nw; i += blockDim.x)
     __hip_atomic_store(&w[i], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -268,9 +273,9 @@ void panel_shape(int m, int* G, int* rpw) {
 // the entries of the new row left of column r are zero (annihilated
 // columns).  Task (s, k) needs only the reflector of (s, k-1), sent by
 // workgroup k-1.  Messages go through rings of 4 slots per workgroup (a
-// producer is at most one sweep ahead of its consumers); a message's
-// entries are written and read with agent-scope atomics and its counter is
-// set once those stores have completed.  Workgroup 0's top rows are the
+// producer is at most one sweep ahead of its consumers); each entry of a
+// message carries the sweep's tag, and its consumer polls it until the tag
+// matches.  Workgroup 0's top rows are the
 // tridiagonal matrix: d and e come out as the window slides.
 constexpr int WLD = 65;  // LDS stride of a window column slot (64 row slots + 1)
 constexpr int BT = 512;  // threads per bulge-chasing workgroup: row ri = tid / 8, eighth tid % 8
@@ -284,22 +289,43 @@ __device__ __forceinline__ double oct_sum_dpp(double v) {
   return v;
 }
 
-// A message's flag: set by lane 0 once the wavefront's coherent stores of
-// the message have completed (relaxed: the message's entries are written
-// and read with agent-scope atomics, which are coherent across the XCDs
-// without the L2 write-back and invalidation of a release/acquire pair).
-__device__ __forceinline__ void publish(unsigned* flag, unsigned value, int lane) {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0);
-  asm volatile("" ::: "memory");
-  if (lane == 0) __hip_atomic_store(flag, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// A message entry: two agent-scope atomic words, each holding the sweep
+// tag in its upper half and one half of the value in its lower half (both
+// words single-copy atomic, so an entry whose two tags match is whole).  The
+// consumer polls its entries until their tags match: no flag, no wait for
+// the producer's stores to complete and no second read.
+__device__ __forceinline__ void msg_put(unsigned long long* e, unsigned tag, double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long t = (unsigned long long)tag << 32;
+  __hip_atomic_store(e, t | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(e + 1, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits for the entry with the given tag; false after an abort or spin_max
+// polls (then raises the abort flag).
+__device__ __forceinline__ bool msg_get(const unsigned long long* e, unsigned tag,
+                                        unsigned* abort_flag, unsigned spin_max, double* v) {
+  for (unsigned it = 0;; ++it) {
+    const unsigned long long lo = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long hi =
+        __hip_atomic_load(e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag) {
+      *v = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+      return true;
+    }
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+        it >= spin_max) {
+      __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
 }
 
 __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__ A, int64_t n,
                                                       double* __restrict__ V2,
                                                       double* __restrict__ d,
                                                       double* __restrict__ e,
-                                                      double* __restrict__ msg,
+                                                      unsigned long long* __restrict__ msg,
                                                       unsigned* __restrict__ flags,
                                                       unsigned spin_max) {
   __shared__ double Wl[2 * B * WLD];
@@ -309,11 +335,10 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
   const int ri = tid >> 3, q8 = tid & 7;
   const int k = blockIdx.x, K = gridDim.x;
   const int64_t s_last = min<int64_t>(n - 3, n - 2 - (int64_t)B * k);
-  unsigned* vflag = flags;
-  unsigned* rflag = flags + K;
   unsigned* abort_flag = flags + 2 * K;
-  double* vmsg = msg;                             // [K][4][B + 1]: v, tau
-  double* rmsg = msg + (size_t)K * 4 * (B + 1);   // [K][4][B + 1]: row r, columns r-64..r
+  constexpr int ME = 2 * (B + 1);                  // words per message: B + 1 entries
+  unsigned long long* vmsg = msg;                  // [K][4][ME]: v, tau
+  unsigned long long* rmsg = msg + (size_t)K * 4 * ME;  // [K][4][ME]: row r, columns r-64..r
   auto W = [&](int64_t i, int64_t j) -> double& {
     return Wl[(int)(j & (2 * B - 1)) * WLD + (int)(i & (B - 1))];
   };
@@ -329,6 +354,7 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
       W(i, j) = val;
     }
     if (k == 0 && tid == 0) d[0] = A[0];
+    if (tid == 0) ok_sh = 1;
   }
   __syncthreads();
   for (int64_t s = 0; s <= s_last; ++s) {
@@ -342,42 +368,36 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
     }
     const bool need_row = s > 0 && rn < n;
     if (need_row || k > 0) {
-      if (tid == 0) {
-        int okv = 1;
-        unsigned it = 0;
-        while (okv && ((need_row && __hip_atomic_load(&rflag[k + 1], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT) < (unsigned)s) ||
-                       (k > 0 && __hip_atomic_load(&vflag[k - 1], __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT) < (unsigned)(s + 1)))) {
-          if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-              ++it > spin_max) {
-            __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            okv = 0;
-          } else {
-            __builtin_amdgcn_s_sleep(1);
-          }
+      // the row of (s-1, k+1) and the reflector of (s, k-1), each entry
+      // polled by the thread that stores it
+      if (need_row && tid <= B) {  // row rn, columns rn-64 .. rn (mirrored into column rn)
+        double val;
+        if (msg_get(&rmsg[((size_t)(k + 1) * 4 + ((s - 1) & 3)) * ME + 2 * tid], (unsigned)s,
+                    abort_flag, spin_max, &val)) {
+          const int64_t c = rn - B + tid;
+          W(rn, c) = val;
+          if (c >= r && c < rn) W(c, rn) = val;
+        } else {
+          ok_sh = 0;
         }
-        ok_sh = okv;
+      }
+      if (k > 0 && tid >= 2 * B && tid <= 3 * B) {
+        double val;
+        if (msg_get(&vmsg[((size_t)(k - 1) * 4 + (s & 3)) * ME + 2 * (tid - 2 * B)],
+                    (unsigned)(s + 1), abort_flag, spin_max, &val)) {
+          if (tid < 3 * B)
+            vp[tid - 2 * B] = val;
+          else
+            sc[0] = val;  // tau of (s, k-1)
+        } else {
+          ok_sh = 0;
+        }
       }
       __syncthreads();
       if (!ok_sh) return;
-      if (need_row && tid <= B) {  // row rn, columns rn-64 .. rn (mirrored into column rn)
-        const double val = ld_coh(&rmsg[((size_t)(k + 1) * 4 + ((s - 1) & 3)) * (B + 1) + tid]);
-        const int64_t c = rn - B + tid;
-        W(rn, c) = val;
-        if (c >= r && c < rn) W(c, rn) = val;
-      }
-      if (k > 0 && tid >= 2 * B && tid <= 3 * B) {
-        const double val = ld_coh(&vmsg[((size_t)(k - 1) * 4 + (s & 3)) * (B + 1) + (tid - 2 * B)]);
-        if (tid < 3 * B)
-          vp[tid - 2 * B] = val;
-        else
-          sc[0] = val;  // tau of (s, k-1)
-      }
-      __syncthreads();
     }
     const bool send_v = k + 1 < K;
-    double* vslot = vmsg + ((size_t)k * 4 + (s & 3)) * (B + 1);
+    unsigned long long* vslot = vmsg + ((size_t)k * 4 + (s & 3)) * ME;
     if (k == 0) {  // the reflector of column s = r-1; published at once
       if (wv == 0) {
         const double x = lane < L ? W(r + lane, r - 1) : 0.0;
@@ -397,9 +417,8 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
           sc[1] = bt;
         }
         if (send_v) {
-          st_coh(&vslot[lane], vl);
-          if (lane == 0) st_coh(&vslot[B], t);
-          publish(&vflag[k], (unsigned)(s + 1), lane);
+          msg_put(&vslot[2 * lane], (unsigned)(s + 1), vl);
+          if (lane == 0) msg_put(&vslot[2 * B], (unsigned)(s + 1), t);
         }
       }
       __syncthreads();
@@ -431,9 +450,8 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
           sc[1] = bt;
         }
         if (send_v) {
-          st_coh(&vslot[lane], vl);
-          if (lane == 0) st_coh(&vslot[B], t);
-          publish(&vflag[k], (unsigned)(s + 1), lane);
+          msg_put(&vslot[2 * lane], (unsigned)(s + 1), vl);
+          if (lane == 0) msg_put(&vslot[2 * B], (unsigned)(s + 1), t);
         }
       }
       __syncthreads();
@@ -471,10 +489,9 @@ __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__
     // row r is final once D(0, 0) is: send it to workgroup k-1 before the
     // rest of the D update
     if (k > 0 && wv == 0) {
-      double* rslot = rmsg + ((size_t)k * 4 + (s & 3)) * (B + 1);
-      st_coh(&rslot[lane], W(r, r - B + lane));
-      if (lane == 0) st_coh(&rslot[B], W(r, r) - (v[0] * w[0] + w[0] * v[0]));
-      publish(&rflag[k], (unsigned)(s + 1), lane);
+      unsigned long long* rslot = rmsg + ((size_t)k * 4 + (s & 3)) * ME;
+      msg_put(&rslot[2 * lane], (unsigned)(s + 1), W(r, r - B + lane));
+      if (lane == 0) msg_put(&rslot[2 * B], (unsigned)(s + 1), W(r, r) - (v[0] * w[0] + w[0] * v[0]));
     }
     {  // D -= v w^T + w v^T (both triangles; v, w vanish past L)
       const double vi = v[ri], wi = w[ri];
@@ -882,7 +899,7 @@ hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs&
 
 size_t bulge_msg_doubles(int64_t n) {
   const int64_t K = n >= 3 ? (n - 2) / B + 1 : 1;
-  return (size_t)2 * K * 4 * (B + 1);
+  return (size_t)2 * K * 4 * 2 * (B + 1);  // 64-bit words: two per entry
 }
 
 size_t bulge_flag_words(int64_t n) {
@@ -903,10 +920,15 @@ hipError_t bulge_chase(hipStream_t st, double* A, int64_t n, double* V2, double*
   const int nw = (int)bulge_flag_words(n);
   hipLaunchKernelGGL(band_zero_kernel, dim3(1), dim3(256), 0, st, flags, nw);
   IMCOM_HIP_TRY(hipGetLastError());
+  // message tags from an earlier chase must not match: clear the rings
+  const size_t mw = bulge_msg_doubles(n) * 2;
+  hipLaunchKernelGGL(msg_zero_kernel, dim3((unsigned)((mw + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<unsigned*>(msg), mw);
+  IMCOM_HIP_TRY(hipGetLastError());
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (prof) IMCOM_HIP_TRY(prof->events(PS_BULGE, 2, &e0, &e1));
-  hipExtLaunchKernelGGL(bulge_kernel, dim3(K), dim3(BT), 0, st, e0, e1, 0, A, n, V2, d, e, msg,
-                        flags, kSpinMax);
+  hipExtLaunchKernelGGL(bulge_kernel, dim3(K), dim3(BT), 0, st, e0, e1, 0, A, n, V2, d, e,
+                        reinterpret_cast<unsigned long long*>(msg), flags, kSpinMax);
   return hipGetLastError();
 }
 
