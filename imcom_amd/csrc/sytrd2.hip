@@ -72,11 +72,84 @@ constexpr unsigned kSpinMax = 1u << 22;
 
 // ---------------------------------------------------------------- stage 1
 
+// A message entry: two agent-scope atomic words, each holding a tag (the
+// sweep, or the column) in its upper half and one half of the value in its
+// lower half (both
+// words single-copy atomic, so an entry whose two tags match is whole).  The
+// consumer polls its entries until their tags match: no flag, no wait for
+// the producer's stores to complete and no second read.
+__device__ __forceinline__ void msg_put(unsigned long long* e, unsigned tag, double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long t = (unsigned long long)tag << 32;
+  __hip_atomic_store(e, t | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(e + 1, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits for the entry with the given tag; false after an abort or spin_max
+// polls (then raises the abort flag).
+__device__ __forceinline__ bool msg_get(const unsigned long long* e, unsigned tag,
+                                        unsigned* abort_flag, unsigned spin_max, double* v) {
+  for (unsigned it = 0;; ++it) {
+    const unsigned long long lo = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long hi =
+        __hip_atomic_load(e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag) {
+      *v = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
+      return true;
+    }
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+        it >= spin_max) {
+      __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// Entries h0 + 4i (i < 16, those below G) of a row of entries B apart,
+// loaded together and re-polled until their tags match; x[i] = 0 past G.
+__device__ __forceinline__ bool msg_get16(const unsigned long long* src, int h0, int G,
+                                          unsigned tag, unsigned* abort_flag, unsigned spin_max,
+                                          double* x) {
+  unsigned long long lo[16], hi[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const unsigned long long* e = src + (size_t)min(h0 + 4 * i, G - 1) * (2 * B);
+    lo[i] = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    hi[i] = __hip_atomic_load(e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  unsigned pend = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if (h0 + 4 * i < G && ((unsigned)(lo[i] >> 32) != tag || (unsigned)(hi[i] >> 32) != tag))
+      pend |= 1u << i;
+  for (unsigned it = 0; pend != 0u; ++it) {
+    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+        it >= spin_max) {
+      __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return false;
+    }
+    __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      if ((pend >> i) & 1u) {
+        const unsigned long long* e = src + (size_t)(h0 + 4 * i) * (2 * B);
+        lo[i] = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        hi[i] = __hip_atomic_load(e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((unsigned)(lo[i] >> 32) == tag && (unsigned)(hi[i] >> 32) == tag) pend &= ~(1u << i);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    x[i] = h0 + 4 * i < G ? __longlong_as_double((long long)((hi[i] << 32) | (lo[i] & 0xffffffffull)))
+                          : 0.0;
+  return true;
+}
+
 __global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, int64_t n,
                                                         int64_t k0, int bc, int nref, int rpw,
                                                         int G, double* __restrict__ U,
                                                         int64_t ldu, double* __restrict__ T1,
-                                                        double* __restrict__ xch,
+                                                        unsigned long long* __restrict__ xch,
                                                         unsigned* __restrict__ sync,
                                                         unsigned base, unsigned spin_max) {
   extern __shared__ double P[];  // owned rows, row-major with stride LDP
@@ -91,12 +164,14 @@ __global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, 
   const int m = (int)(n - r0);
   const int i0 = g * rpw;
   const int nr = min(rpw, m - i0);
-  double* part_x = xch;                         // [nref][G][B], written once
-  double* piv_x = xch + (size_t)nref * G * B;  // [nref][B]
+  // tagged entries (two words each), each written once per band reduction
+  unsigned long long* part_x = xch;                             // [nref][G][B]
+  unsigned long long* piv_x = xch + (size_t)nref * G * B * 2;  // [nref][B]
 
   for (int j = 0; j < B; ++j)
     for (int i = tid; i < nr; i += PT)
       P[i * LDP + j] = j < bc ? A[(r0 + i0 + i) + (k0 + j) * n] : 0.0;
+  if (tid == 0) ok_sh = 1;
   __syncthreads();
 
   bool ok = true;
@@ -124,38 +199,43 @@ __global__ __launch_bounds__(PT) void band_panel_kernel(double* __restrict__ A, 
       }
       __syncthreads();
     } else {
+      double tp = 0.0;
+      if (tid < B) tp = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
+      __syncthreads();  // part is rewritten below
+      const unsigned tag = (unsigned)(k0 + c + 1);  // this column's, unique in the reduction
       if (tid < B) {
-        const double t = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
-        st_coh(&part_x[((size_t)c * G + g) * B + tid], t);
-        if (c >= i0 && c < i0 + nr)
-          st_coh(&piv_x[(size_t)c * B + tid], P[(c - i0) * LDP + tid]);
+        msg_put(&part_x[(((size_t)c * G + g) * B + tid) * 2], tag, tp);
+        if (c >= i0 && c < i0 + nr) msg_put(&piv_x[((size_t)c * B + tid) * 2], tag, P[(c - i0) * LDP + tid]);
       }
-      if (!grid_barrier_coh(sync, base + (unsigned)(c + 1) * G, spin_max, &ok_sh)) {
-        ok = false;
-        break;
-      }
-      // (b) totals in a fixed order (the same in every workgroup), read
-      // with agent-scope atomics like they were written
-      // (h = q, q + 4, ...), sixteen loads in flight at a time (their
-      // addresses clamped to the last slot, the values past it dropped, so
-      // that no load waits behind a branch)
+      // (b) totals in a fixed order (the same in every workgroup), each
+      // workgroup's entry polled until its tag is this column's, sixteen in
+      // flight at a time (h = q, q + 4, ...)
+      bool okt = true;
       {
-        const double* src = part_x + (size_t)c * G * B + jj;
+        const unsigned long long* src = part_x + ((size_t)c * G * B + jj) * 2;
         double t0 = 0.0, t1 = 0.0;
-        for (int h0 = q; h0 < G; h0 += 64) {
+        for (int h0 = q; h0 < G && okt; h0 += 64) {
           double x[16];
-#pragma unroll
-          for (int i = 0; i < 16; ++i) x[i] = ld_coh(&src[(size_t)min(h0 + 4 * i, G - 1) * B]);
+          okt = msg_get16(src, h0, G, tag, &sync[1], spin_max, x);
 #pragma unroll
           for (int i = 0; i < 16; i += 2) {
-            t0 += h0 + 4 * i < G ? x[i] : 0.0;
-            t1 += h0 + 4 * i + 4 < G ? x[i + 1] : 0.0;
+            t0 += x[i];
+            t1 += x[i + 1];
           }
         }
         part[q][jj] = t0 + t1;
       }
-      if (tid < B) pr[tid] = ld_coh(&piv_x[(size_t)c * B + tid]);
+      if (tid < B && okt) {
+        double v;
+        okt = msg_get(&piv_x[((size_t)c * B + tid) * 2], tag, &sync[1], spin_max, &v);
+        pr[tid] = v;
+      }
+      if (!okt) ok_sh = 0;
       __syncthreads();
+      if (!ok_sh) {
+        ok = false;
+        break;
+      }
       if (tid < B) sv[tid] = (part[0][tid] + part[1][tid]) + (part[2][tid] + part[3][tid]);
       __syncthreads();
     }
@@ -287,38 +367,6 @@ __device__ __forceinline__ double oct_sum_dpp(double v) {
   v += dpp_d<0x4e>(v);   // quad_perm [2, 3, 0, 1]
   v += dpp_d<0x141>(v);  // row_half_mirror: the other quad of the eight
   return v;
-}
-
-// A message entry: two agent-scope atomic words, each holding the sweep
-// tag in its upper half and one half of the value in its lower half (both
-// words single-copy atomic, so an entry whose two tags match is whole).  The
-// consumer polls its entries until their tags match: no flag, no wait for
-// the producer's stores to complete and no second read.
-__device__ __forceinline__ void msg_put(unsigned long long* e, unsigned tag, double v) {
-  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-  const unsigned long long t = (unsigned long long)tag << 32;
-  __hip_atomic_store(e, t | (b & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(e + 1, t | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// Waits for the entry with the given tag; false after an abort or spin_max
-// polls (then raises the abort flag).
-__device__ __forceinline__ bool msg_get(const unsigned long long* e, unsigned tag,
-                                        unsigned* abort_flag, unsigned spin_max, double* v) {
-  for (unsigned it = 0;; ++it) {
-    const unsigned long long lo = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long hi =
-        __hip_atomic_load(e + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((unsigned)(lo >> 32) == tag && (unsigned)(hi >> 32) == tag) {
-      *v = __longlong_as_double((long long)((hi << 32) | (lo & 0xffffffffull)));
-      return true;
-    }
-    if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-        it >= spin_max) {
-      __hip_atomic_store(abort_flag, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      return false;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
 }
 
 __global__ __launch_bounds__(BT, 1) void bulge_kernel(const double* __restrict__ A, int64_t n,
@@ -826,7 +874,7 @@ __global__ __launch_bounds__(256) void transpose_kernel(int64_t n, const double*
 size_t band_xch_doubles(int64_t n) {
   int G, rpw;
   panel_shape((int)std::max<int64_t>(n - B, 1), &G, &rpw);
-  return (size_t)B * G * B + (size_t)B * B;
+  return ((size_t)B * G * B + (size_t)B * B) * 2;  // tagged entries: two words each
 }
 
 size_t band_kwork_doubles(int64_t n) {
@@ -856,6 +904,11 @@ int band_panels(int64_t n) {
 hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs& w) {
   hipLaunchKernelGGL(band_zero_kernel, dim3(1), dim3(64), 0, st, w.sync, 2);
   IMCOM_HIP_TRY(hipGetLastError());
+  // tags from an earlier reduction must not match: clear the exchange
+  const size_t xw = band_xch_doubles(n) * 2;
+  hipLaunchKernelGGL(msg_zero_kernel, dim3((unsigned)((xw + 255) / 256)), dim3(256), 0, st,
+                     reinterpret_cast<unsigned*>(w.xch), xw);
+  IMCOM_HIP_TRY(hipGetLastError());
   static bool attr_set = false;
   if (!attr_set) {
     IMCOM_HIP_TRY(hipFuncSetAttribute((const void*)band_panel_kernel,
@@ -876,7 +929,8 @@ hipError_t band_reduce(hipStream_t st, double* A, int64_t n, const TwoStageBufs&
     panel_shape(m, &G, &rpw);
     double* T1p = w.T1 + (size_t)p * B * B;
     hipLaunchKernelGGL(band_panel_kernel, dim3(G), dim3(PT), (size_t)rpw * LDP * 8, st, A, n, k0,
-                       bc, nref, rpw, G, w.U, ldu, T1p, w.xch, w.sync, base, kSpinMax);
+                       bc, nref, rpw, G, w.U, ldu, T1p,
+                       reinterpret_cast<unsigned long long*>(w.xch), w.sync, base, kSpinMax);
     IMCOM_HIP_TRY(hipGetLastError());
     base += (unsigned)nref * G;
     double* A22 = A + r0 + r0 * n;
