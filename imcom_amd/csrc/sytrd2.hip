@@ -14,11 +14,11 @@
 // the band):
 //   * one persistent launch QR-factors the panel A(r0:n, k0:k0+64): its rows
 //     are split among G workgroups (up to 224 rows each, held in LDS), and
-//     each column c takes one grid barrier: every workgroup publishes the
-//     sums P(c+1:, c)^T P(c+1:, :) over its rows, and the pivot row's owner
-//     publishes row c; after the barrier every workgroup forms the same
-//     reflector (dlarfg conventions) and w = tau v^T P(c:, :) from these
-//     totals and updates its rows.  Workgroup 0 also builds the panel's
+//     per column c every workgroup publishes the sums
+//     P(c+1:, c)^T P(c+1:, :) over its rows, and the pivot row's owner
+//     publishes row c, as entries tagged with the column; once it holds all
+//     of them every workgroup forms the same reflector (dlarfg conventions)
+//     and w = tau v^T P(c:, :) from these totals and updates its rows.  Workgroup 0 also builds the panel's
 //     triangular factor T (LAPACK dlarft, forward, columnwise) from the same
 //     sums: V(:, 0:c)^T v_c needs only the pivot row and the published sums.
 //   * the panel's R goes to the band (zeros below it), V^T to the upper

@@ -74,68 +74,3 @@ __device__ __forceinline__ double quad_sum_dpp(double v) {
 }
 
 }  // namespace imcom
-
-namespace imcom {
-
-// Grid barrier for persistent kernels whose workgroups are all resident:
-// arrival count sync[0] (monotonic within a launch), abort flag sync[1].
-// Thread 0 arrives with an agent-scope release (the L2's dirty lines are
-// written back first, so the workgroup's stores are visible to every XCD)
-// and leaves with an agent-scope acquire (stale L2 lines are dropped); the
-// workgroup barriers around it order the other threads' accesses.  A wait
-// longer than spin_max polls raises the abort flag, which ends every other
-// workgroup's wait too; returns false then.
-__device__ __forceinline__ bool grid_barrier(unsigned* sync, unsigned target, unsigned spin_max,
-                                             int* ok_sh) {
-  __builtin_amdgcn_s_waitcnt(0);  // every wavefront's stores have completed
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned it = 0;
-    while (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-          ++it > spin_max) {
-        __hip_atomic_store(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    *ok_sh = ok;
-  }
-  __syncthreads();
-  return *ok_sh != 0;
-}
-
-// Grid barrier for data exchanged with agent-scope atomics only (st_coh,
-// ld_coh, which are coherent across the XCDs by themselves): the stores
-// completed before the arrival, relaxed arrival and wait, no L2 write-back
-// or invalidation; same counter, abort flag and result as grid_barrier.
-__device__ __forceinline__ bool grid_barrier_coh(unsigned* sync, unsigned target,
-                                                 unsigned spin_max, int* ok_sh) {
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_waitcnt(0);  // every wavefront's stores have completed
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int ok = 1;
-    __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned it = 0;
-    while (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&sync[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
-          ++it > spin_max) {
-        __hip_atomic_store(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-    *ok_sh = ok;
-  }
-  __syncthreads();
-  asm volatile("" ::: "memory");
-  return *ok_sh != 0;
-}
-
-}  // namespace imcom
